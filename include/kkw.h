@@ -1,0 +1,213 @@
+/*
+ * kkw.h -- C ABI of the MI355X random-walk engine (libkkw.so).
+ *
+ * The reference (KnightKing) exposes no C ABI: its boundary is the C++
+ * header-only templates WalkEngine / WalkConfig / WalkerConfig /
+ * TransitionConfig / SecondOrderTransitionConfig and the app entry functions
+ * node2vec(), deepwalk(), ppr(), metapath().  Each entry point below names the
+ * reference interface it replaces (paths relative to the KnightKing tree);
+ * the headers in include/knightking/ rebuild the C++ templates on top of these calls.
+ *
+ * Conventions: every function returns KKW_OK (0) or a negative KKW_E* code
+ * and, on failure, leaves a message for kkw_last_error() (per thread).  Host
+ * arrays passed in are copied; the caller keeps ownership.  Buffers returned in
+ * kkw_paths are owned by the engine until kkw_paths_free().  One engine drives
+ * one GPU on its own HIP stream; calls on one engine from several threads are
+ * serialised internally, except kkw_submit/kkw_poll/kkw_wait, which are made
+ * for concurrent callers.
+ */
+#ifndef KKW_H
+#define KKW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KKW_ABI_VERSION 1
+
+enum kkw_status {
+    KKW_OK = 0,
+    KKW_E_INVALID = -1,   /* bad argument or descriptor                       */
+    KKW_E_NOGRAPH = -2,   /* no graph loaded / alias tables not prepared       */
+    KKW_E_NOMEM = -3,     /* device memory (or the admission budget) exhausted */
+    KKW_E_DEVICE = -4,    /* HIP runtime error                                 */
+    KKW_E_BUSY = -5,      /* serving queue full (back-pressure)                */
+    KKW_E_CANCELLED = -6, /* request cancelled                                 */
+    KKW_E_TIMEOUT = -7,   /* kkw_wait timed out                                */
+    KKW_E_IO = -8         /* file could not be read or written                 */
+};
+
+/* Walk kinds: src/apps/deepwalk.hpp:30, ppr.hpp:30, node2vec.hpp:108,
+ * metapath.hpp:70. */
+enum kkw_kind { KKW_DEEPWALK = 0, KKW_PPR = 1, KKW_NODE2VEC = 2, KKW_METAPATH = 3 };
+
+/* Start-vertex distributions: get_equal_dist_func (include/walk.hpp:403-411,
+ * walker w starts at w % |V|), a caller list indexed by w % n (ppr.hpp:38-50),
+ * get_uniform_dist_func (walk.hpp:413-421). */
+enum kkw_start { KKW_START_EQUAL = 0, KKW_START_LIST = 1, KKW_START_UNIFORM = 2 };
+
+/* kkw_run modes: one plain launch per job, or all jobs in one persistent
+ * work-queue launch. */
+enum kkw_mode { KKW_MODE_PLAIN = 0, KKW_MODE_PERSISTENT = 1 };
+
+/* where kkw_run leaves the paths */
+enum kkw_out { KKW_OUT_HOST = 0, KKW_OUT_DEVICE = 1 };
+
+typedef struct kkw_engine kkw_engine;
+
+typedef struct kkw_engine_opts {
+    int32_t device;          /* HIP device ordinal                                 */
+    int32_t verbose;         /* 1: per-phase timings on stderr                     */
+    uint64_t hbm_budget;     /* admission budget for serving, bytes (0 = free HBM) */
+} kkw_engine_opts;
+
+/* One walk job: the parameters of node2vec()/deepwalk()/ppr()/metapath() and of
+ * WalkerConfig(walker_num, ..., init_dist) (include/walk.hpp:147-193). */
+typedef struct kkw_job {
+    int32_t kind;            /* enum kkw_kind                                        */
+    int32_t use_weights;     /* static component = edge weight (alias tables);
+                                0 = uniform (get_trivial_static_comp, static_comp.hpp:29-42) */
+    uint32_t walker_begin;   /* first walker id (epochs of WalkConfig::rate)         */
+    uint32_t walker_num;
+    uint32_t walk_length;    /* L; for PPR the step cap (paths stop there, flagged)   */
+    float terminate_prob;    /* PPR                                                  */
+    float p, q;              /* node2vec                                             */
+    uint64_t seed;           /* Philox key of this job's stream                      */
+    int32_t start_mode;      /* enum kkw_start                                        */
+    uint32_t n_starts;
+    const uint32_t *starts;  /* host array, KKW_START_LIST                           */
+    uint32_t n_schemes;      /* metapath                                             */
+    const uint32_t *scheme_len;  /* [n_schemes], 1..8 states each                    */
+    const uint8_t *scheme_mask;  /* [n_schemes * 8]: allowed edge types per state    */
+} kkw_job;
+
+/* Paths of one job: PathSet (include/path.hpp:46-84) with one segment in
+ * walker-id order.  Row i holds lengths[i] vertices (steps + 1) of walker
+ * walker_begin + i; bit 31 of lengths[i] flags a path stopped by the PPR cap. */
+typedef struct kkw_paths {
+    uint32_t walker_begin;
+    uint32_t walker_num;
+    uint32_t stride;         /* entries per row                                      */
+    uint32_t on_device;      /* 1: vertices/lengths are device pointers              */
+    uint32_t *vertices;      /* [walker_num * stride]                                */
+    uint32_t *lengths;       /* [walker_num]                                         */
+    uint64_t steps;          /* accepted transitions (sum of lengths - 1)            */
+    uint64_t truncated;      /* walkers stopped by the cap                           */
+    void *priv;
+} kkw_paths;
+
+typedef struct kkw_timing {
+    double kernel_ms;        /* walk kernel(s), HIP events on the engine stream      */
+    double total_ms;         /* host wall time of the last kkw_run / batch launch     */
+    double alias_ms;         /* last kkw_prepare                                     */
+    double graph_ms;         /* last graph build                                     */
+    uint64_t steps, trials, searches, search_probes, row_loads;  /* KKW_FLAG_STATS   */
+} kkw_timing;
+
+enum kkw_flags { KKW_FLAG_STATS = 1 };
+
+const char *kkw_last_error(void);
+int kkw_abi_version(void);
+
+/* Engine lifetime.  Replaces WalkEngine() / GraphEngine() (include/walk.hpp:378-389,
+ * include/graph.hpp:236-257) and set_concurrency (walk.hpp:391-396, a no-op here). */
+int kkw_engine_create(const kkw_engine_opts *opts, kkw_engine **out);
+int kkw_engine_destroy(kkw_engine *eng);
+
+/* Graph loading.  Replaces GraphEngine::load_graph (include/graph.hpp:408-530):
+ * edges are Edge<real_t> records {u32 src, u32 dst, f32 w} (12 B, weighted=1) or
+ * Edge<EmptyData> {u32 src, u32 dst} (8 B), include/type.hpp:43-85; mirrored when
+ * make_undirected (graph.hpp:430-443); CSR keeps file order per source
+ * (graph.hpp:309-330); sort_adj sorts each list by neighbour id, stably, as
+ * node2vec() does before walking (src/apps/node2vec.hpp:122-125). */
+int kkw_graph_from_edges(kkw_engine *eng, uint32_t v_num, const void *edges,
+                         uint64_t n_edges, int weighted, int make_undirected, int sort_adj);
+/* Binary edge file (read_graph, include/storage.hpp:35-55) or text edge list
+ * (read_edgelist, storage.hpp:124-157: "src dst[ w]" per line). */
+int kkw_graph_load_file(kkw_engine *eng, uint32_t v_num, const char *path, int text,
+                        int weighted, int make_undirected, int sort_adj);
+/* A CSR the caller already holds (row_ptr[v_num+1], col[e], optional w[e]). */
+int kkw_graph_from_csr(kkw_engine *eng, uint32_t v_num, const uint64_t *row_ptr,
+                       const uint32_t *col, const float *w);
+/* Synthetic undirected R-MAT built on the device (no reference counterpart; the
+ * benchmark graphs): 2^scale vertices, edge_factor * 2^scale raw edges, quadrant
+ * probabilities a, b, c (d = 1-a-b-c), self loops and duplicates removed, each
+ * edge mirrored, lists sorted, weights 1 + 4U per undirected edge. */
+int kkw_graph_rmat(kkw_engine *eng, int scale, int edge_factor, double a, double b,
+                   double c, uint64_t seed, int weighted);
+/* Edge types for metapath: WeightedMetaData/UnweightedMetaData::meta_info
+ * (src/apps/metascheme.hpp:42-67), one byte per CSR entry (< 8 types). */
+int kkw_graph_set_edge_types(kkw_engine *eng, const uint8_t *etype);
+/* Synthetic typing: vertex v gets type floor(ntypes * U(seed, v)); edge (u,v)
+ * gets the type of v. */
+int kkw_graph_type_by_dst(kkw_engine *eng, int ntypes, uint64_t seed);
+/* get_vertex_num / get_edge_num (include/graph.hpp:178-185). */
+int kkw_graph_info(kkw_engine *eng, uint32_t *v_num, uint64_t *e_num);
+/* Copy the CSR back (csr / vertex_out_degree, graph.hpp:152-156); any pointer may
+ * be NULL. */
+int kkw_graph_download(kkw_engine *eng, uint64_t *row_ptr, uint32_t *col, float *w,
+                       uint8_t *etype);
+
+/* Static component.  Replaces init_alias_tables (include/walk.hpp:538-652), kept
+ * resident across runs instead of being rebuilt per random_walk call. */
+int kkw_prepare(kkw_engine *eng);
+/* Copy the alias buckets back (creation order per vertex) and the per-vertex
+ * sequential weight sums; any pointer may be NULL. */
+int kkw_alias_download(kkw_engine *eng, float *p, uint32_t *primary, uint32_t *alias,
+                       uint32_t *meta, float *vsum);
+
+/* Run jobs to completion.  Replaces WalkEngine::random_walk (include/walk.hpp:423-436)
+ * as called by the four apps.  out[i] receives job i's paths. */
+int kkw_run(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mode, int out_where,
+            int flags, kkw_paths *out);
+int kkw_paths_free(kkw_paths *paths);
+/* PathSet::dump text format (include/path.hpp:85-109): per walker, optional
+ * "id len", then " v" per vertex, then newline; fopen mode "w" or "a". */
+int kkw_paths_dump(const kkw_paths *paths, const char *file, const char *mode,
+                   int with_head_info);
+int kkw_last_timing(kkw_engine *eng, kkw_timing *t);
+
+/* Resident batches: the job table, path buffers and (persistent mode) a
+ * captured hipGraph stay on the device; kkw_batch_launch replays them. */
+typedef struct kkw_batch kkw_batch;
+enum kkw_batch_flags {
+    KKW_BATCH_GRAPH = 1,     /* capture the persistent launch in a hipGraph and replay it */
+    KKW_BATCH_STATS = 2      /* instrumented kernels: trial / search / probe counters     */
+};
+int kkw_batch_create(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mode,
+                     int batch_flags, kkw_batch **out);
+int kkw_batch_set_shape(kkw_batch *b, int block, int blocks_per_cu, int chunk);
+int kkw_batch_launch(kkw_batch *b);          /* asynchronous on the engine stream */
+int kkw_batch_sync(kkw_batch *b, double *kernel_ms);
+int kkw_batch_paths(kkw_batch *b, int job, int out_where, kkw_paths *out);
+int kkw_batch_destroy(kkw_batch *b);
+
+/* Autotuner: times candidate (block, blocks_per_cu, chunk) shapes of the
+ * persistent kernel on this batch and keeps the fastest (cache file optional). */
+int kkw_autotune(kkw_batch *b, const char *cache_file, int *block, int *blocks_per_cu,
+                 int *chunk);
+
+/* Serving: a resident loop admits queries (usually PPR) by free HBM and slot
+ * count, batches everything admitted into rounds of the persistent kernel, and
+ * retires queries as their last walker finishes. */
+typedef struct kkw_serve_opts {
+    uint32_t max_walkers_in_flight;  /* slot budget (walkers)                        */
+    uint32_t round_steps;            /* walker steps per kernel round (0 = to end)    */
+    uint32_t queue_capacity;         /* pending queries before KKW_E_BUSY             */
+    uint32_t reserved;
+} kkw_serve_opts;
+int kkw_serve_start(kkw_engine *eng, const kkw_serve_opts *opts);
+int kkw_submit(kkw_engine *eng, const kkw_job *job, uint64_t *ticket);
+int kkw_poll(kkw_engine *eng, uint64_t ticket, int *done);
+int kkw_wait(kkw_engine *eng, uint64_t ticket, double timeout_s, kkw_paths *out,
+             double *latency_ms);
+int kkw_cancel(kkw_engine *eng, uint64_t ticket);
+int kkw_serve_stop(kkw_engine *eng);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

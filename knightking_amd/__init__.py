@@ -1,0 +1,419 @@
+"""knightking_amd -- MI355X random-walk engine (Python view of libkkw's C ABI).
+
+The product is libkkw.so (HIP kernels + C ABI, include/kkw.h).  This module
+loads it with ctypes and mirrors the reference's walk entry points
+(KnightKing src/apps/node2vec.hpp:108, deepwalk.hpp:30, ppr.hpp:30,
+metapath.hpp:70; WalkEngine::random_walk include/walk.hpp:423-436) for tests and
+the benchmark.  There is no CPU fallback: without the built library, or
+without a GPU, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkkw.so")
+
+KKW_OK = 0
+KKW_E_INVALID, KKW_E_NOGRAPH, KKW_E_NOMEM, KKW_E_DEVICE = -1, -2, -3, -4
+KKW_E_BUSY, KKW_E_CANCELLED, KKW_E_TIMEOUT, KKW_E_IO = -5, -6, -7, -8
+
+DEEPWALK, PPR, NODE2VEC, METAPATH = 0, 1, 2, 3
+START_EQUAL, START_LIST, START_UNIFORM = 0, 1, 2
+MODE_PLAIN, MODE_PERSISTENT = 0, 1
+OUT_HOST, OUT_DEVICE = 0, 1
+FLAG_STATS = 1
+BATCH_GRAPH, BATCH_STATS = 1, 2
+
+
+class KkwError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"kkw error {code}: {msg}")
+        self.code = code
+
+
+class _Opts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("verbose", C.c_int32), ("hbm_budget", C.c_uint64)]
+
+
+class _Job(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("use_weights", C.c_int32), ("walker_begin", C.c_uint32),
+        ("walker_num", C.c_uint32), ("walk_length", C.c_uint32), ("terminate_prob", C.c_float),
+        ("p", C.c_float), ("q", C.c_float), ("seed", C.c_uint64), ("start_mode", C.c_int32),
+        ("n_starts", C.c_uint32), ("starts", C.POINTER(C.c_uint32)), ("n_schemes", C.c_uint32),
+        ("scheme_len", C.POINTER(C.c_uint32)), ("scheme_mask", C.POINTER(C.c_uint8)),
+    ]
+
+
+class _Paths(C.Structure):
+    _fields_ = [
+        ("walker_begin", C.c_uint32), ("walker_num", C.c_uint32), ("stride", C.c_uint32),
+        ("on_device", C.c_uint32), ("vertices", C.POINTER(C.c_uint32)),
+        ("lengths", C.POINTER(C.c_uint32)), ("steps", C.c_uint64), ("truncated", C.c_uint64),
+        ("priv", C.c_void_p),
+    ]
+
+
+class _Timing(C.Structure):
+    _fields_ = [
+        ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("alias_ms", C.c_double),
+        ("graph_ms", C.c_double), ("steps", C.c_uint64), ("trials", C.c_uint64),
+        ("searches", C.c_uint64), ("search_probes", C.c_uint64), ("row_loads", C.c_uint64),
+    ]
+
+
+class _ServeOpts(C.Structure):
+    _fields_ = [("max_walkers_in_flight", C.c_uint32), ("round_steps", C.c_uint32),
+                ("queue_capacity", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+# Every symbol include/kkw.h declares (tests check the library exports them all).
+EXPORTED = [
+    "kkw_last_error", "kkw_abi_version", "kkw_engine_create", "kkw_engine_destroy",
+    "kkw_graph_from_edges", "kkw_graph_load_file", "kkw_graph_from_csr", "kkw_graph_rmat",
+    "kkw_graph_set_edge_types", "kkw_graph_type_by_dst", "kkw_graph_info", "kkw_graph_download",
+    "kkw_prepare", "kkw_alias_download", "kkw_run", "kkw_paths_free", "kkw_paths_dump",
+    "kkw_last_timing", "kkw_batch_create", "kkw_batch_set_shape", "kkw_batch_launch",
+    "kkw_batch_sync", "kkw_batch_paths", "kkw_batch_destroy", "kkw_autotune",
+    "kkw_serve_start", "kkw_submit", "kkw_poll", "kkw_wait", "kkw_cancel", "kkw_serve_stop",
+]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libkkw.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() "
+                           "(the engine has no CPU fallback)")
+    lib = C.CDLL(path)
+    P, U32, U64, I32, D = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_double
+    sig = {
+        "kkw_last_error": (C.c_char_p, []),
+        "kkw_abi_version": (I32, []),
+        "kkw_engine_create": (I32, [C.POINTER(_Opts), C.POINTER(P)]),
+        "kkw_engine_destroy": (I32, [P]),
+        "kkw_graph_from_edges": (I32, [P, U32, P, U64, I32, I32, I32]),
+        "kkw_graph_load_file": (I32, [P, U32, C.c_char_p, I32, I32, I32, I32]),
+        "kkw_graph_from_csr": (I32, [P, U32, P, P, P]),
+        "kkw_graph_rmat": (I32, [P, I32, I32, D, D, D, U64, I32]),
+        "kkw_graph_set_edge_types": (I32, [P, P]),
+        "kkw_graph_type_by_dst": (I32, [P, I32, U64]),
+        "kkw_graph_info": (I32, [P, C.POINTER(U32), C.POINTER(U64)]),
+        "kkw_graph_download": (I32, [P, P, P, P, P]),
+        "kkw_prepare": (I32, [P]),
+        "kkw_alias_download": (I32, [P, P, P, P, P, P]),
+        "kkw_run": (I32, [P, C.POINTER(_Job), I32, I32, I32, I32, C.POINTER(_Paths)]),
+        "kkw_paths_free": (I32, [C.POINTER(_Paths)]),
+        "kkw_paths_dump": (I32, [C.POINTER(_Paths), C.c_char_p, C.c_char_p, I32]),
+        "kkw_last_timing": (I32, [P, C.POINTER(_Timing)]),
+        "kkw_batch_create": (I32, [P, C.POINTER(_Job), I32, I32, I32, C.POINTER(P)]),
+        "kkw_batch_set_shape": (I32, [P, I32, I32, I32]),
+        "kkw_batch_launch": (I32, [P]),
+        "kkw_batch_sync": (I32, [P, C.POINTER(D)]),
+        "kkw_batch_paths": (I32, [P, I32, I32, C.POINTER(_Paths)]),
+        "kkw_batch_destroy": (I32, [P]),
+        "kkw_autotune": (I32, [P, C.c_char_p, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
+        "kkw_serve_start": (I32, [P, C.POINTER(_ServeOpts)]),
+        "kkw_submit": (I32, [P, C.POINTER(_Job), C.POINTER(U64)]),
+        "kkw_poll": (I32, [P, U64, C.POINTER(I32)]),
+        "kkw_wait": (I32, [P, U64, D, C.POINTER(_Paths), C.POINTER(D)]),
+        "kkw_cancel": (I32, [P, U64]),
+        "kkw_serve_stop": (I32, [P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != KKW_OK:
+        msg = _lib.kkw_last_error().decode(errors="replace")
+        raise KkwError(rc, msg)
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+@dataclass
+class Job:
+    """One walk job (kkw_job): the arguments of node2vec()/deepwalk()/ppr()/metapath()."""
+    kind: int
+    walker_num: int
+    walk_length: int = 0
+    use_weights: bool = False
+    walker_begin: int = 0
+    terminate_prob: float = 0.0
+    p: float = 1.0
+    q: float = 1.0
+    seed: int = 1
+    start_mode: int = START_EQUAL
+    starts: Optional[np.ndarray] = None
+    schemes: Optional[List[List[int]]] = None  # per scheme: allowed-type bitmask per state
+    _keep: list = field(default_factory=list, repr=False)
+
+    def to_c(self) -> _Job:
+        j = _Job()
+        j.kind = self.kind
+        j.use_weights = 1 if self.use_weights else 0
+        j.walker_begin = self.walker_begin
+        j.walker_num = self.walker_num
+        j.walk_length = self.walk_length
+        j.terminate_prob = self.terminate_prob
+        j.p, j.q = self.p, self.q
+        j.seed = self.seed
+        j.start_mode = self.start_mode
+        self._keep = []
+        if self.starts is not None:
+            s = np.ascontiguousarray(self.starts, dtype=np.uint32)
+            self._keep.append(s)
+            j.n_starts = len(s)
+            j.starts = s.ctypes.data_as(C.POINTER(C.c_uint32))
+        if self.schemes is not None:
+            lens = np.array([len(s) for s in self.schemes], dtype=np.uint32)
+            masks = np.zeros((len(self.schemes), 8), dtype=np.uint8)
+            for i, s in enumerate(self.schemes):
+                masks[i, :len(s)] = s
+            masks = np.ascontiguousarray(masks.reshape(-1))
+            self._keep += [lens, masks]
+            j.n_schemes = len(self.schemes)
+            j.scheme_len = lens.ctypes.data_as(C.POINTER(C.c_uint32))
+            j.scheme_mask = masks.ctypes.data_as(C.POINTER(C.c_uint8))
+        return j
+
+
+def node2vec_job(walker_num, walk_length, p, q, weighted=True, seed=1, **kw) -> Job:
+    return Job(NODE2VEC, walker_num, walk_length, use_weights=weighted, p=p, q=q, seed=seed, **kw)
+
+
+def deepwalk_job(walker_num, walk_length, weighted=True, seed=1, **kw) -> Job:
+    return Job(DEEPWALK, walker_num, walk_length, use_weights=weighted, seed=seed, **kw)
+
+
+def ppr_job(walker_num, terminate_prob, weighted=True, seed=1, start_vertices=None, cap=0, **kw) -> Job:
+    mode = START_LIST if start_vertices is not None else START_EQUAL
+    return Job(PPR, walker_num, cap, use_weights=weighted, terminate_prob=terminate_prob, seed=seed,
+               start_mode=mode, starts=start_vertices, **kw)
+
+
+def metapath_job(walker_num, walk_length, schemes, weighted=True, seed=1, **kw) -> Job:
+    return Job(METAPATH, walker_num, walk_length, use_weights=weighted, schemes=schemes, seed=seed, **kw)
+
+
+@dataclass
+class Paths:
+    """PathSet of one job, one segment, walker-id order (include/path.hpp:46-84)."""
+    walker_begin: int
+    vertices: np.ndarray      # [walker_num, stride] uint32
+    lengths: np.ndarray       # [walker_num] uint32 (vertices per path)
+    truncated: np.ndarray     # [walker_num] bool
+    steps: int
+
+    def path(self, i: int) -> np.ndarray:
+        return self.vertices[i, : self.lengths[i]]
+
+
+def _take_paths(p: _Paths) -> Paths:
+    n, s = p.walker_num, p.stride
+    if n:
+        verts = np.ctypeslib.as_array(p.vertices, shape=(n * s,)).reshape(n, s).copy()
+        raw = np.ctypeslib.as_array(p.lengths, shape=(n,)).copy()
+    else:
+        verts = np.zeros((0, s), np.uint32)
+        raw = np.zeros(0, np.uint32)
+    out = Paths(p.walker_begin, verts, raw & 0x7FFFFFFF, (raw >> 31).astype(bool), int(p.steps))
+    _lib.kkw_paths_free(C.byref(p))
+    return out
+
+
+class Batch:
+    """Resident batch (kkw_batch): job table, path buffers, optional hipGraph."""
+
+    def __init__(self, eng: "Engine", jobs: Sequence[Job], mode: int, flags: int = 0):
+        self.eng = eng
+        self._cj = (_Job * len(jobs))(*[j.to_c() for j in jobs])
+        self._jobs = list(jobs)
+        h = C.c_void_p()
+        _check(_lib.kkw_batch_create(eng.h, self._cj, len(jobs), mode, flags, C.byref(h)))
+        self.h = h
+
+    def set_shape(self, block=256, blocks_per_cu=8, chunk=64):
+        _check(_lib.kkw_batch_set_shape(self.h, block, blocks_per_cu, chunk))
+
+    def launch(self):
+        _check(_lib.kkw_batch_launch(self.h))
+
+    def sync(self) -> float:
+        ms = C.c_double()
+        _check(_lib.kkw_batch_sync(self.h, C.byref(ms)))
+        return ms.value
+
+    def paths(self, i: int) -> Paths:
+        p = _Paths()
+        _check(_lib.kkw_batch_paths(self.h, i, OUT_HOST, C.byref(p)))
+        return _take_paths(p)
+
+    def steps(self, i: int) -> int:
+        p = _Paths()
+        _check(_lib.kkw_batch_paths(self.h, i, OUT_DEVICE, C.byref(p)))
+        return int(p.steps)
+
+    def autotune(self, cache_file: Optional[str] = None):
+        b, bp, ch = C.c_int(), C.c_int(), C.c_int()
+        _check(_lib.kkw_autotune(self.h, cache_file.encode() if cache_file else None,
+                                 C.byref(b), C.byref(bp), C.byref(ch)))
+        return b.value, bp.value, ch.value
+
+    def close(self):
+        if self.h:
+            _lib.kkw_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Engine:
+    """WalkEngine on one GPU (include/walk.hpp:365-437)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, verbose: bool = False):
+        load_library()
+        o = _Opts(device, 1 if verbose else 0, hbm_budget)
+        h = C.c_void_p()
+        _check(_lib.kkw_engine_create(C.byref(o), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.kkw_engine_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- graphs --------------------------------------------------------
+    def graph_rmat(self, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, weighted=True):
+        _check(_lib.kkw_graph_rmat(self.h, scale, edge_factor, a, b, c, seed, 1 if weighted else 0))
+
+    def graph_from_edges(self, v_num, src, dst, w=None, make_undirected=False, sort_adj=False):
+        n = len(src)
+        if w is None:
+            rec = np.zeros(n, dtype=[("s", "<u4"), ("d", "<u4")])
+        else:
+            rec = np.zeros(n, dtype=[("s", "<u4"), ("d", "<u4"), ("w", "<f4")])
+            rec["w"] = w
+        rec["s"], rec["d"] = src, dst
+        _check(_lib.kkw_graph_from_edges(self.h, v_num, _ptr(rec), n, 0 if w is None else 1,
+                                         1 if make_undirected else 0, 1 if sort_adj else 0))
+
+    def load_graph(self, v_num, path, text=False, weighted=False, make_undirected=False, sort_adj=False):
+        _check(_lib.kkw_graph_load_file(self.h, v_num, path.encode(), int(text), int(weighted),
+                                        int(make_undirected), int(sort_adj)))
+
+    def graph_from_csr(self, row_ptr, col, w=None):
+        row_ptr = np.ascontiguousarray(row_ptr, np.uint64)
+        col = np.ascontiguousarray(col, np.uint32)
+        w = None if w is None else np.ascontiguousarray(w, np.float32)
+        _check(_lib.kkw_graph_from_csr(self.h, len(row_ptr) - 1, _ptr(row_ptr), _ptr(col), _ptr(w)))
+
+    def set_edge_types(self, etype):
+        etype = np.ascontiguousarray(etype, np.uint8)
+        _check(_lib.kkw_graph_set_edge_types(self.h, _ptr(etype)))
+
+    def type_by_dst(self, ntypes, seed):
+        _check(_lib.kkw_graph_type_by_dst(self.h, ntypes, seed))
+
+    def info(self):
+        v, e = C.c_uint32(), C.c_uint64()
+        _check(_lib.kkw_graph_info(self.h, C.byref(v), C.byref(e)))
+        return v.value, e.value
+
+    def download(self, weights=True, etype=False):
+        V, E = self.info()
+        rp = np.zeros(V + 1, np.uint64)
+        col = np.zeros(E, np.uint32)
+        w = np.zeros(E, np.float32) if weights else None
+        et = np.zeros(E, np.uint8) if etype else None
+        _check(_lib.kkw_graph_download(self.h, _ptr(rp), _ptr(col), _ptr(w), _ptr(et)))
+        return rp, col, w, et
+
+    def prepare(self):
+        _check(_lib.kkw_prepare(self.h))
+
+    def alias_download(self):
+        V, E = self.info()
+        p = np.zeros(E, np.float32)
+        prim = np.zeros(E, np.uint32)
+        al = np.zeros(E, np.uint32)
+        meta = np.zeros(E, np.uint32)
+        vs = np.zeros(V, np.float32)
+        _check(_lib.kkw_alias_download(self.h, _ptr(p), _ptr(prim), _ptr(al), _ptr(meta), _ptr(vs)))
+        return p, prim, al, meta, vs
+
+    # ---- walking ---------------------------------------------------------
+    def run(self, jobs: Sequence[Job], mode: int = MODE_PLAIN, stats: bool = False) -> List[Paths]:
+        cj = (_Job * len(jobs))(*[j.to_c() for j in jobs])
+        outs = (_Paths * len(jobs))()
+        _check(_lib.kkw_run(self.h, cj, len(jobs), mode, OUT_HOST, FLAG_STATS if stats else 0, outs))
+        return [_take_paths(outs[i]) for i in range(len(jobs))]
+
+    def batch(self, jobs: Sequence[Job], mode: int = MODE_PERSISTENT, flags: int = 0) -> Batch:
+        return Batch(self, jobs, mode, flags)
+
+    def timing(self) -> dict:
+        t = _Timing()
+        _check(_lib.kkw_last_timing(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in _Timing._fields_}
+
+    # ---- serving ---------------------------------------------------------
+    def serve_start(self, max_walkers_in_flight=1 << 22, queue_capacity=1 << 16):
+        o = _ServeOpts(max_walkers_in_flight, 0, queue_capacity, 0)
+        _check(_lib.kkw_serve_start(self.h, C.byref(o)))
+
+    def submit(self, job: Job) -> int:
+        t = C.c_uint64()
+        cj = job.to_c()
+        _check(_lib.kkw_submit(self.h, C.byref(cj), C.byref(t)))
+        return t.value
+
+    def poll(self, ticket: int) -> bool:
+        d = C.c_int()
+        _check(_lib.kkw_poll(self.h, ticket, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket: int, timeout_s: float = -1.0):
+        p = _Paths()
+        lat = C.c_double()
+        _check(_lib.kkw_wait(self.h, ticket, timeout_s, C.byref(p), C.byref(lat)))
+        return _take_paths(p), lat.value
+
+    def cancel(self, ticket: int):
+        _check(_lib.kkw_cancel(self.h, ticket))
+
+    def serve_stop(self):
+        _check(_lib.kkw_serve_stop(self.h))

@@ -1,0 +1,1184 @@
+// engine.hip -- the C ABI of libkkw (include/kkw.h): engine state, graph
+// loading, job validation, resident batches, kkw_run, path output, the
+// autotuner and the serving loop.
+#include "kkw_internal.h"
+#include "../../include/kkw.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+using namespace kkw;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(F &&f)
+{
+    try {
+        return f();
+    } catch (const Error &e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(KKW_E_NOMEM, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(KKW_E_INVALID, e.what());
+    }
+}
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+// Device allocation owned by a batch / request.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n)
+    {
+        bytes = n;
+        hipError_t e = hipMalloc(&p, n ? n : 1);
+        if (e != hipSuccess)
+            throw Error(KKW_E_NOMEM, "hipMalloc of " + std::to_string(n) + " bytes: " + hipGetErrorString(e));
+    }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+uint32_t ppr_default_cap(float t)
+{
+    // P(walk longer than cap) = (1-t)^cap <= 1e-12
+    if (!(t > 0.0f)) return 0;
+    double c = std::ceil(27.7 / (double)t);
+    return (uint32_t)std::min(c, 1048576.0);
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------
+struct ServeState;
+
+struct kkw_engine {
+    int device = 0;
+    int verbose = 0;
+    uint64_t hbm_budget = 0;
+    int cu_count = 256;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    GraphStore gs;
+    kkw_timing timing{};
+    std::unique_ptr<ServeState> serve;
+};
+
+// Host copy of a validated job plus its device-side arrays.
+struct PreparedJob {
+    kkw_job src{};
+    DevJob dj{};
+    uint32_t stride = 0;
+    std::vector<uint32_t> starts;
+    std::vector<uint32_t> scheme_len;
+    std::vector<uint8_t> scheme_mask;
+};
+
+static void node2vec_constants(float p, float q, DevJob &d)
+{
+    // src/apps/node2vec.hpp:126-127 (double max/min stored as real_t),
+    // 136 (overflow), 182-191 (float 1/p, 1/q).
+    double iq = 1.0 / (double)q, ip = 1.0 / (double)p;
+    double up = std::max(1.0, iq);
+    double lo = std::min(ip, std::min(1.0, iq));
+    d.upper = (float)up;
+    d.lower = (float)lo;
+    d.inv_p = 1.0f / p;
+    d.inv_q = 1.0f / q;
+    d.overflow = (float)(1.0 / (double)p - (double)d.upper);
+    d.acc_lo = std::min(1.0f, d.inv_q);
+    d.acc_hi = std::max(1.0f, d.inv_q);
+}
+
+static PreparedJob prepare_job(kkw_engine *eng, const kkw_job &j)
+{
+    GraphStore &gs = eng->gs;
+    if (!gs.row_ptr) throw Error(KKW_E_NOGRAPH, "no graph loaded");
+    PreparedJob pj;
+    pj.src = j;
+    DevJob &d = pj.dj;
+    d = DevJob{};
+    if (j.kind < KKW_DEEPWALK || j.kind > KKW_METAPATH) throw Error(KKW_E_INVALID, "unknown job kind");
+    d.kind = j.kind;
+    d.use_alias = j.use_weights ? 1 : 0;
+    d.walker_begin = j.walker_begin;
+    d.walker_num = j.walker_num;
+    if ((uint64_t)j.walker_begin + j.walker_num > 0xFFFFFFFFull)
+        throw Error(KKW_E_INVALID, "walker ids must fit in 32 bits");
+    d.seed = j.seed;
+    uint32_t L = j.walk_length;
+    if (j.kind == KKW_PPR) {
+        if (!(j.terminate_prob >= 0.0f && j.terminate_prob <= 1.0f))
+            throw Error(KKW_E_INVALID, "terminate_prob must be in [0,1]");
+        d.ppr_ext = 1.0f - j.terminate_prob;  // ppr.hpp:36
+        if (L == 0) L = ppr_default_cap(j.terminate_prob);
+        if (L == 0) throw Error(KKW_E_INVALID, "PPR with terminate_prob 0 needs a walk_length cap");
+    }
+    if (L > (1u << 24)) throw Error(KKW_E_INVALID, "walk_length above 2^24");
+    d.walk_length = L;
+    pj.stride = L + 1;
+    d.stride = pj.stride;
+    if (j.kind == KKW_NODE2VEC) {
+        if (!(j.p > 0.0f && std::isfinite(j.p) && j.q > 0.0f && std::isfinite(j.q)))
+            throw Error(KKW_E_INVALID, "node2vec p and q must be positive and finite");
+        if (!gs.sorted)
+            throw Error(KKW_E_INVALID, "node2vec needs adjacency lists sorted by neighbour "
+                                       "(load with sort_adj=1, as node2vec.hpp:122-125 sorts)");
+        node2vec_constants(j.p, j.q, d);
+    }
+    if (j.use_weights) {
+        if (!gs.w) throw Error(KKW_E_INVALID, "use_weights on a graph without weights");
+        if (!gs.bucket) {
+            double t0 = now_ms();
+            build_alias(gs, eng->stream);
+            eng->timing.alias_ms = now_ms() - t0;
+        }
+    }
+    d.start_mode = j.start_mode;
+    if (j.start_mode == KKW_START_LIST) {
+        if (j.n_starts == 0 || !j.starts) throw Error(KKW_E_INVALID, "start list is empty");
+        pj.starts.assign(j.starts, j.starts + j.n_starts);
+        for (uint32_t v : pj.starts)
+            if (v >= gs.V) throw Error(KKW_E_INVALID, "start vertex >= v_num");
+        d.n_starts = j.n_starts;
+    } else if (j.start_mode != KKW_START_EQUAL && j.start_mode != KKW_START_UNIFORM) {
+        throw Error(KKW_E_INVALID, "unknown start mode");
+    }
+    if (j.kind == KKW_METAPATH) {
+        if (!gs.vmask) throw Error(KKW_E_INVALID, "metapath needs edge types (kkw_graph_set_edge_types)");
+        if (j.n_schemes == 0 || !j.scheme_len || !j.scheme_mask)
+            throw Error(KKW_E_INVALID, "metapath needs at least one scheme");
+        pj.scheme_len.assign(j.scheme_len, j.scheme_len + j.n_schemes);
+        for (uint32_t l : pj.scheme_len)
+            if (l < 1 || l > 8) throw Error(KKW_E_INVALID, "scheme length must be in [1,8]");
+        pj.scheme_mask.assign(j.scheme_mask, j.scheme_mask + (size_t)j.n_schemes * 8);
+        d.n_schemes = j.n_schemes;
+    }
+    return pj;
+}
+
+// --------------------------------------------------------------------------
+struct kkw_batch {
+    kkw_engine *eng = nullptr;
+    hipStream_t stream = nullptr;   // stream the batch runs on (engine's or a serving stream)
+    int mode = KKW_MODE_PLAIN;
+    bool stats = false;
+    bool use_graph = false;
+    std::vector<PreparedJob> jobs;
+    std::vector<DevBuf> bufs;
+    DevJob *d_jobs = nullptr;
+    unsigned long long *d_queue = nullptr;
+    DevStats *d_stats = nullptr;
+    uint64_t n_chunks = 0;
+    int block = 256, blocks_per_cu = 8, chunk = 64;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool launched = false;
+    double host_t0 = 0.0;
+
+    ~kkw_batch()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+    void *alloc(size_t bytes)
+    {
+        bufs.emplace_back(bytes);
+        return bufs.back().p;
+    }
+};
+
+static void batch_setup(kkw_batch *b, const kkw_job *jobs, int n_jobs)
+{
+    kkw_engine *eng = b->eng;
+    if (n_jobs < 0 || (n_jobs > 0 && !jobs)) throw Error(KKW_E_INVALID, "bad job array");
+    for (int i = 0; i < n_jobs; i++) b->jobs.push_back(prepare_job(eng, jobs[i]));
+    KKW_HIP(hipEventCreate(&b->e0));
+    KKW_HIP(hipEventCreate(&b->e1));
+    b->d_queue = (unsigned long long *)b->alloc(sizeof(unsigned long long) * 2);
+    b->d_stats = (DevStats *)b->alloc(sizeof(DevStats));
+    uint64_t chunk_acc = 0;
+    for (auto &pj : b->jobs) {
+        DevJob &d = pj.dj;
+        size_t pbytes = (size_t)d.walker_num * pj.stride * sizeof(uint32_t);
+        d.paths = (uint32_t *)b->alloc(pbytes);
+        d.lens = (uint32_t *)b->alloc((size_t)d.walker_num * sizeof(uint32_t));
+        if (!pj.starts.empty()) {
+            uint32_t *p = (uint32_t *)b->alloc(pj.starts.size() * 4);
+            KKW_HIP(hipMemcpyAsync(p, pj.starts.data(), pj.starts.size() * 4, hipMemcpyHostToDevice, b->stream));
+            d.starts = p;
+        }
+        if (!pj.scheme_len.empty()) {
+            uint32_t *p = (uint32_t *)b->alloc(pj.scheme_len.size() * 4);
+            KKW_HIP(hipMemcpyAsync(p, pj.scheme_len.data(), pj.scheme_len.size() * 4, hipMemcpyHostToDevice,
+                                   b->stream));
+            d.scheme_len = p;
+            uint8_t *m = (uint8_t *)b->alloc(pj.scheme_mask.size());
+            KKW_HIP(hipMemcpyAsync(m, pj.scheme_mask.data(), pj.scheme_mask.size(), hipMemcpyHostToDevice,
+                                   b->stream));
+            d.scheme_mask = m;
+        }
+        d.chunk_begin = chunk_acc;
+        chunk_acc += (d.walker_num + b->chunk - 1) / b->chunk;
+    }
+    b->n_chunks = chunk_acc;
+    if (!b->jobs.empty()) {
+        b->d_jobs = (DevJob *)b->alloc(sizeof(DevJob) * b->jobs.size());
+        std::vector<DevJob> h;
+        for (auto &pj : b->jobs) h.push_back(pj.dj);
+        KKW_HIP(hipMemcpyAsync(b->d_jobs, h.data(), sizeof(DevJob) * h.size(), hipMemcpyHostToDevice, b->stream));
+    }
+    KKW_HIP(hipStreamSynchronize(b->stream));
+}
+
+// recompute chunk boundaries after a chunk-size change
+static void batch_rechunk(kkw_batch *b)
+{
+    uint64_t acc = 0;
+    std::vector<DevJob> h;
+    for (auto &pj : b->jobs) {
+        pj.dj.chunk_begin = acc;
+        acc += (pj.dj.walker_num + b->chunk - 1) / b->chunk;
+        h.push_back(pj.dj);
+    }
+    b->n_chunks = acc;
+    if (!h.empty()) {
+        KKW_HIP(hipMemcpyAsync(b->d_jobs, h.data(), sizeof(DevJob) * h.size(), hipMemcpyHostToDevice, b->stream));
+        KKW_HIP(hipStreamSynchronize(b->stream));
+    }
+    if (b->gexec) {
+        (void)hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+}
+
+static void batch_enqueue(kkw_batch *b)
+{
+    DevGraph g = b->eng->gs.view();
+    hipStream_t s = b->stream;
+    if (b->stats) launch_fill_u64((unsigned long long *)b->d_stats, 0ull, 5, s);
+    if (b->mode == KKW_MODE_PLAIN) {
+        for (auto &pj : b->jobs) launch_walk_plain(g, pj.dj, b->stats, b->d_stats, b->block, s);
+    } else {
+        if (b->n_chunks == 0) return;
+        launch_fill_u64(b->d_queue, 0ull, 1, s);
+        uint64_t want = (uint64_t)b->eng->cu_count * b->blocks_per_cu;
+        uint64_t waves_needed = b->n_chunks;  // one chunk per wave keeps every wave busy
+        uint64_t blocks_needed = (waves_needed * 64 + b->block - 1) / b->block;
+        int grid = (int)std::max<uint64_t>(1, std::min(want, blocks_needed));
+        if (b->jobs.size() == 1)
+            launch_walk_persistent_single(g, b->jobs[0].dj, b->n_chunks, (uint32_t)b->chunk, b->d_queue, b->stats,
+                                          b->d_stats, grid, b->block, s);
+        else
+            launch_walk_persistent_mixed(g, b->d_jobs, (int)b->jobs.size(), b->n_chunks, (uint32_t)b->chunk,
+                                         b->d_queue, b->stats, b->d_stats, grid, b->block, s);
+    }
+    KKW_HIP(hipGetLastError());
+}
+
+static void batch_launch(kkw_batch *b)
+{
+    b->host_t0 = now_ms();
+    KKW_HIP(hipEventRecord(b->e0, b->stream));
+    if (b->use_graph && b->mode == KKW_MODE_PERSISTENT) {
+        if (!b->gexec) {
+            KKW_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                batch_enqueue(b);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(b->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            KKW_HIP(hipStreamEndCapture(b->stream, &b->graph));
+            KKW_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        }
+        KKW_HIP(hipGraphLaunch(b->gexec, b->stream));
+    } else {
+        batch_enqueue(b);
+    }
+    KKW_HIP(hipEventRecord(b->e1, b->stream));
+    b->launched = true;
+}
+
+static double batch_sync(kkw_batch *b)
+{
+    if (!b->launched) throw Error(KKW_E_INVALID, "batch was not launched");
+    KKW_HIP(hipEventSynchronize(b->e1));
+    float ms = 0.0f;
+    KKW_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
+    kkw_timing &t = b->eng->timing;
+    t.kernel_ms = ms;
+    t.total_ms = now_ms() - b->host_t0;
+    if (b->stats) {
+        DevStats h{};
+        KKW_HIP(hipMemcpy(&h, b->d_stats, sizeof(h), hipMemcpyDeviceToHost));
+        t.steps = h.steps;
+        t.trials = h.trials;
+        t.searches = h.searches;
+        t.search_probes = h.probes;
+        t.row_loads = h.row_loads;
+    }
+    return ms;
+}
+
+static void fill_paths_host(const PreparedJob &pj, hipStream_t s, kkw_paths *out)
+{
+    const DevJob &d = pj.dj;
+    memset(out, 0, sizeof(*out));
+    out->walker_begin = d.walker_begin;
+    out->walker_num = d.walker_num;
+    out->stride = pj.stride;
+    size_t n = (size_t)d.walker_num * pj.stride;
+    out->vertices = (uint32_t *)malloc(n ? n * 4 : 4);
+    out->lengths = (uint32_t *)malloc(d.walker_num ? (size_t)d.walker_num * 4 : 4);
+    if (!out->vertices || !out->lengths) {
+        free(out->vertices);
+        free(out->lengths);
+        throw Error(KKW_E_NOMEM, "host allocation for paths failed");
+    }
+    if (n) KKW_HIP(hipMemcpyAsync(out->vertices, d.paths, n * 4, hipMemcpyDeviceToHost, s));
+    if (d.walker_num) KKW_HIP(hipMemcpyAsync(out->lengths, d.lens, (size_t)d.walker_num * 4, hipMemcpyDeviceToHost, s));
+    KKW_HIP(hipStreamSynchronize(s));
+    uint64_t steps = 0, trunc = 0;
+    for (uint32_t i = 0; i < d.walker_num; i++) {
+        uint32_t l = out->lengths[i];
+        steps += (l & 0x7fffffffu) - 1;
+        trunc += l >> 31;
+    }
+    out->steps = steps;
+    out->truncated = trunc;
+    out->on_device = 0;
+    out->priv = (void *)1;  // host buffers owned by the caller (free with kkw_paths_free)
+}
+
+static void fill_paths_device(const PreparedJob &pj, hipStream_t s, kkw_paths *out)
+{
+    const DevJob &d = pj.dj;
+    memset(out, 0, sizeof(*out));
+    out->walker_begin = d.walker_begin;
+    out->walker_num = d.walker_num;
+    out->stride = pj.stride;
+    out->vertices = d.paths;
+    out->lengths = d.lens;
+    out->on_device = 1;
+    std::vector<uint32_t> lens(d.walker_num);
+    if (d.walker_num) {
+        KKW_HIP(hipMemcpyAsync(lens.data(), d.lens, (size_t)d.walker_num * 4, hipMemcpyDeviceToHost, s));
+        KKW_HIP(hipStreamSynchronize(s));
+    }
+    uint64_t steps = 0, trunc = 0;
+    for (uint32_t l : lens) {
+        steps += (l & 0x7fffffffu) - 1;
+        trunc += l >> 31;
+    }
+    out->steps = steps;
+    out->truncated = trunc;
+    out->priv = nullptr;  // owned by the batch
+}
+
+// ==========================================================================
+// Serving
+// ==========================================================================
+struct Request {
+    uint64_t ticket = 0;
+    PreparedJob pj;
+    double t_submit = 0.0, t_done = 0.0;
+    int status = 1;  // 1 pending, 0 done, <0 error
+    bool cancelled = false;
+    kkw_paths result{};
+    size_t bytes = 0;
+};
+
+struct ServeState {
+    kkw_engine *eng = nullptr;
+    kkw_serve_opts opts{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> pending;
+    std::map<uint64_t, std::shared_ptr<Request>> all;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    std::thread worker;
+    hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint64_t walkers_in_flight = 0;
+    size_t bytes_in_flight = 0;
+};
+
+struct InFlight {
+    std::unique_ptr<kkw_batch> batch;
+    std::vector<std::shared_ptr<Request>> reqs;
+    int stream_idx = 0;
+    uint64_t walkers = 0;
+    size_t bytes = 0;
+};
+
+static size_t request_bytes(const PreparedJob &pj)
+{
+    return (size_t)pj.dj.walker_num * (pj.stride + 1) * 4 + pj.starts.size() * 4 + 4096;
+}
+
+static void serve_loop(ServeState *S)
+{
+    kkw_engine *eng = S->eng;
+    (void)hipSetDevice(eng->device);
+    std::vector<InFlight> flights;
+    bool stream_busy[4] = {false, false, false, false};
+    for (;;) {
+        // 1. retire finished launches
+        for (size_t i = 0; i < flights.size();) {
+            InFlight &f = flights[i];
+            if (hipEventQuery(f.batch->e1) == hipSuccess) {
+                std::vector<kkw_paths> outs(f.reqs.size());
+                int st = 0;
+                std::string err;
+                try {
+                    for (size_t k = 0; k < f.reqs.size(); k++)
+                        fill_paths_host(f.batch->jobs[k], f.batch->stream, &outs[k]);
+                } catch (const Error &e) {
+                    st = e.code;
+                    err = e.what();
+                }
+                double t = now_ms();
+                {
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    for (size_t k = 0; k < f.reqs.size(); k++) {
+                        f.reqs[k]->result = outs[k];
+                        f.reqs[k]->status = st;
+                        f.reqs[k]->t_done = t;
+                    }
+                    S->walkers_in_flight -= f.walkers;
+                    S->bytes_in_flight -= f.bytes;
+                }
+                stream_busy[f.stream_idx] = false;
+                S->cv_done.notify_all();
+                flights.erase(flights.begin() + i);
+            } else {
+                i++;
+            }
+        }
+        // 2. admit pending requests into a new launch on a free stream
+        std::vector<std::shared_ptr<Request>> admit;
+        int sidx = -1;
+        for (int k = 0; k < 4; k++)
+            if (!stream_busy[k]) {
+                sidx = k;
+                break;
+            }
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (S->stop && S->pending.empty() && flights.empty()) break;
+            if (sidx >= 0) {
+                size_t free_b = 0, total_b = 0;
+                (void)hipMemGetInfo(&free_b, &total_b);
+                size_t budget = eng->hbm_budget ? eng->hbm_budget : free_b;
+                size_t used = S->bytes_in_flight;
+                uint64_t walkers = S->walkers_in_flight;
+                while (!S->pending.empty()) {
+                    auto r = S->pending.front();
+                    if (r->cancelled) {
+                        S->pending.pop_front();
+                        r->status = KKW_E_CANCELLED;
+                        r->t_done = now_ms();
+                        S->cv_done.notify_all();
+                        continue;
+                    }
+                    bool fits_mem = eng->hbm_budget ? (used + r->bytes <= budget) : (r->bytes + (64u << 20) <= budget);
+                    bool fits_slots = walkers + r->pj.dj.walker_num <= S->opts.max_walkers_in_flight ||
+                                      (walkers == 0 && admit.empty());
+                    if (!fits_mem) {
+                        if (used == 0 && admit.empty()) {
+                            // can never fit: fail it instead of blocking the queue
+                            S->pending.pop_front();
+                            r->status = KKW_E_NOMEM;
+                            r->t_done = now_ms();
+                            S->cv_done.notify_all();
+                            continue;
+                        }
+                        break;
+                    }
+                    if (!fits_slots) break;
+                    S->pending.pop_front();
+                    admit.push_back(r);
+                    used += r->bytes;
+                    walkers += r->pj.dj.walker_num;
+                    if (!eng->hbm_budget) budget = budget > r->bytes ? budget - r->bytes : 0;
+                    if (admit.size() >= 1024) break;
+                }
+                if (!admit.empty()) {
+                    S->bytes_in_flight = used;
+                    S->walkers_in_flight = walkers;
+                }
+            }
+            if (admit.empty()) {
+                if (flights.empty()) {
+                    if (S->stop && S->pending.empty()) break;
+                    S->cv_work.wait_for(lk, std::chrono::milliseconds(50));
+                    continue;
+                }
+            }
+        }
+        if (!admit.empty()) {
+            InFlight f;
+            f.stream_idx = sidx;
+            f.reqs = admit;
+            try {
+                std::lock_guard<std::recursive_mutex> elk(eng->mu);
+                auto b = std::make_unique<kkw_batch>();
+                b->eng = eng;
+                b->stream = S->streams[sidx];
+                b->mode = KKW_MODE_PERSISTENT;
+                b->chunk = 64;
+                b->block = 256;
+                std::vector<kkw_job> js;
+                for (auto &r : admit) js.push_back(r->pj.src);
+                batch_setup(b.get(), js.data(), (int)js.size());
+                batch_launch(b.get());
+                for (auto &r : admit) {
+                    f.walkers += r->pj.dj.walker_num;
+                    f.bytes += r->bytes;
+                }
+                f.batch = std::move(b);
+                stream_busy[sidx] = true;
+                flights.push_back(std::move(f));
+            } catch (const Error &e) {
+                std::lock_guard<std::mutex> lk(S->mu);
+                for (auto &r : admit) {
+                    r->status = e.code;
+                    r->t_done = now_ms();
+                    S->walkers_in_flight -= r->pj.dj.walker_num;
+                    S->bytes_in_flight -= r->bytes;
+                }
+                S->cv_done.notify_all();
+            }
+            continue;
+        }
+        // 3. nothing to admit: wait briefly for completions
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+extern "C" {
+
+const char *kkw_last_error(void) { return g_last_error.c_str(); }
+
+int kkw_abi_version(void) { return KKW_ABI_VERSION; }
+
+int kkw_engine_create(const kkw_engine_opts *opts, kkw_engine **out)
+{
+    return guarded([&]() -> int {
+        if (!out) return fail(KKW_E_INVALID, "out is null");
+        *out = nullptr;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            return fail(KKW_E_DEVICE, "no HIP device visible (libkkw needs a GPU; there is no CPU fallback)");
+        auto eng = std::make_unique<kkw_engine>();
+        eng->device = opts ? opts->device : 0;
+        eng->verbose = opts ? opts->verbose : 0;
+        eng->hbm_budget = opts ? opts->hbm_budget : 0;
+        if (eng->device < 0 || eng->device >= ndev) return fail(KKW_E_INVALID, "device ordinal out of range");
+        KKW_HIP(hipSetDevice(eng->device));
+        KKW_HIP(hipDeviceGetAttribute(&eng->cu_count, hipDeviceAttributeMultiprocessorCount, eng->device));
+        KKW_HIP(hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking));
+        *out = eng.release();
+        return KKW_OK;
+    });
+}
+
+int kkw_engine_destroy(kkw_engine *eng)
+{
+    if (!eng) return KKW_OK;
+    kkw_serve_stop(eng);
+    (void)hipSetDevice(eng->device);
+    eng->gs.release();
+    if (eng->stream) (void)hipStreamDestroy(eng->stream);
+    delete eng;
+    return KKW_OK;
+}
+
+int kkw_graph_from_edges(kkw_engine *eng, uint32_t v_num, const void *edges, uint64_t n_edges, int weighted,
+                         int make_undirected, int sort_adj)
+{
+    return guarded([&]() -> int {
+        if (!eng) return fail(KKW_E_INVALID, "engine is null");
+        if (n_edges && !edges) return fail(KKW_E_INVALID, "edges is null");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        double t0 = now_ms();
+        std::vector<uint32_t> src(n_edges), dst(n_edges);
+        std::vector<float> w(weighted ? n_edges : 0);
+        const size_t rec = weighted ? 12 : 8;  // Edge<real_t> / Edge<EmptyData>, type.hpp:43-85
+        const uint8_t *p = (const uint8_t *)edges;
+        for (uint64_t i = 0; i < n_edges; i++) {
+            memcpy(&src[i], p + i * rec, 4);
+            memcpy(&dst[i], p + i * rec + 4, 4);
+            if (weighted) memcpy(&w[i], p + i * rec + 8, 4);
+        }
+        build_from_edges(eng->gs, v_num, src.data(), dst.data(), weighted ? w.data() : nullptr, n_edges,
+                         make_undirected != 0, sort_adj != 0, eng->stream);
+        eng->timing.graph_ms = now_ms() - t0;
+        return KKW_OK;
+    });
+}
+
+int kkw_graph_load_file(kkw_engine *eng, uint32_t v_num, const char *path, int text, int weighted,
+                        int make_undirected, int sort_adj)
+{
+    return guarded([&]() -> int {
+        if (!eng || !path) return fail(KKW_E_INVALID, "engine or path is null");
+        FILE *f = fopen(path, text ? "r" : "rb");
+        if (!f) return fail(KKW_E_IO, std::string("cannot open ") + path);
+        std::vector<uint8_t> buf;
+        const size_t rec = weighted ? 12 : 8;
+        uint64_t n = 0;
+        if (!text) {
+            // read_graph (include/storage.hpp:35-55), one partition: the whole file
+            fseek(f, 0, SEEK_END);
+            long sz = ftell(f);
+            fseek(f, 0, SEEK_SET);
+            n = (uint64_t)sz / rec;
+            buf.resize(n * rec);
+            if (n && fread(buf.data(), rec, n, f) != n) {
+                fclose(f);
+                return fail(KKW_E_IO, "short read");
+            }
+        } else {
+            // read_edgelist / read_edge_txt (storage.hpp:110-157): "%u %u[ %f]"
+            unsigned a, b;
+            float x = 0.0f;
+            for (;;) {
+                int r = weighted ? fscanf(f, "%u %u %f", &a, &b, &x) : fscanf(f, "%u %u", &a, &b);
+                if (r == EOF) break;
+                if (r != (weighted ? 3 : 2)) {
+                    // skip the rest of a malformed line
+                    int c;
+                    while ((c = fgetc(f)) != EOF && c != '\n') {
+                    }
+                    if (c == EOF) break;
+                    continue;
+                }
+                size_t o = buf.size();
+                buf.resize(o + rec);
+                memcpy(&buf[o], &a, 4);
+                memcpy(&buf[o + 4], &b, 4);
+                if (weighted) memcpy(&buf[o + 8], &x, 4);
+                n++;
+            }
+        }
+        fclose(f);
+        return kkw_graph_from_edges(eng, v_num, buf.data(), n, weighted, make_undirected, sort_adj);
+    });
+}
+
+int kkw_graph_from_csr(kkw_engine *eng, uint32_t v_num, const uint64_t *row_ptr, const uint32_t *col,
+                       const float *w)
+{
+    return guarded([&]() -> int {
+        if (!eng || !row_ptr || (!col && row_ptr[v_num])) return fail(KKW_E_INVALID, "null argument");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        double t0 = now_ms();
+        for (uint32_t v = 0; v < v_num; v++)
+            if (row_ptr[v + 1] < row_ptr[v]) return fail(KKW_E_INVALID, "row_ptr must be non-decreasing");
+        build_from_csr(eng->gs, v_num, row_ptr, col, w, eng->stream);
+        eng->timing.graph_ms = now_ms() - t0;
+        return KKW_OK;
+    });
+}
+
+int kkw_graph_rmat(kkw_engine *eng, int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                   int weighted)
+{
+    return guarded([&]() -> int {
+        if (!eng) return fail(KKW_E_INVALID, "engine is null");
+        if (!(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0))
+            return fail(KKW_E_INVALID, "R-MAT probabilities must be non-negative with a+b+c <= 1");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        double t0 = now_ms();
+        auto thr = [](double x) -> uint32_t {
+            double v = std::floor(x * 4294967296.0);
+            return v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)v;
+        };
+        build_rmat(eng->gs, scale, edge_factor, thr(a), thr(a + b), thr(a + b + c), seed, weighted != 0,
+                   eng->stream);
+        eng->timing.graph_ms = now_ms() - t0;
+        return KKW_OK;
+    });
+}
+
+int kkw_graph_set_edge_types(kkw_engine *eng, const uint8_t *etype)
+{
+    return guarded([&]() -> int {
+        if (!eng || !etype) return fail(KKW_E_INVALID, "null argument");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        set_edge_types(eng->gs, etype, eng->stream);
+        return KKW_OK;
+    });
+}
+
+int kkw_graph_type_by_dst(kkw_engine *eng, int ntypes, uint64_t seed)
+{
+    return guarded([&]() -> int {
+        if (!eng) return fail(KKW_E_INVALID, "engine is null");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        type_by_dst(eng->gs, ntypes, seed, eng->stream);
+        return KKW_OK;
+    });
+}
+
+int kkw_graph_info(kkw_engine *eng, uint32_t *v_num, uint64_t *e_num)
+{
+    return guarded([&]() -> int {
+        if (!eng) return fail(KKW_E_INVALID, "engine is null");
+        if (!eng->gs.row_ptr) return fail(KKW_E_NOGRAPH, "no graph loaded");
+        if (v_num) *v_num = eng->gs.V;
+        if (e_num) *e_num = eng->gs.E;
+        return KKW_OK;
+    });
+}
+
+int kkw_graph_download(kkw_engine *eng, uint64_t *row_ptr, uint32_t *col, float *w, uint8_t *etype)
+{
+    return guarded([&]() -> int {
+        if (!eng) return fail(KKW_E_INVALID, "engine is null");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        GraphStore &gs = eng->gs;
+        if (!gs.row_ptr) return fail(KKW_E_NOGRAPH, "no graph loaded");
+        KKW_HIP(hipSetDevice(eng->device));
+        if (row_ptr) KKW_HIP(hipMemcpy(row_ptr, gs.row_ptr, ((size_t)gs.V + 1) * 8, hipMemcpyDeviceToHost));
+        if (col && gs.E) KKW_HIP(hipMemcpy(col, gs.col, gs.E * 4, hipMemcpyDeviceToHost));
+        if (w) {
+            if (!gs.w) return fail(KKW_E_INVALID, "graph has no weights");
+            if (gs.E) KKW_HIP(hipMemcpy(w, gs.w, gs.E * 4, hipMemcpyDeviceToHost));
+        }
+        if (etype) {
+            if (!gs.etype) return fail(KKW_E_INVALID, "graph has no edge types");
+            if (gs.E) KKW_HIP(hipMemcpy(etype, gs.etype, gs.E, hipMemcpyDeviceToHost));
+        }
+        return KKW_OK;
+    });
+}
+
+int kkw_prepare(kkw_engine *eng)
+{
+    return guarded([&]() -> int {
+        if (!eng) return fail(KKW_E_INVALID, "engine is null");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        double t0 = now_ms();
+        build_alias(eng->gs, eng->stream);
+        eng->timing.alias_ms = now_ms() - t0;
+        return KKW_OK;
+    });
+}
+
+int kkw_alias_download(kkw_engine *eng, float *p, uint32_t *primary, uint32_t *alias, uint32_t *meta, float *vsum)
+{
+    return guarded([&]() -> int {
+        if (!eng) return fail(KKW_E_INVALID, "engine is null");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        GraphStore &gs = eng->gs;
+        if (!gs.bucket) return fail(KKW_E_NOGRAPH, "alias tables not prepared");
+        KKW_HIP(hipSetDevice(eng->device));
+        std::vector<Bucket> h(gs.E);
+        if (gs.E) KKW_HIP(hipMemcpy(h.data(), gs.bucket, gs.E * sizeof(Bucket), hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < gs.E; i++) {
+            if (p) p[i] = h[i].p;
+            if (primary) primary[i] = h[i].primary;
+            if (alias) alias[i] = h[i].alias;
+            if (meta) meta[i] = h[i].meta;
+        }
+        if (vsum) KKW_HIP(hipMemcpy(vsum, gs.vsum, (size_t)gs.V * 4, hipMemcpyDeviceToHost));
+        return KKW_OK;
+    });
+}
+
+int kkw_batch_create(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mode, int batch_flags, kkw_batch **out)
+{
+    return guarded([&]() -> int {
+        if (!eng || !out) return fail(KKW_E_INVALID, "null argument");
+        *out = nullptr;
+        if (mode != KKW_MODE_PLAIN && mode != KKW_MODE_PERSISTENT) return fail(KKW_E_INVALID, "unknown mode");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        auto b = std::make_unique<kkw_batch>();
+        b->eng = eng;
+        b->stream = eng->stream;
+        b->mode = mode;
+        b->stats = (batch_flags & KKW_BATCH_STATS) != 0;
+        b->use_graph = (batch_flags & KKW_BATCH_GRAPH) != 0 && !b->stats;
+        batch_setup(b.get(), jobs, n_jobs);
+        *out = b.release();
+        return KKW_OK;
+    });
+}
+
+int kkw_batch_set_shape(kkw_batch *b, int block, int blocks_per_cu, int chunk)
+{
+    return guarded([&]() -> int {
+        if (!b) return fail(KKW_E_INVALID, "batch is null");
+        if (block < 64 || block > 256 || block % 64) return fail(KKW_E_INVALID, "block must be 64..256, multiple of 64");
+        if (blocks_per_cu < 1 || blocks_per_cu > 64) return fail(KKW_E_INVALID, "blocks_per_cu must be 1..64");
+        if (chunk < 1 || chunk > 65536) return fail(KKW_E_INVALID, "chunk must be 1..65536");
+        std::lock_guard<std::recursive_mutex> lk(b->eng->mu);
+        KKW_HIP(hipSetDevice(b->eng->device));
+        bool rechunk = chunk != b->chunk;
+        bool reshape = rechunk || block != b->block || blocks_per_cu != b->blocks_per_cu;
+        b->block = block;
+        b->blocks_per_cu = blocks_per_cu;
+        b->chunk = chunk;
+        if (rechunk) batch_rechunk(b);
+        else if (reshape && b->gexec) {
+            (void)hipGraphExecDestroy(b->gexec);
+            b->gexec = nullptr;
+            (void)hipGraphDestroy(b->graph);
+            b->graph = nullptr;
+        }
+        return KKW_OK;
+    });
+}
+
+int kkw_batch_launch(kkw_batch *b)
+{
+    return guarded([&]() -> int {
+        if (!b) return fail(KKW_E_INVALID, "batch is null");
+        std::lock_guard<std::recursive_mutex> lk(b->eng->mu);
+        KKW_HIP(hipSetDevice(b->eng->device));
+        batch_launch(b);
+        return KKW_OK;
+    });
+}
+
+int kkw_batch_sync(kkw_batch *b, double *kernel_ms)
+{
+    return guarded([&]() -> int {
+        if (!b) return fail(KKW_E_INVALID, "batch is null");
+        std::lock_guard<std::recursive_mutex> lk(b->eng->mu);
+        KKW_HIP(hipSetDevice(b->eng->device));
+        double ms = batch_sync(b);
+        if (kernel_ms) *kernel_ms = ms;
+        return KKW_OK;
+    });
+}
+
+int kkw_batch_paths(kkw_batch *b, int job, int out_where, kkw_paths *out)
+{
+    return guarded([&]() -> int {
+        if (!b || !out) return fail(KKW_E_INVALID, "null argument");
+        if (job < 0 || job >= (int)b->jobs.size()) return fail(KKW_E_INVALID, "job index out of range");
+        std::lock_guard<std::recursive_mutex> lk(b->eng->mu);
+        KKW_HIP(hipSetDevice(b->eng->device));
+        KKW_HIP(hipStreamSynchronize(b->stream));
+        if (out_where == KKW_OUT_DEVICE) fill_paths_device(b->jobs[job], b->stream, out);
+        else fill_paths_host(b->jobs[job], b->stream, out);
+        return KKW_OK;
+    });
+}
+
+int kkw_batch_destroy(kkw_batch *b)
+{
+    if (!b) return KKW_OK;
+    std::lock_guard<std::recursive_mutex> lk(b->eng->mu);
+    (void)hipSetDevice(b->eng->device);
+    (void)hipStreamSynchronize(b->stream);
+    delete b;
+    return KKW_OK;
+}
+
+int kkw_run(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mode, int out_where, int flags, kkw_paths *out)
+{
+    return guarded([&]() -> int {
+        if (!eng || (n_jobs > 0 && (!jobs || !out))) return fail(KKW_E_INVALID, "null argument");
+        if (out_where == KKW_OUT_DEVICE) return fail(KKW_E_INVALID, "kkw_run returns host paths; use a batch for device paths");
+        kkw_batch *b = nullptr;
+        int rc = kkw_batch_create(eng, jobs, n_jobs, mode, (flags & KKW_FLAG_STATS) ? KKW_BATCH_STATS : 0, &b);
+        if (rc) return rc;
+        std::unique_ptr<kkw_batch, int (*)(kkw_batch *)> hold(b, kkw_batch_destroy);
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        batch_launch(b);
+        batch_sync(b);
+        for (int i = 0; i < n_jobs; i++) fill_paths_host(b->jobs[i], b->stream, &out[i]);
+        return KKW_OK;
+    });
+}
+
+int kkw_paths_free(kkw_paths *p)
+{
+    if (!p) return KKW_OK;
+    if (p->priv && !p->on_device) {
+        free(p->vertices);
+        free(p->lengths);
+    }
+    memset(p, 0, sizeof(*p));
+    return KKW_OK;
+}
+
+int kkw_paths_dump(const kkw_paths *p, const char *file, const char *mode, int with_head_info)
+{
+    return guarded([&]() -> int {
+        if (!p || !file || !mode) return fail(KKW_E_INVALID, "null argument");
+        if (p->on_device) return fail(KKW_E_INVALID, "paths are on the device");
+        FILE *f = fopen(file, mode);
+        if (!f) return fail(KKW_E_IO, std::string("cannot open ") + file);
+        // PathSet::dump, include/path.hpp:85-109
+        for (uint32_t i = 0; i < p->walker_num; i++) {
+            uint32_t len = p->lengths[i] & 0x7fffffffu;
+            if (with_head_info) fprintf(f, "%u %u", p->walker_begin + i, len);
+            const uint32_t *row = p->vertices + (size_t)i * p->stride;
+            for (uint32_t k = 0; k < len; k++) fprintf(f, " %u", row[k]);
+            fprintf(f, "\n");
+        }
+        fclose(f);
+        return KKW_OK;
+    });
+}
+
+int kkw_last_timing(kkw_engine *eng, kkw_timing *t)
+{
+    if (!eng || !t) return fail(KKW_E_INVALID, "null argument");
+    *t = eng->timing;
+    return KKW_OK;
+}
+
+int kkw_autotune(kkw_batch *b, const char *cache_file, int *block, int *blocks_per_cu, int *chunk)
+{
+    return guarded([&]() -> int {
+        if (!b) return fail(KKW_E_INVALID, "batch is null");
+        if (b->mode != KKW_MODE_PERSISTENT) return fail(KKW_E_INVALID, "autotune applies to persistent batches");
+        std::string key;
+        {
+            char buf[256];
+            uint64_t walkers = 0;
+            for (auto &pj : b->jobs) walkers += pj.dj.walker_num;
+            snprintf(buf, sizeof buf, "V%u_E%llu_jobs%zu_k%d_w%llu", b->eng->gs.V, (unsigned long long)b->eng->gs.E,
+                     b->jobs.size(), b->jobs.empty() ? -1 : b->jobs[0].dj.kind, (unsigned long long)walkers);
+            key = buf;
+        }
+        if (cache_file) {
+            FILE *f = fopen(cache_file, "r");
+            if (f) {
+                char k[256];
+                int bl, bp, ch;
+                while (fscanf(f, "%255s %d %d %d", k, &bl, &bp, &ch) == 4) {
+                    if (key == k) {
+                        fclose(f);
+                        int rc = kkw_batch_set_shape(b, bl, bp, ch);
+                        if (rc) return rc;
+                        if (block) *block = bl;
+                        if (blocks_per_cu) *blocks_per_cu = bp;
+                        if (chunk) *chunk = ch;
+                        return KKW_OK;
+                    }
+                }
+                fclose(f);
+            }
+        }
+        const int blocks[] = {64, 128, 256};
+        const int bpcs[] = {4, 8, 16, 32};
+        const int chunks[] = {16, 64, 256};
+        double best = 1e300;
+        int bb = b->block, bbp = b->blocks_per_cu, bc = b->chunk;
+        for (int bl : blocks)
+            for (int bp : bpcs) {
+                if (bl * bp > 2048) continue;  // at most 32 waves per CU
+                for (int ch : chunks) {
+                    int rc = kkw_batch_set_shape(b, bl, bp, ch);
+                    if (rc) return rc;
+                    std::lock_guard<std::recursive_mutex> lk(b->eng->mu);
+                    batch_launch(b);
+                    double ms = batch_sync(b);
+                    if (ms < best) {
+                        best = ms;
+                        bb = bl;
+                        bbp = bp;
+                        bc = ch;
+                    }
+                }
+            }
+        int rc = kkw_batch_set_shape(b, bb, bbp, bc);
+        if (rc) return rc;
+        if (cache_file) {
+            FILE *f = fopen(cache_file, "a");
+            if (f) {
+                fprintf(f, "%s %d %d %d\n", key.c_str(), bb, bbp, bc);
+                fclose(f);
+            }
+        }
+        if (block) *block = bb;
+        if (blocks_per_cu) *blocks_per_cu = bbp;
+        if (chunk) *chunk = bc;
+        return KKW_OK;
+    });
+}
+
+int kkw_serve_start(kkw_engine *eng, const kkw_serve_opts *opts)
+{
+    return guarded([&]() -> int {
+        if (!eng) return fail(KKW_E_INVALID, "engine is null");
+        if (eng->serve) return fail(KKW_E_INVALID, "serving already started");
+        auto S = std::make_unique<ServeState>();
+        S->eng = eng;
+        if (opts) S->opts = *opts;
+        if (S->opts.max_walkers_in_flight == 0) S->opts.max_walkers_in_flight = 1u << 22;
+        if (S->opts.queue_capacity == 0) S->opts.queue_capacity = 1u << 16;
+        KKW_HIP(hipSetDevice(eng->device));
+        for (auto &s : S->streams) KKW_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        ServeState *raw = S.get();
+        S->worker = std::thread(serve_loop, raw);
+        eng->serve = std::move(S);
+        return KKW_OK;
+    });
+}
+
+int kkw_submit(kkw_engine *eng, const kkw_job *job, uint64_t *ticket)
+{
+    return guarded([&]() -> int {
+        if (!eng || !job || !ticket) return fail(KKW_E_INVALID, "null argument");
+        ServeState *S = eng->serve.get();
+        if (!S) return fail(KKW_E_INVALID, "serving not started");
+        auto r = std::make_shared<Request>();
+        {
+            std::lock_guard<std::recursive_mutex> lk(eng->mu);
+            KKW_HIP(hipSetDevice(eng->device));
+            r->pj = prepare_job(eng, *job);
+        }
+        // prepare_job keeps host copies; pointers inside src must not be used later
+        r->pj.src.starts = r->pj.starts.empty() ? nullptr : r->pj.starts.data();
+        r->pj.src.scheme_len = r->pj.scheme_len.empty() ? nullptr : r->pj.scheme_len.data();
+        r->pj.src.scheme_mask = r->pj.scheme_mask.empty() ? nullptr : r->pj.scheme_mask.data();
+        r->bytes = request_bytes(r->pj);
+        r->t_submit = now_ms();
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (S->stop) return fail(KKW_E_INVALID, "serving is stopping");
+            if (S->pending.size() >= S->opts.queue_capacity) return fail(KKW_E_BUSY, "serving queue is full");
+            r->ticket = S->next_ticket++;
+            S->pending.push_back(r);
+            S->all[r->ticket] = r;
+            *ticket = r->ticket;
+        }
+        S->cv_work.notify_one();
+        return KKW_OK;
+    });
+}
+
+int kkw_poll(kkw_engine *eng, uint64_t ticket, int *done)
+{
+    return guarded([&]() -> int {
+        if (!eng || !done) return fail(KKW_E_INVALID, "null argument");
+        ServeState *S = eng->serve.get();
+        if (!S) return fail(KKW_E_INVALID, "serving not started");
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->all.find(ticket);
+        if (it == S->all.end()) return fail(KKW_E_INVALID, "unknown ticket");
+        *done = it->second->status != 1;
+        return KKW_OK;
+    });
+}
+
+int kkw_wait(kkw_engine *eng, uint64_t ticket, double timeout_s, kkw_paths *out, double *latency_ms)
+{
+    return guarded([&]() -> int {
+        if (!eng) return fail(KKW_E_INVALID, "engine is null");
+        ServeState *S = eng->serve.get();
+        if (!S) return fail(KKW_E_INVALID, "serving not started");
+        std::unique_lock<std::mutex> lk(S->mu);
+        auto it = S->all.find(ticket);
+        if (it == S->all.end()) return fail(KKW_E_INVALID, "unknown ticket");
+        auto r = it->second;
+        auto pred = [&] { return r->status != 1; };
+        if (timeout_s < 0) S->cv_done.wait(lk, pred);
+        else if (!S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred))
+            return fail(KKW_E_TIMEOUT, "request not finished in time");
+        S->all.erase(ticket);
+        if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+        if (r->status != 0) {
+            kkw_paths_free(&r->result);
+            return fail(r->status, r->status == KKW_E_CANCELLED ? "request cancelled" : "request failed");
+        }
+        if (out) *out = r->result;
+        else kkw_paths_free(&r->result);
+        return KKW_OK;
+    });
+}
+
+int kkw_cancel(kkw_engine *eng, uint64_t ticket)
+{
+    return guarded([&]() -> int {
+        if (!eng) return fail(KKW_E_INVALID, "engine is null");
+        ServeState *S = eng->serve.get();
+        if (!S) return fail(KKW_E_INVALID, "serving not started");
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->all.find(ticket);
+        if (it == S->all.end()) return fail(KKW_E_INVALID, "unknown ticket");
+        it->second->cancelled = true;   // honoured while the request is still pending
+        return KKW_OK;
+    });
+}
+
+int kkw_serve_stop(kkw_engine *eng)
+{
+    if (!eng || !eng->serve) return KKW_OK;
+    ServeState *S = eng->serve.get();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    for (auto &kv : S->all) kkw_paths_free(&kv.second->result);
+    for (auto &s : S->streams)
+        if (s) (void)hipStreamDestroy(s);
+    eng->serve.reset();
+    return KKW_OK;
+}
+
+}  // extern "C"

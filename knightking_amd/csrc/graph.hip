@@ -1,0 +1,586 @@
+// graph.hip -- graph storage on the device: CSR construction (from an edge
+// list, from a CSR, or from a synthetic R-MAT), edge types, vertex type masks
+// and the static-component alias tables.
+//
+// CSR semantics follow KnightKing include/graph.hpp: mirroring of undirected
+// input (430-443), per-source lists in file order (309-330; one partition keeps
+// file order through shuffle_edges, 332-406), optional per-list sort by
+// neighbour as node2vec() does (src/apps/node2vec.hpp:122-125).  Order is made
+// deterministic on the device with stable radix sorts.
+#include "kkw_internal.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <vector>
+
+namespace kkw {
+
+namespace {
+
+template <typename T>
+T *dalloc(size_t n)
+{
+    T *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)&p, n * sizeof(T));
+    if (e != hipSuccess)
+        throw Error(-3, std::string("hipMalloc of ") + std::to_string(n * sizeof(T)) + " bytes: " +
+                            hipGetErrorString(e));
+    return p;
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+struct Temp {
+    void *p = nullptr;
+    size_t n = 0;
+    void ensure(size_t bytes)
+    {
+        if (bytes <= n) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        KKW_HIP(hipMalloc(&p, bytes ? bytes : 1));
+        n = bytes;
+    }
+    ~Temp()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+int bits_for(uint64_t v)  // bits needed for values < v
+{
+    int b = 0;
+    while (b < 64 && (1ull << b) < v) b++;
+    return b ? b : 1;
+}
+
+// ---------------------------------------------------------------- R-MAT ----
+// Edge i, levels 4g..4g+3 take the words of Philox(key seed, ctr (i_lo, i_hi,
+// g, TAG_RMAT)); level l sets bit (scale-1-l).  Key = min << scale | max.
+__global__ void k_rmat_keys(uint64_t n, int scale, uint32_t ta, uint32_t tab, uint32_t tabc,
+                            uint64_t seed, uint64_t *keys)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t s = 0, d = 0;
+    for (int g = 0; g * 4 < scale; g++) {
+        u32x4 r = philox((uint32_t)i, (uint32_t)(i >> 32), (uint32_t)g, TAG_RMAT, (uint32_t)seed,
+                         (uint32_t)(seed >> 32));
+        uint32_t w4[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            int l = g * 4 + j;
+            if (l >= scale) break;
+            uint32_t bit = 1u << (scale - 1 - l);
+            uint32_t u = w4[j];
+            if (u < ta) {
+            } else if (u < tab) {
+                d |= bit;
+            } else if (u < tabc) {
+                s |= bit;
+            } else {
+                s |= bit;
+                d |= bit;
+            }
+        }
+    }
+    uint32_t a = s < d ? s : d, b = s < d ? d : s;
+    keys[i] = ((uint64_t)a << scale) | b;
+}
+
+// keep the first of each run of equal keys, drop self loops (min == max)
+__global__ void k_unique_flags(const uint64_t *keys, uint64_t n, int scale, uint8_t *flags)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = keys[i];
+    uint64_t mask = (1ull << scale) - 1;
+    bool self = (k >> scale) == (k & mask);
+    flags[i] = (!self && (i == 0 || keys[i - 1] != k)) ? 1 : 0;
+}
+
+__global__ void k_mirror(const uint64_t *keys, uint64_t u, int scale, uint64_t *out)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= u) return;
+    uint64_t k = keys[i];
+    uint64_t mask = (1ull << scale) - 1;
+    out[2 * i] = k;
+    out[2 * i + 1] = ((k & mask) << scale) | (k >> scale);
+}
+
+// row_ptr[v] = first position whose source is >= v (keys sorted)
+__global__ void k_rowptr_from_sorted(const uint64_t *keys, uint64_t n, int shift, uint32_t V,
+                                     uint64_t *row_ptr)
+{
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > V) return;
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if ((keys[mid] >> shift) < v) lo = mid + 1;
+        else hi = mid;
+    }
+    row_ptr[v] = lo;
+}
+
+__global__ void k_col_weight_from_keys(const uint64_t *keys, uint64_t n, int scale, uint64_t seed,
+                                       uint32_t *col, float *w)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = keys[i];
+    uint64_t mask = (1ull << scale) - 1;
+    uint32_t s = (uint32_t)(k >> scale), d = (uint32_t)(k & mask);
+    col[i] = d;
+    if (w) {
+        uint32_t a = s < d ? s : d, b = s < d ? d : s;
+        u32x4 r = philox(a, b, 0, TAG_WEIGHT, (uint32_t)seed, (uint32_t)(seed >> 32));
+        w[i] = 1.0f + 4.0f * u01(r.x);
+    }
+}
+
+// ------------------------------------------------------ edges -> CSR ----
+// Entry j of the (possibly mirrored) list: j = 2i is edge i as read, j = 2i+1
+// its reverse (graph.hpp:430-443).  Sort key = src (file order) or
+// src << vb | dst (sorted lists); value = j, so the stable sort keeps file
+// order among equal keys.
+__global__ void k_entry_keys(const uint32_t *src, const uint32_t *dst, uint64_t n, int undirected,
+                             int sort_adj, int vb, uint32_t V, uint64_t *keys, uint64_t *vals,
+                             unsigned long long *bad)
+{
+    uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t m = undirected ? 2 * n : n;
+    if (j >= m) return;
+    uint64_t i = undirected ? (j >> 1) : j;
+    bool rev = undirected && (j & 1);
+    uint32_t s = rev ? dst[i] : src[i];
+    uint32_t d = rev ? src[i] : dst[i];
+    if (s >= V || d >= V) {
+        atomicAdd(bad, 1ull);
+        s = 0;
+        d = 0;
+    }
+    keys[j] = sort_adj ? (((uint64_t)s << vb) | d) : (uint64_t)s;
+    vals[j] = j;
+}
+
+__global__ void k_gather_entries(const uint64_t *perm, uint64_t m, const uint32_t *src,
+                                 const uint32_t *dst, const float *w, int undirected, uint32_t *col,
+                                 float *wout, uint32_t *deg_src_out)
+{
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    uint64_t j = perm[k];
+    uint64_t i = undirected ? (j >> 1) : j;
+    bool rev = undirected && (j & 1);
+    col[k] = rev ? src[i] : dst[i];
+    if (wout) wout[k] = w[i];
+    deg_src_out[k] = rev ? dst[i] : src[i];
+}
+
+__global__ void k_rowptr_from_src(const uint32_t *srcs, uint64_t m, uint32_t V, uint64_t *row_ptr)
+{
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > V) return;
+    uint64_t lo = 0, hi = m;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (srcs[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    row_ptr[v] = lo;
+}
+
+// ------------------------------------------------------------- checks ----
+__global__ void k_check_csr(const uint64_t *row_ptr, const uint32_t *col, const float *w, uint32_t V,
+                            uint64_t E, unsigned long long *bad, unsigned int *sorted_flag)
+{
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    uint64_t b = row_ptr[v], e = row_ptr[v + 1];
+    if (b > e || e > E) {
+        atomicAdd(bad, 1ull);
+        return;
+    }
+    for (uint64_t k = b; k < e; k++) {
+        if (col[k] >= V) atomicAdd(bad, 1ull);
+        if (w && !(w[k] >= 0.0f && w[k] < 3.0e38f)) atomicAdd(bad, 1ull);
+        if (k > b && col[k] < col[k - 1]) atomicAnd(sorted_flag, 0u);
+    }
+}
+
+// -------------------------------------------------------------- types ----
+__global__ void k_vertex_types(uint32_t V, int ntypes, uint64_t seed, uint8_t *vtype)
+{
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    u32x4 r = philox((uint32_t)v, 0, 0, TAG_VTYPE, (uint32_t)seed, (uint32_t)(seed >> 32));
+    vtype[v] = (uint8_t)(((uint64_t)r.x * (uint32_t)ntypes) >> 32);
+}
+
+__global__ void k_etype_by_dst(const uint32_t *col, uint64_t E, const uint8_t *vtype, uint8_t *etype)
+{
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= E) return;
+    etype[k] = vtype[col[k]];
+}
+
+// vertex_masks[v] = OR of 1 << meta(e) over out-edges (src/apps/metapath.hpp:77-88)
+__global__ void k_vmask(const uint64_t *row_ptr, const uint8_t *etype, uint32_t V, uint8_t *vmask,
+                        unsigned long long *bad)
+{
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    uint32_t m = 0;
+    for (uint64_t k = row_ptr[v]; k < row_ptr[v + 1]; k++) {
+        uint32_t t = etype[k];
+        if (t >= 8) atomicAdd(bad, 1ull);
+        m |= 1u << (t & 7u);
+    }
+    vmask[v] = (uint8_t)m;
+}
+
+// -------------------------------------------------------------- alias ----
+// include/walk.hpp:538-652, one vertex per lane.  sum: sequential float sum
+// (591-595); ave = sum / d (600); edges split in order into small (< ave) and
+// large FIFO queues (601-615); each small is paired with the next large:
+// bucket {p = prob[s] / ave, primary s, alias l}, prob[l] -= ave - prob[s], l
+// re-queued by its new value (617-634); leftovers get p = 1 and no alias
+// (635-647).  Buckets are stored in creation order.  An element sits in at most
+// one queue and its mass changes only when it is popped as the large partner,
+// so each queue entry carries its current mass with its index: every access
+// to the queues and the weights is then a sequential stream.
+__global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_ptr,
+                                              const uint32_t *__restrict__ col,
+                                              const float *__restrict__ w,
+                                              const uint8_t *__restrict__ etype, uint32_t V,
+                                              Bucket *__restrict__ out, float *__restrict__ vsum,
+                                              uint2 *__restrict__ qs, uint2 *__restrict__ ql)
+{
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const uint64_t b = row_ptr[v];
+    const uint32_t d = (uint32_t)(row_ptr[v + 1] - b);
+    if (d == 0) {
+        vsum[v] = 0.0f;
+        return;
+    }
+    const float *wv = w + b;
+    float sum = 0.0f;
+    for (uint32_t e = 0; e < d; e++) sum += wv[e];
+    vsum[v] = sum;
+    const float ave = sum / (float)d;
+    uint2 *S = qs + b;
+    uint2 *L = ql + b;
+    uint32_t st = 0, lt = 0;
+    for (uint32_t e = 0; e < d; e++) {
+        float p = wv[e];
+        uint2 ent = make_uint2(e, __float_as_uint(p));
+        if (p < ave) S[st++] = ent;
+        else L[lt++] = ent;
+    }
+    const uint32_t *cv = col + b;
+    const uint8_t *tv = etype ? etype + b : nullptr;
+    Bucket *o = out + b;
+    uint32_t sh = 0, lh = 0;
+    while (sh != st) {
+        uint2 se = S[sh++];
+        float ps = __uint_as_float(se.y);
+        Bucket bk;
+        if (lh != lt) {
+            uint2 le = L[lh++];
+            float pl = __uint_as_float(le.y);
+            bk.p = ps / ave;
+            bk.primary = cv[se.x];
+            bk.alias = cv[le.x];
+            bk.meta = tv ? ((uint32_t)tv[se.x] | ((uint32_t)tv[le.x] << 8)) : 0u;
+            float diff = ave - ps;
+            pl -= diff;
+            uint2 ne = make_uint2(le.x, __float_as_uint(pl));
+            if (pl < ave) S[st++] = ne;
+            else L[lt++] = ne;
+        } else {
+            bk.p = 1.0f;
+            bk.primary = cv[se.x];
+            bk.alias = bk.primary;
+            bk.meta = tv ? (uint32_t)tv[se.x] * 0x101u : 0u;
+        }
+        *o++ = bk;
+    }
+    while (lh != lt) {
+        uint2 le = L[lh++];
+        Bucket bk;
+        bk.p = 1.0f;
+        bk.primary = cv[le.x];
+        bk.alias = bk.primary;
+        bk.meta = tv ? (uint32_t)tv[le.x] * 0x101u : 0u;
+        *o++ = bk;
+    }
+}
+
+unsigned long long read_counter(unsigned long long *d, hipStream_t s)
+{
+    unsigned long long h = 0;
+    KKW_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s));
+    KKW_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+void check_graph(GraphStore &gs, hipStream_t s)
+{
+    unsigned long long *bad = dalloc<unsigned long long>(1);
+    unsigned int *sorted = dalloc<unsigned int>(1);
+    KKW_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned long long), s));
+    unsigned int one = 1;
+    KKW_HIP(hipMemcpyAsync(sorted, &one, sizeof(one), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_check_csr, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.col,
+                       gs.w, gs.V, gs.E, bad, sorted);
+    KKW_HIP(hipGetLastError());
+    unsigned long long nbad = read_counter(bad, s);
+    unsigned int hs = 0;
+    KKW_HIP(hipMemcpy(&hs, sorted, sizeof(hs), hipMemcpyDeviceToHost));
+    dfree(bad);
+    dfree(sorted);
+    if (nbad) throw Error(-1, "graph has " + std::to_string(nbad) +
+                                  " invalid entries (vertex id >= v_num, bad row offsets, or a "
+                                  "weight that is negative or not finite)");
+    gs.sorted = hs != 0;
+}
+
+}  // namespace
+
+void GraphStore::release()
+{
+    dfree(row_ptr);
+    dfree(col);
+    dfree(w);
+    dfree(etype);
+    dfree(vmask);
+    dfree(bucket);
+    dfree(vsum);
+    V = 0;
+    E = 0;
+    sorted = false;
+}
+
+DevGraph GraphStore::view() const
+{
+    DevGraph g;
+    g.V = V;
+    g.E = E;
+    g.row_ptr = row_ptr;
+    g.col = col;
+    g.w = w;
+    g.etype = etype;
+    g.vmask = vmask;
+    g.bucket = bucket;
+    g.vsum = vsum;
+    return g;
+}
+
+void build_rmat(GraphStore &gs, int scale, int edge_factor, uint32_t thr_a, uint32_t thr_ab,
+                uint32_t thr_abc, uint64_t seed, bool weighted, hipStream_t s)
+{
+    if (scale < 1 || scale > 30 || edge_factor < 1)
+        throw Error(-1, "rmat: scale must be in [1,30] and edge_factor >= 1");
+    gs.release();
+    const uint32_t V = 1u << scale;
+    const uint64_t n = (uint64_t)edge_factor << scale;
+    const int kb = 2 * scale;
+    uint64_t *k0 = dalloc<uint64_t>(n), *k1 = dalloc<uint64_t>(n);
+    hipLaunchKernelGGL(k_rmat_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, n, scale, thr_a, thr_ab,
+                       thr_abc, seed, k0);
+    KKW_HIP(hipGetLastError());
+    Temp tmp;
+    size_t tb = 0;
+    KKW_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, k0, k1, n, 0, kb, s));
+    tmp.ensure(tb);
+    KKW_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, k0, k1, n, 0, kb, s));
+    uint8_t *flags = dalloc<uint8_t>(n);
+    hipLaunchKernelGGL(k_unique_flags, dim3(grid_for(n, 256)), dim3(256), 0, s, k1, n, scale, flags);
+    KKW_HIP(hipGetLastError());
+    unsigned long long *nsel = dalloc<unsigned long long>(1);
+    tb = 0;
+    KKW_HIP(hipcub::DeviceSelect::Flagged(nullptr, tb, k1, flags, k0, nsel, (int64_t)n, s));
+    tmp.ensure(tb);
+    KKW_HIP(hipcub::DeviceSelect::Flagged(tmp.p, tb, k1, flags, k0, nsel, (int64_t)n, s));
+    const uint64_t u = read_counter(nsel, s);
+    dfree(nsel);
+    dfree(flags);
+    dfree(k1);
+    const uint64_t m = 2 * u;
+    uint64_t *m0 = dalloc<uint64_t>(m), *m1 = dalloc<uint64_t>(m);
+    hipLaunchKernelGGL(k_mirror, dim3(grid_for(u, 256)), dim3(256), 0, s, k0, u, scale, m0);
+    KKW_HIP(hipGetLastError());
+    dfree(k0);
+    tb = 0;
+    KKW_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, m0, m1, m, 0, kb, s));
+    tmp.ensure(tb);
+    KKW_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, m0, m1, m, 0, kb, s));
+    dfree(m0);
+    gs.V = V;
+    gs.E = m;
+    gs.row_ptr = dalloc<uint64_t>((uint64_t)V + 1);
+    gs.col = dalloc<uint32_t>(m);
+    gs.w = weighted ? dalloc<float>(m) : nullptr;
+    hipLaunchKernelGGL(k_rowptr_from_sorted, dim3(grid_for((uint64_t)V + 1, 256)), dim3(256), 0, s, m1, m,
+                       scale, V, gs.row_ptr);
+    hipLaunchKernelGGL(k_col_weight_from_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, m1, m, scale, seed,
+                       gs.col, gs.w);
+    KKW_HIP(hipGetLastError());
+    KKW_HIP(hipStreamSynchronize(s));
+    dfree(m1);
+    gs.sorted = true;
+}
+
+void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const uint32_t *h_dst,
+                      const float *h_w, uint64_t n, bool undirected, bool sort_adj, hipStream_t s)
+{
+    if (V == 0) throw Error(-1, "v_num must be positive");
+    gs.release();
+    const uint64_t m = undirected ? 2 * n : n;
+    uint32_t *src = dalloc<uint32_t>(n), *dst = dalloc<uint32_t>(n);
+    float *w = h_w ? dalloc<float>(n) : nullptr;
+    KKW_HIP(hipMemcpyAsync(src, h_src, n * 4, hipMemcpyHostToDevice, s));
+    KKW_HIP(hipMemcpyAsync(dst, h_dst, n * 4, hipMemcpyHostToDevice, s));
+    if (w) KKW_HIP(hipMemcpyAsync(w, h_w, n * 4, hipMemcpyHostToDevice, s));
+    const int vb = bits_for(V);
+    uint64_t *k0 = dalloc<uint64_t>(m), *k1 = dalloc<uint64_t>(m);
+    uint64_t *v0 = dalloc<uint64_t>(m), *v1 = dalloc<uint64_t>(m);
+    unsigned long long *bad = dalloc<unsigned long long>(1);
+    KKW_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned long long), s));
+    if (m) {
+        hipLaunchKernelGGL(k_entry_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, src, dst, n,
+                           undirected ? 1 : 0, sort_adj ? 1 : 0, vb, V, k0, v0, bad);
+        KKW_HIP(hipGetLastError());
+    }
+    if (read_counter(bad, s)) {
+        dfree(bad); dfree(k0); dfree(k1); dfree(v0); dfree(v1); dfree(src); dfree(dst); dfree(w);
+        throw Error(-1, "edge list holds a vertex id >= v_num");
+    }
+    dfree(bad);
+    Temp tmp;
+    if (m) {
+        size_t tb = 0;
+        const int eb = sort_adj ? 2 * vb : vb;
+        KKW_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, m, 0, eb, s));
+        tmp.ensure(tb);
+        KKW_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0, k1, v0, v1, m, 0, eb, s));
+    }
+    dfree(k0);
+    dfree(k1);
+    dfree(v0);
+    gs.V = V;
+    gs.E = m;
+    gs.row_ptr = dalloc<uint64_t>((uint64_t)V + 1);
+    gs.col = dalloc<uint32_t>(m);
+    gs.w = w ? dalloc<float>(m) : nullptr;
+    uint32_t *srcs = dalloc<uint32_t>(m);
+    if (m) {
+        hipLaunchKernelGGL(k_gather_entries, dim3(grid_for(m, 256)), dim3(256), 0, s, v1, m, src, dst, w,
+                           undirected ? 1 : 0, gs.col, gs.w, srcs);
+        KKW_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_rowptr_from_src, dim3(grid_for((uint64_t)V + 1, 256)), dim3(256), 0, s, srcs, m,
+                       V, gs.row_ptr);
+    KKW_HIP(hipGetLastError());
+    KKW_HIP(hipStreamSynchronize(s));
+    dfree(srcs);
+    dfree(v1);
+    dfree(src);
+    dfree(dst);
+    dfree(w);
+    check_graph(gs, s);
+}
+
+void build_from_csr(GraphStore &gs, uint32_t V, const uint64_t *h_row_ptr, const uint32_t *h_col,
+                    const float *h_w, hipStream_t s)
+{
+    if (V == 0) throw Error(-1, "v_num must be positive");
+    if (h_row_ptr[0] != 0) throw Error(-1, "row_ptr[0] must be 0");
+    gs.release();
+    const uint64_t E = h_row_ptr[V];
+    gs.V = V;
+    gs.E = E;
+    gs.row_ptr = dalloc<uint64_t>((uint64_t)V + 1);
+    gs.col = dalloc<uint32_t>(E);
+    gs.w = h_w ? dalloc<float>(E) : nullptr;
+    KKW_HIP(hipMemcpyAsync(gs.row_ptr, h_row_ptr, ((uint64_t)V + 1) * 8, hipMemcpyHostToDevice, s));
+    if (E) KKW_HIP(hipMemcpyAsync(gs.col, h_col, E * 4, hipMemcpyHostToDevice, s));
+    if (gs.w && E) KKW_HIP(hipMemcpyAsync(gs.w, h_w, E * 4, hipMemcpyHostToDevice, s));
+    KKW_HIP(hipStreamSynchronize(s));
+    check_graph(gs, s);
+}
+
+static void rebuild_vmask(GraphStore &gs, hipStream_t s)
+{
+    dfree(gs.vmask);
+    gs.vmask = dalloc<uint8_t>(gs.V);
+    unsigned long long *bad = dalloc<unsigned long long>(1);
+    KKW_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_vmask, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.etype, gs.V,
+                       gs.vmask, bad);
+    KKW_HIP(hipGetLastError());
+    unsigned long long nb = read_counter(bad, s);
+    dfree(bad);
+    if (nb) throw Error(-1, "edge types must be < 8 (one bit each in a u8 mask, as metapath.hpp:30)");
+}
+
+void set_edge_types(GraphStore &gs, const uint8_t *h_etype, hipStream_t s)
+{
+    if (!gs.row_ptr) throw Error(-2, "no graph loaded");
+    dfree(gs.etype);
+    gs.etype = dalloc<uint8_t>(gs.E);
+    if (gs.E) KKW_HIP(hipMemcpyAsync(gs.etype, h_etype, gs.E, hipMemcpyHostToDevice, s));
+    rebuild_vmask(gs, s);
+    dfree(gs.bucket);  // bucket meta carries types: rebuild on next prepare
+    dfree(gs.vsum);
+}
+
+void type_by_dst(GraphStore &gs, int ntypes, uint64_t seed, hipStream_t s)
+{
+    if (!gs.row_ptr) throw Error(-2, "no graph loaded");
+    if (ntypes < 1 || ntypes > 8) throw Error(-1, "ntypes must be in [1,8]");
+    uint8_t *vt = dalloc<uint8_t>(gs.V);
+    hipLaunchKernelGGL(k_vertex_types, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.V, ntypes, seed, vt);
+    dfree(gs.etype);
+    gs.etype = dalloc<uint8_t>(gs.E);
+    if (gs.E)
+        hipLaunchKernelGGL(k_etype_by_dst, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.col, gs.E, vt,
+                           gs.etype);
+    KKW_HIP(hipGetLastError());
+    KKW_HIP(hipStreamSynchronize(s));
+    dfree(vt);
+    rebuild_vmask(gs, s);
+    dfree(gs.bucket);
+    dfree(gs.vsum);
+}
+
+void build_alias(GraphStore &gs, hipStream_t s)
+{
+    if (!gs.row_ptr) throw Error(-2, "no graph loaded");
+    if (!gs.w) throw Error(-2, "graph has no weights: alias tables need a weighted graph");
+    dfree(gs.bucket);
+    dfree(gs.vsum);
+    gs.bucket = dalloc<Bucket>(gs.E);
+    gs.vsum = dalloc<float>(gs.V);
+    uint2 *qs = dalloc<uint2>(gs.E), *ql = dalloc<uint2>(gs.E);
+    hipLaunchKernelGGL(k_alias, dim3(grid_for(gs.V, 64)), dim3(64), 0, s, gs.row_ptr, gs.col, gs.w,
+                       gs.etype, gs.V, gs.bucket, gs.vsum, qs, ql);
+    KKW_HIP(hipGetLastError());
+    KKW_HIP(hipStreamSynchronize(s));
+    dfree(qs);
+    dfree(ql);
+}
+
+}  // namespace kkw

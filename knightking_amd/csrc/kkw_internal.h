@@ -1,0 +1,63 @@
+// kkw_internal.h -- declarations shared by the translation units of libkkw.
+#pragma once
+
+#include "kkw_dev.h"
+
+#include <hip/hip_runtime.h>
+#include <stdexcept>
+#include <string>
+
+namespace kkw {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define KKW_HIP(call)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw ::kkw::Error(-4, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+// Device-resident CSR and its derived arrays.  Every pointer is a device
+// allocation owned by the store (null when absent).
+struct GraphStore {
+    uint32_t V = 0;
+    uint64_t E = 0;
+    uint64_t *row_ptr = nullptr;
+    uint32_t *col = nullptr;
+    float *w = nullptr;
+    uint8_t *etype = nullptr;
+    uint8_t *vmask = nullptr;
+    Bucket *bucket = nullptr;
+    float *vsum = nullptr;
+    bool sorted = false;
+    void release();
+    DevGraph view() const;
+};
+
+// graph.hip
+void build_rmat(GraphStore &gs, int scale, int edge_factor, uint32_t thr_a, uint32_t thr_ab,
+                uint32_t thr_abc, uint64_t seed, bool weighted, hipStream_t s);
+void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const uint32_t *h_dst,
+                      const float *h_w, uint64_t n, bool undirected, bool sort_adj, hipStream_t s);
+void build_from_csr(GraphStore &gs, uint32_t V, const uint64_t *h_row_ptr, const uint32_t *h_col,
+                    const float *h_w, hipStream_t s);
+void set_edge_types(GraphStore &gs, const uint8_t *h_etype, hipStream_t s);
+void type_by_dst(GraphStore &gs, int ntypes, uint64_t seed, hipStream_t s);
+void build_alias(GraphStore &gs, hipStream_t s);
+
+// walk.hip
+void launch_walk_plain(const DevGraph &g, const DevJob &J, bool stats, DevStats *st, int block,
+                       hipStream_t stream);
+void launch_walk_persistent_single(const DevGraph &g, const DevJob &J, uint64_t n_chunks,
+                                   uint32_t chunk, unsigned long long *queue, bool stats,
+                                   DevStats *st, int grid, int block, hipStream_t stream);
+void launch_walk_persistent_mixed(const DevGraph &g, const DevJob *jobs, int n_jobs, uint64_t n_chunks,
+                                  uint32_t chunk, unsigned long long *queue, bool stats, DevStats *st,
+                                  int grid, int block, hipStream_t stream);
+void launch_fill_u64(unsigned long long *p, unsigned long long v, int n, hipStream_t stream);
+
+}  // namespace kkw

@@ -1,0 +1,60 @@
+"""The C-ABI library builds, loads and exports every entry point include/kkw.h
+declares; without a GPU it refuses to create an engine (there is no CPU path)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import knightking_amd as kk
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "kkw.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(kkw_[a-z_]+)\s*\(", text)))
+
+
+def test_header_declares_the_python_mirror():
+    assert declared_symbols() == sorted(kk.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = kk.load_library()
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+    assert lib.kkw_abi_version() == 1
+
+
+def test_no_gpu_means_no_engine():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(kk.KkwError) as e:
+        kk.Engine()
+    assert e.value.code == kk.KKW_E_DEVICE
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors of the POD structs have the sizes and offsets gcc gives the header's."""
+    import subprocess
+    fields = {"kkw_job": kk._Job, "kkw_paths": kk._Paths, "kkw_engine_opts": kk._Opts,
+              "kkw_timing": kk._Timing, "kkw_serve_opts": kk._ServeOpts}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "kkw.h"', "int main(void){"]
+    for cname, py in fields.items():
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = []
+    for cname, py in fields.items():
+        want.append(ctypes.sizeof(py))
+        want += [getattr(py, f).offset for f, _ in py._fields_]
+    assert got == want

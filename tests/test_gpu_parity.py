@@ -1,0 +1,226 @@
+"""GPU parity: the HIP path through the C ABI against the oracle, bit for bit.
+
+Graph construction (R-MAT, edge lists), alias tables and every walk kind in
+both launch forms are compared with the CPU restatement on the same seeded
+inputs.  Larger sizes are covered by size-independent properties.
+"""
+import numpy as np
+import pytest
+
+import knightking_amd as kk
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_paths_equal(gpu, o_paths, o_lens, o_trunc, what=""):
+    assert np.array_equal(gpu.lengths, o_lens), f"{what}: lengths differ"
+    assert np.array_equal(gpu.truncated, o_trunc), f"{what}: truncation flags differ"
+    n, stride = o_paths.shape
+    assert gpu.vertices.shape == (n, stride)
+    mask = np.arange(stride)[None, :] < o_lens[:, None]
+    bad = np.nonzero((gpu.vertices != o_paths) & mask)
+    assert bad[0].size == 0, f"{what}: {bad[0].size} path entries differ, first at walker {bad[0][:1]}"
+
+
+@pytest.fixture(scope="module")
+def rmat10(engine):
+    g = O.rmat_graph(10, 16, seed=1, weighted=True)
+    return O.Graph(*g)
+
+
+def load_rmat(engine, scale, seed=1, weighted=True):
+    print(f"[gpu] R-MAT scale {scale} seed {seed} weighted={weighted}", flush=True)
+    engine.graph_rmat(scale, 16, seed=seed, weighted=weighted)
+
+
+@pytest.mark.parametrize("scale", [6, 10, 14])
+def test_rmat_csr_matches_oracle(engine, scale):
+    load_rmat(engine, scale)
+    rp, col, w, _ = engine.download()
+    rp_o, col_o, w_o = O.rmat_graph(scale, 16, seed=1, weighted=True)
+    assert np.array_equal(rp, rp_o)
+    assert np.array_equal(col, col_o)
+    assert np.array_equal(w.view(np.uint32), w_o.view(np.uint32))
+
+
+@pytest.mark.parametrize("undirected,sort_adj", [(False, False), (False, True), (True, False), (True, True)])
+def test_edges_csr_matches_oracle(engine, karate, undirected, sort_adj):
+    src, dst, w = karate
+    if undirected:  # keep one direction of each pair, let the loader mirror it
+        keep = src < dst
+        src, dst, w = src[keep], dst[keep], w[keep]
+    print(f"[gpu] karate CSR undirected={undirected} sort_adj={sort_adj}", flush=True)
+    engine.graph_from_edges(34, src, dst, w, make_undirected=undirected, sort_adj=sort_adj)
+    rp, col, wd, _ = engine.download()
+    rp_o, col_o, w_o = O.csr_from_edges(34, src, dst, w, undirected=undirected, sort_adj=sort_adj)
+    assert np.array_equal(rp, rp_o) and np.array_equal(col, col_o) and np.array_equal(wd, w_o)
+
+
+def test_edges_csr_rejects_bad_vertex(engine):
+    with pytest.raises(kk.KkwError):
+        engine.graph_from_edges(4, np.array([0, 1], np.uint32), np.array([1, 9], np.uint32))
+
+
+@pytest.mark.parametrize("scale", [10, 14])
+def test_alias_matches_oracle(engine, scale):
+    load_rmat(engine, scale)
+    engine.type_by_dst(5, 3)
+    engine.prepare()
+    p, prim, al, meta, vs = engine.alias_download()
+    rp, col, w, et = engine.download(etype=True)
+    p_o, prim_o, al_o, meta_o, vs_o = O.alias_build(rp, col, w, et)
+    assert np.array_equal(p.view(np.uint32), p_o.view(np.uint32))
+    assert np.array_equal(prim, prim_o)
+    assert np.array_equal(al, al_o)
+    assert np.array_equal(meta, meta_o)
+    assert np.array_equal(vs.view(np.uint32), vs_o.view(np.uint32))
+
+
+def _walk_both(engine, g, mode, **job):
+    kind = job.pop("kind")
+    n = job.pop("walker_num")
+    L = job.pop("walk_length", 0)
+    weighted = job.pop("use_weights", True)
+    kj = kk.Job(kind, n, L, use_weights=weighted, **job)
+    (gpu,) = engine.run([kj], mode=mode)
+    okw = dict(job)
+    okw.pop("schemes", None)
+    o_paths, o_lens, o_tr, st = O.walk(g, kind, n, L, weighted, schemes=job.get("schemes"), **okw)
+    return gpu, o_paths, o_lens, o_tr
+
+
+WALK_CASES = {
+    "deepwalk_w": dict(kind=kk.DEEPWALK, walker_num=3000, walk_length=40, seed=11),
+    "deepwalk_u": dict(kind=kk.DEEPWALK, walker_num=3000, walk_length=40, seed=12, use_weights=False),
+    "ppr_w": dict(kind=kk.PPR, walker_num=3000, terminate_prob=1 / 80, seed=13),
+    "ppr_list_u": dict(kind=kk.PPR, walker_num=2000, terminate_prob=0.15, seed=14, use_weights=False,
+                       start_mode=kk.START_LIST, starts=np.array([5, 77, 300], np.uint32)),
+    "n2v_w": dict(kind=kk.NODE2VEC, walker_num=3000, walk_length=40, p=2.0, q=0.5, seed=15),
+    "n2v_u": dict(kind=kk.NODE2VEC, walker_num=3000, walk_length=40, p=2.0, q=0.5, seed=16, use_weights=False),
+    "n2v_outlier_w": dict(kind=kk.NODE2VEC, walker_num=2000, walk_length=30, p=0.25, q=2.0, seed=17),
+    "n2v_outlier_u": dict(kind=kk.NODE2VEC, walker_num=2000, walk_length=30, p=0.5, q=4.0, seed=18,
+                          use_weights=False),
+    "n2v_q_small": dict(kind=kk.NODE2VEC, walker_num=2000, walk_length=30, p=1.0, q=0.25, seed=19),
+    "n2v_uniform_start": dict(kind=kk.NODE2VEC, walker_num=2000, walk_length=20, p=4.0, q=1.0, seed=20,
+                              start_mode=kk.START_UNIFORM),
+    "deepwalk_epoch": dict(kind=kk.DEEPWALK, walker_num=1500, walk_length=25, seed=21, walker_begin=1500),
+}
+
+
+@pytest.mark.parametrize("mode", [kk.MODE_PLAIN, kk.MODE_PERSISTENT], ids=["plain", "persistent"])
+@pytest.mark.parametrize("case", sorted(WALK_CASES))
+def test_walk_matches_oracle(engine, rmat10, case, mode):
+    load_rmat(engine, 10)
+    print(f"[gpu] walk {case} mode={mode}", flush=True)
+    gpu, op, ol, ot = _walk_both(engine, rmat10, mode, **dict(WALK_CASES[case]))
+    assert_paths_equal(gpu, op, ol, ot, case)
+
+
+SCHEMES = [[0b00011, 0b11100], [0b10101, 0b01010, 0b11111], [0b00001, 0b00010, 0b00100, 0b01000]]
+
+
+@pytest.mark.parametrize("weighted", [True, False])
+@pytest.mark.parametrize("mode", [kk.MODE_PLAIN, kk.MODE_PERSISTENT], ids=["plain", "persistent"])
+def test_metapath_matches_oracle(engine, weighted, mode):
+    load_rmat(engine, 10)
+    engine.type_by_dst(5, 9)
+    rp, col, w, et = engine.download(etype=True)
+    g = O.Graph(rp, col, w, et)
+    job = dict(kind=kk.METAPATH, walker_num=3000, walk_length=30, seed=23, use_weights=weighted, schemes=SCHEMES)
+    print(f"[gpu] metapath weighted={weighted} mode={mode}", flush=True)
+    gpu, op, ol, ot = _walk_both(engine, g, mode, **job)
+    assert_paths_equal(gpu, op, ol, ot, "metapath")
+
+
+def test_node2vec_rmat14_config(engine):
+    """node2vec p=2 q=0.5 weighted, R-MAT scale 14, one walker per vertex, L=80, seed 1."""
+    load_rmat(engine, 14)
+    rp, col, w = O.rmat_graph(14, 16, seed=1, weighted=True)
+    g = O.Graph(rp, col, w)
+    for mode in (kk.MODE_PLAIN, kk.MODE_PERSISTENT):
+        (gpu,) = engine.run([kk.node2vec_job(16384, 80, 2.0, 0.5, True, seed=1)], mode=mode)
+        op, ol, ot, st = O.walk(g, O.NODE2VEC, 16384, 80, True, p=2.0, q=0.5, seed=1)
+        assert_paths_equal(gpu, op, ol, ot, f"rmat14 mode {mode}")
+        assert gpu.steps == st.steps
+
+
+def test_mixed_batch_equals_single_jobs(engine):
+    """All kinds in one persistent launch give the same paths as each job alone."""
+    load_rmat(engine, 10)
+    engine.type_by_dst(5, 9)
+    jobs = [
+        kk.deepwalk_job(700, 33, True, seed=31),
+        kk.ppr_job(900, 0.05, True, seed=32, start_vertices=np.array([1, 2, 3], np.uint32)),
+        kk.node2vec_job(1100, 25, 0.5, 2.0, True, seed=33),
+        kk.metapath_job(500, 20, SCHEMES, True, seed=34),
+        kk.node2vec_job(300, 15, 2.0, 0.5, False, seed=35),
+        kk.deepwalk_job(1, 10, False, seed=36),
+        kk.deepwalk_job(0, 10, False, seed=37),
+    ]
+    mixed = engine.run(jobs, mode=kk.MODE_PERSISTENT)
+    for j, m in zip(jobs, mixed):
+        (alone,) = engine.run([j], mode=kk.MODE_PLAIN)
+        assert np.array_equal(alone.lengths, m.lengths)
+        mask = np.arange(alone.vertices.shape[1])[None, :] < alone.lengths[:, None]
+        assert np.array_equal(np.where(mask, alone.vertices, 0), np.where(mask, m.vertices, 0))
+
+
+def test_graph_replay_matches(engine):
+    """A batch captured in a hipGraph and replayed gives the same paths every time."""
+    load_rmat(engine, 10)
+    jobs = [kk.node2vec_job(2048, 30, 2.0, 0.5, True, seed=41), kk.ppr_job(1024, 0.1, True, seed=42)]
+    ref = engine.run(jobs, mode=kk.MODE_PLAIN)
+    b = engine.batch(jobs, kk.MODE_PERSISTENT, kk.BATCH_GRAPH)
+    for rep in range(3):
+        b.launch()
+        b.sync()
+        for i in range(2):
+            got = b.paths(i)
+            assert np.array_equal(got.lengths, ref[i].lengths), f"replay {rep} job {i}"
+            mask = np.arange(got.vertices.shape[1])[None, :] < got.lengths[:, None]
+            assert np.array_equal(np.where(mask, got.vertices, 0), np.where(mask, ref[i].vertices, 0))
+    b.close()
+
+
+def test_stats_variant_same_paths(engine):
+    load_rmat(engine, 10)
+    job = kk.node2vec_job(2048, 30, 2.0, 0.5, True, seed=43)
+    (a,) = engine.run([job], mode=kk.MODE_PERSISTENT)
+    (b,) = engine.run([job], mode=kk.MODE_PERSISTENT, stats=True)
+    t = engine.timing()
+    assert np.array_equal(a.lengths, b.lengths)
+    assert t["steps"] == a.steps and t["trials"] >= t["steps"]
+
+
+def test_node2vec_needs_sorted_lists(engine, karate):
+    src, dst, w = (a[::-1].copy() for a in karate)   # file order descending: lists unsorted
+    engine.graph_from_edges(34, src, dst, w, sort_adj=False)
+    with pytest.raises(kk.KkwError):
+        engine.run([kk.node2vec_job(10, 5, 2.0, 0.5)])
+
+
+def test_large_node2vec_properties(engine):
+    """R-MAT scale 20, node2vec p=2 q=0.5 weighted, one walker per vertex, L=80:
+    every path follows CSR edges, non-isolated walkers take exactly 80 steps,
+    isolated ones none; plain and persistent launches agree."""
+    load_rmat(engine, 20)
+    rp, col, _, _ = engine.download(weights=False)
+    V = len(rp) - 1
+    job = kk.node2vec_job(V, 80, 2.0, 0.5, True, seed=5)
+    (pp,) = engine.run([job], mode=kk.MODE_PERSISTENT)
+    deg = np.diff(rp.astype(np.int64))
+    expect = np.where(deg[np.arange(V)] > 0, 81, 1)
+    assert np.array_equal(pp.lengths, expect)
+    # edge check on a sample of walkers: (u,v) present iff binary search finds v in u's list
+    rng = np.random.default_rng(0)
+    sample = rng.choice(np.nonzero(deg > 0)[0], 20000, replace=False)
+    u = pp.vertices[sample, :-1].reshape(-1).astype(np.int64)
+    v = pp.vertices[sample, 1:].reshape(-1).astype(np.int64)
+    key = (np.repeat(np.arange(V, dtype=np.int64), deg) << 32) | col.astype(np.int64)
+    probe = (u << 32) | v
+    pos = np.searchsorted(key, probe)
+    assert np.all(key[np.minimum(pos, len(key) - 1)] == probe)
+    (pl,) = engine.run([job], mode=kk.MODE_PLAIN)
+    assert np.array_equal(pl.lengths, pp.lengths)
+    assert np.array_equal(pl.vertices[sample], pp.vertices[sample])
