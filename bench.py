@@ -1,0 +1,294 @@
+#!/usr/bin/env python3
+"""Benchmark: walker steps/s of node2vec (p=2, q=0.5, weighted) on an undirected
+R-MAT scale-24 graph on one MI355X, persistent work-queue kernel (the value)
+beside the plain one-job-per-launch kernel, plus p50/p99 latency of PPR
+queries through the serving loop.
+
+One "step" of this benchmark is one pass of the hot path over one batch: every
+walker of the job (one per vertex, length 80) walked to the end, graph and
+alias tables resident in HBM.  The CPU baseline is the oracle (the C
+restatement) on the box's host cores, on a strided sample of the same walkers;
+the sampled paths are also compared with the GPU's, bit for bit.
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, published
+LINE = 64                   # bytes charged per data-dependent random access
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(t, walkers, weighted):
+    """Bytes one launch needs: one 64 B line per data-dependent access (row_ptr
+    pair, alias bucket or col entry per trial, each distinct line a binary
+    search touches) plus the exact path and length bytes written."""
+    rows = t["row_loads"]
+    trials = t["trials"]
+    lines = t["search_lines"]
+    path_bytes = 4 * (t["steps"] + walkers) + 4 * walkers
+    return LINE * (rows + trials + lines) + path_bytes
+
+
+def time_batch(b, steps, warmup):
+    for _ in range(warmup):
+        b.launch()
+        b.sync()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.launch()
+        kms.append(b.sync())
+    wall = time.perf_counter() - t0
+    return wall, kms
+
+
+def cpu_baseline(eng, job_args, gpu_batch, V, seconds=15.0):
+    """Oracle on host cores over a strided sample of the walkers; returns
+    (steps/s, threads, sample description, parity summary)."""
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    rp, col, w, _ = eng.download(weights=True)
+    p, prim, al, meta, vs = eng.alias_download()
+    g = O.Graph(rp, col, w, None, (p, prim, al, meta, vs))
+    log(f"cpu baseline: graph and alias tables copied to host in {time.perf_counter() - t0:.1f}s")
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    # calibrate on a small sample, then size the timed sample to ~`seconds`
+    stride = 4096
+    n = V // stride
+    t0 = time.perf_counter()
+    _, _, _, st = O.walk(g, O.NODE2VEC, n, 80, True, p=2.0, q=0.5, seed=job_args["seed"], threads=threads,
+                         walker_stride=stride)
+    dt = max(time.perf_counter() - t0, 1e-3)
+    per_walker = dt / max(n, 1)
+    n = int(min(V, max(1000, seconds / per_walker)))
+    stride = max(1, V // n)
+    n = V // stride
+    t0 = time.perf_counter()
+    paths, lens, tr, st = O.walk(g, O.NODE2VEC, n, 80, True, p=2.0, q=0.5, seed=job_args["seed"],
+                                 threads=threads, walker_stride=stride)
+    dt = time.perf_counter() - t0
+    rate = st.steps / dt
+    # parity of the sample against the GPU's rows
+    gp = gpu_batch.paths(0)
+    rows = np.arange(n, dtype=np.int64) * stride
+    glens = gp.lengths[rows]
+    same_len = np.array_equal(glens, lens)
+    mask = np.arange(81)[None, :] < lens[:, None]
+    same = same_len and bool(np.all((gp.vertices[rows] == paths) | ~mask))
+    sample = (f"node2vec p=2 q=0.5 weighted on the same R-MAT-24 graph, walkers with id = k*{stride} "
+              f"(k < {n}), L=80, {st.steps} steps in {dt:.1f}s, {threads} threads")
+    parity = {"walkers": int(n), "bit_exact": bool(same)}
+    return rate, threads, sample, parity
+
+
+def serve_latency(eng, rp, rates, seconds, seed=7):
+    """Poisson PPR queries (t=0.15, 1e3..1e5 walkers log-uniform, source =
+    source of a uniformly drawn directed edge) at each rate; latency from
+    submit to the query's last path on the host."""
+    import knightking_amd as kk
+    rng = np.random.default_rng(seed)
+    E = int(rp[-1])
+    out = {}
+    eng.serve_start(max_walkers_in_flight=1 << 22)
+    try:
+        for rate in rates:
+            n_q = int(rate * seconds)
+            gaps = rng.exponential(1.0 / rate, n_q)
+            edges = rng.integers(0, E, n_q)
+            srcs = np.searchsorted(rp, edges, side="right") - 1
+            walkers = np.round(10 ** (3 + 2 * rng.random(n_q))).astype(np.int64)
+            tickets = []
+            lat = []
+            steps = [0]
+            done = threading.Event()
+
+            def collect():
+                i = 0
+                while i < n_q:
+                    if i >= len(tickets):
+                        time.sleep(0.0002)
+                        continue
+                    paths, ms = eng.wait(tickets[i])
+                    lat.append(ms)
+                    steps[0] += paths.steps
+                    i += 1
+                done.set()
+
+            th = threading.Thread(target=collect)
+            th.start()
+            t_start = time.perf_counter()
+            t_next = t_start
+            for i in range(n_q):
+                t_next += gaps[i]
+                while time.perf_counter() < t_next:
+                    pass
+                job = kk.ppr_job(int(walkers[i]), 0.15, True, seed=1000 + i,
+                                 start_vertices=np.array([srcs[i]], np.uint32))
+                tickets.append(eng.submit(job))
+            done.wait()
+            th.join()
+            span = time.perf_counter() - t_start
+            lat = np.array(lat)
+            out[str(rate)] = {"queries": n_q, "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                              "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                              "steps_per_s": round(steps[0] / span, 1)}
+            log(f"serving {rate} qps: {out[str(rate)]}")
+    finally:
+        eng.serve_stop()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--unweighted", action="store_true", help="node2vec without edge weights")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--autotune", action="store_true", help="search persistent launch shapes first")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "the multi-GPU design is documented but not built", file=sys.stderr)
+        sys.exit(2)
+
+    import knightking_amd as kk
+
+    weighted = not args.unweighted
+    eng = kk.Engine()
+    t0 = time.perf_counter()
+    eng.graph_rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=1, weighted=True)
+    graph_s = time.perf_counter() - t0
+    V, E = eng.info()
+    log(f"R-MAT-{args.scale}: V={V} E={E} built in {graph_s:.2f}s")
+    alias_s = 0.0
+    if weighted:
+        t0 = time.perf_counter()
+        eng.prepare()
+        alias_s = time.perf_counter() - t0
+        log(f"alias tables in {alias_s:.2f}s")
+
+    job_args = dict(seed=1)
+    job = kk.node2vec_job(V, 80, 2.0, 0.5, weighted, seed=1)
+
+    # instrumented run: exact trial / search / line counts for the byte model
+    sb = eng.batch([job], kk.MODE_PERSISTENT, kk.BATCH_STATS)
+    sb.launch()
+    sb.sync()
+    t = eng.timing()
+    sb.close()
+    bytes_launch = algorithmic_bytes(t, V, weighted)
+    log(f"stats: steps={t['steps']} trials={t['trials']} searches={t['searches']} "
+        f"lines={t['search_lines']} rows={t['row_loads']} bytes/step={bytes_launch / max(t['steps'], 1):.1f}")
+
+    pb = eng.batch([job], kk.MODE_PERSISTENT, kk.BATCH_GRAPH)
+    shape = (256, 8, 64)
+    if args.autotune:
+        shape = pb.autotune(os.path.join(ROOT, "profiles", "autotune_cache.txt"))
+        log(f"autotuned shape block/blocks_per_cu/chunk = {shape}")
+    wall_p, kms_p = time_batch(pb, args.steps, args.warmup)
+    steps_per_launch = pb.steps(0)
+    assert steps_per_launch == t["steps"], "instrumented and timed runs disagree"
+    value = steps_per_launch * args.steps / wall_p
+    kernel_s = float(np.mean(kms_p)) / 1e3
+    log(f"persistent: {wall_p / args.steps * 1e3:.2f} ms/launch wall, kernel {kernel_s * 1e3:.2f} ms, "
+        f"{value / 1e9:.3f} G steps/s")
+
+    plain = None
+    if not args.no_plain:
+        lb = eng.batch([job], kk.MODE_PLAIN, 0)
+        wall_l, kms_l = time_batch(lb, args.steps, args.warmup)
+        assert lb.steps(0) == steps_per_launch
+        plain = steps_per_launch * args.steps / wall_l
+        log(f"plain: {wall_l / args.steps * 1e3:.2f} ms/launch, {plain / 1e9:.3f} G steps/s")
+        lb.close()
+
+    achieved = bytes_launch / kernel_s / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "node2vec_rmat24_hbm_traffic.json")
+    if os.path.exists(tf) and weighted and args.scale == 24:
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    parity = None
+    if not args.no_cpu and weighted:
+        rate, threads, sample, parity = cpu_baseline(eng, job_args, pb, V)
+        cpu = {"value": round(rate, 1), "unit": "walker steps/s", "cores": threads, "kind": "port",
+               "sample": sample}
+        log(f"cpu baseline {rate / 1e6:.2f} M steps/s on {threads} threads; sample parity {parity}")
+    pb.close()
+
+    serving = None
+    if not args.no_serve:
+        rp, _, _, _ = eng.download(weights=False)
+        serving = serve_latency(eng, rp, [200, 1000, 2000], args.serve_seconds)
+
+    line = {
+        "metric": "walker steps/s, node2vec RMAT-24 1 GPU (batched vs plain); p99 PPR query latency",
+        "value": round(value, 1),
+        "unit": "walker steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_p / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32+f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"node2vec p=2 q=0.5 {'weighted' if weighted else 'unweighted'}, undirected R-MAT "
+                        f"scale {args.scale} (a/b/c/d=0.57/0.19/0.19/0.05, edge factor 16, dedup+mirror), "
+                        f"walkers=|V|={V}, L=80, persistent work-queue kernel",
+            "vertices": V, "edges": E, "walk_length": 80, "walkers": V,
+            "steps_per_launch": int(steps_per_launch),
+            "launch_shape": {"block": shape[0], "blocks_per_cu": shape[1], "chunk": shape[2]},
+            "parallelism": "single GPU",
+        },
+        "plain_steps_per_s": None if plain is None else round(plain, 1),
+        "persistent_over_plain": None if plain is None else round(value / plain, 3),
+        "graph_build_s": round(graph_s, 3),
+        "alias_build_s": round(alias_s, 3),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": int(bytes_launch),
+            "algorithmic_bytes_per_step": round(bytes_launch / max(steps_per_launch, 1), 1),
+            "kernel_ms": round(kernel_s * 1e3, 3),
+            "trials_per_step": round(t["trials"] / max(t["steps"], 1), 4),
+            "searches_per_step": round(t["searches"] / max(t["steps"], 1), 4),
+            "search_lines_per_search": round(t["search_lines"] / max(t["searches"], 1), 3),
+        },
+        "cpu_baseline": cpu,
+        "sample_parity": parity,
+        "ppr_serving": serving,
+    }
+    print(json.dumps(line), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -104,7 +104,10 @@ typedef struct kkw_timing {
     double total_ms;         /* host wall time of the last kkw_run / batch launch     */
     double alias_ms;         /* last kkw_prepare                                     */
     double graph_ms;         /* last graph build                                     */
-    uint64_t steps, trials, searches, search_probes, row_loads;  /* KKW_FLAG_STATS   */
+    /* instrumented runs (KKW_FLAG_STATS / KKW_BATCH_STATS): accepted steps, trials,
+     * adjacency searches, elements they read, distinct 64-B lines they read,
+     * row_ptr pairs read */
+    uint64_t steps, trials, searches, search_probes, row_loads, search_lines;
 } kkw_timing;
 
 enum kkw_flags { KKW_FLAG_STATS = 1 };

@@ -65,6 +65,7 @@ class _Timing(C.Structure):
         ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("alias_ms", C.c_double),
         ("graph_ms", C.c_double), ("steps", C.c_uint64), ("trials", C.c_uint64),
         ("searches", C.c_uint64), ("search_probes", C.c_uint64), ("row_loads", C.c_uint64),
+        ("search_lines", C.c_uint64),
     ]
 
 

@@ -296,7 +296,7 @@ static void batch_enqueue(kkw_batch *b)
 {
     DevGraph g = b->eng->gs.view();
     hipStream_t s = b->stream;
-    if (b->stats) launch_fill_u64((unsigned long long *)b->d_stats, 0ull, 5, s);
+    if (b->stats) launch_fill_u64((unsigned long long *)b->d_stats, 0ull, 6, s);
     if (b->mode == KKW_MODE_PLAIN) {
         for (auto &pj : b->jobs) launch_walk_plain(g, pj.dj, b->stats, b->d_stats, b->block, s);
     } else {
@@ -359,6 +359,7 @@ static double batch_sync(kkw_batch *b)
         t.searches = h.searches;
         t.search_probes = h.probes;
         t.row_loads = h.row_loads;
+        t.search_lines = h.search_lines;
     }
     return ms;
 }

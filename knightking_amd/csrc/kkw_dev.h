@@ -104,7 +104,7 @@ struct DevJob {
 };
 
 struct DevStats {
-    unsigned long long steps, trials, searches, probes, row_loads;
+    unsigned long long steps, trials, searches, probes, row_loads, search_lines;
 };
 
 }  // namespace kkw

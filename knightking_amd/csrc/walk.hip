@@ -32,20 +32,27 @@ struct Walker {
 };
 
 struct LaneStats {
-    uint32_t steps, trials, searches, probes, rows;
+    uint32_t steps, trials, searches, probes, rows, lines;
 };
 
 static constexpr uint32_t MAX_TRIALS = 1u << 20;  // per step; guards degenerate inputs
 
+// Membership of x in the sorted list col[lo..hi) (std::binary_search of
+// node2vec.hpp:158-166).  probes counts elements read, lines the distinct
+// 64-byte lines they fall in (instrumented builds only use the counts).
 __device__ __forceinline__ bool adj_contains(const uint32_t *__restrict__ col, uint64_t lo,
-                                             uint64_t hi, uint32_t x, uint32_t &probes)
+                                             uint64_t hi, uint32_t x, uint32_t &probes, uint32_t &lines)
 {
     const uint32_t *base = col + lo;
     uint32_t b = 0, len = (uint32_t)(hi - lo);
+    uintptr_t last = ~(uintptr_t)0;
     while (len > 0) {
         uint32_t half = len >> 1;
         uint32_t v = base[b + half];
         probes++;
+        uintptr_t line = (uintptr_t)(base + b + half) >> 6;
+        lines += line != last;
+        last = line;
         if (v == x) return true;
         if (v < x) {
             b += half + 1;
@@ -150,14 +157,12 @@ __device__ __forceinline__ bool advance(const DevGraph &g, const DevJob &J, Walk
         bool have = false, pre = false;
         if (J.overflow > 0.0f && s.step > 0) {
             // Outlier appendix (walk.hpp:1181-1208, node2vec.hpp:47-106).
-            uint32_t pr = 0;
             const uint32_t *base = g.col + s.beg;
             uint32_t lo = 0, len = deg;
             while (len > 0) {  // lower_bound of prev in the current list
                 uint32_t half = len >> 1;
                 if (base[lo + half] < s.prev) { lo += half + 1; len -= half + 1; }
                 else len = half;
-                pr++;
             }
             if (lo < deg && base[lo] == s.prev) {
                 const bool use_alias = ALIAS_KNOWN ? ALIAS : (J.use_alias != 0);
@@ -195,10 +200,11 @@ __device__ __forceinline__ bool advance(const DevGraph &g, const DevJob &J, Walk
         } else if (dart > J.acc_hi) {
             accept = false;                       // both < dart
         } else {
-            uint32_t pr = 0;
-            bool found = adj_contains(g.col, s.pbeg, s.pend, cand, pr);
+            uint32_t pr = 0, ln = 0;
+            bool found = adj_contains(g.col, s.pbeg, s.pend, cand, pr, ln);
             ls.searches++;
             ls.probes += pr;
+            ls.lines += ln;
             accept = dart <= (found ? 1.0f : J.inv_q);   // node2vec.hpp:187-191
         }
         if (accept) move_to(g, J, s, cand, ls);
@@ -264,9 +270,9 @@ __device__ __forceinline__ void flush_stats(const LaneStats &ls, DevStats *st)
 {
     if (!STATS) return;
     // wave reduction, one atomic per counter per wave
-    unsigned long long v[5] = {ls.steps, ls.trials, ls.searches, ls.probes, ls.rows};
+    unsigned long long v[6] = {ls.steps, ls.trials, ls.searches, ls.probes, ls.rows, ls.lines};
 #pragma unroll
-    for (int k = 0; k < 5; k++) {
+    for (int k = 0; k < 6; k++) {
         unsigned long long x = v[k];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
         v[k] = x;
@@ -277,6 +283,7 @@ __device__ __forceinline__ void flush_stats(const LaneStats &ls, DevStats *st)
         atomicAdd(&st->searches, v[2]);
         atomicAdd(&st->probes, v[3]);
         atomicAdd(&st->row_loads, v[4]);
+        atomicAdd(&st->search_lines, v[5]);
     }
 }
 
@@ -285,7 +292,7 @@ template <int KIND, bool ALIAS, bool STATS>
 __global__ void __launch_bounds__(256) k_walk_plain(DevGraph g, DevJob J, DevStats *st)
 {
     uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    LaneStats ls = {0, 0, 0, 0, 0};
+    LaneStats ls = {0, 0, 0, 0, 0, 0};
     if (idx < J.walker_num) {
         Walker s;
         init_walker<KIND>(g, J, 0, idx, s, ls);
@@ -315,7 +322,7 @@ __global__ void __launch_bounds__(256) k_walk_persistent(DevGraph g, DevJob J0, 
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    LaneStats ls = {0, 0, 0, 0, 0};
+    LaneStats ls = {0, 0, 0, 0, 0, 0};
     Walker s;
     bool active = false;
     // wave-uniform pool of walker rows [pool_next, pool_end) of job pool_job

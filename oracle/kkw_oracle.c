@@ -565,7 +565,7 @@ void orc_walk(const orc_graph *g, const orc_job *job, uint32_t *paths,
         orc_stats st = {0, 0, 0, 0, 0};
         #pragma omp for schedule(dynamic, 64)
         for (int64_t i = 0; i < (int64_t)job->walker_num; i++) {
-            uint32_t wid = job->walker_begin + (uint32_t)i;
+            uint32_t wid = job->walker_begin + (uint32_t)i * (job->walker_stride ? job->walker_stride : 1u);
             uint32_t *path = paths + (uint64_t)i * job->path_stride;
             if (job->kind == ORC_NODE2VEC)
                 walk_node2vec(g, job, wid, path, &lens[i], &st);

@@ -100,6 +100,8 @@ typedef struct orc_job {
     const uint32_t *scheme_len;   /* [n_schemes]                              */
     const uint8_t *scheme_mask;   /* [n_schemes][8] allowed-type bitmask      */
     const uint8_t *vmask;         /* [V] OR of out-edge type bits             */
+    uint32_t walker_stride;       /* row i walks walker id begin + i*stride (0 = 1);
+                                     lets the CPU baseline sample a workload    */
 } orc_job;
 
 typedef struct orc_stats {

@@ -39,7 +39,7 @@ class OrcJob(C.Structure):
         ("n2v_lower", C.c_float), ("n2v_inv_p", C.c_float), ("n2v_inv_q", C.c_float),
         ("n2v_overflow", C.c_float), ("start_mode", C.c_int32), ("n_starts", C.c_uint32),
         ("starts", C.c_void_p), ("n_schemes", C.c_uint32), ("scheme_len", C.c_void_p),
-        ("scheme_mask", C.c_void_p), ("vmask", C.c_void_p),
+        ("scheme_mask", C.c_void_p), ("vmask", C.c_void_p), ("walker_stride", C.c_uint32),
     ]
 
 
@@ -203,7 +203,8 @@ class Graph:
 
 
 def walk(g: Graph, kind, walker_num, walk_length=0, use_weights=True, walker_begin=0, terminate_prob=0.0,
-         p=1.0, q=1.0, seed=1, start_mode=START_EQUAL, starts=None, schemes=None, threads=0):
+         p=1.0, q=1.0, seed=1, start_mode=START_EQUAL, starts=None, schemes=None, threads=0,
+         walker_stride=1):
     """Run one job on the oracle; returns (paths [n, stride], lengths, truncated, stats)."""
     keep = []
     j = OrcJob()
@@ -211,6 +212,7 @@ def walk(g: Graph, kind, walker_num, walk_length=0, use_weights=True, walker_beg
     j.use_alias = int(bool(use_weights))
     j.walker_begin = walker_begin
     j.walker_num = walker_num
+    j.walker_stride = walker_stride
     L = walk_length
     if kind == PPR:
         j.ppr_ext = np.float32(1.0) - np.float32(terminate_prob)
