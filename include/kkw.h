@@ -128,6 +128,15 @@ int kkw_engine_destroy(kkw_engine *eng);
  * node2vec() does before walking (src/apps/node2vec.hpp:122-125). */
 int kkw_graph_from_edges(kkw_engine *eng, uint32_t v_num, const void *edges,
                          uint64_t n_edges, int weighted, int make_undirected, int sort_adj);
+/* The same from structure-of-arrays input, with optional per-edge types (the
+ * meta_info of WeightedMetaData / UnweightedMetaData, src/apps/metascheme.hpp:42-67);
+ * w and etype may be NULL. */
+int kkw_graph_from_arrays(kkw_engine *eng, uint32_t v_num, const uint32_t *src,
+                          const uint32_t *dst, const float *w, const uint8_t *etype,
+                          uint64_t n_edges, int make_undirected, int sort_adj);
+/* Sort every adjacency list by neighbour id in place (stable), as node2vec()
+ * does with process_vertices + std::sort (src/apps/node2vec.hpp:122-125). */
+int kkw_graph_sort_adj(kkw_engine *eng);
 /* Binary edge file (read_graph, include/storage.hpp:35-55) or text edge list
  * (read_edgelist, storage.hpp:124-157: "src dst[ w]" per line). */
 int kkw_graph_load_file(kkw_engine *eng, uint32_t v_num, const char *path, int text,

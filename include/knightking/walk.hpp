@@ -1,0 +1,466 @@
+// walk.hpp -- WalkEngine and its configuration classes, the C++ boundary of
+// KnightKing (include/walk.hpp:31-437, include/graph.hpp:408-530), on top of
+// the C ABI of libkkw (kkw.h).  Link with -lkkw.
+//
+// What carries over unchanged: the class and function names, the config
+// fields and setters, WalkConfig's output file / consumer / epoch rate, the
+// binary and text graph formats, PathSet.
+//
+// What differs: the transition functions run on the GPU, so random_walk()
+// executes the device descriptor that node2vec(), deepwalk(), ppr() and
+// metapath() attach to the transition config.  A config built from arbitrary
+// std::function callbacks carries no descriptor and random_walk() throws
+// std::logic_error for it (closures cannot run on the device, and there is no
+// CPU fallback).  Walks are reproducible: each random_walk() call draws from
+// the Philox stream keyed by the engine seed (set_seed, default 1) plus the
+// call's index on this engine.
+#pragma once
+
+#include <assert.h>
+#include <stdio.h>
+
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../kkw.h"
+#include "path.hpp"
+#include "type.hpp"
+
+template <typename edge_data_t>
+struct AdjUnit {
+    vertex_id_t neighbour;
+    edge_data_t data;
+};
+
+template <>
+struct AdjUnit<EmptyData> {
+    union {
+        vertex_id_t neighbour;
+        EmptyData data;
+    };
+};
+
+template <typename edge_data_t>
+struct AdjList {
+    AdjUnit<edge_data_t> *begin;
+    AdjUnit<edge_data_t> *end;
+};
+
+template <typename walker_data_t>
+struct Walker {
+    walker_id_t id;
+    step_t step;
+    walker_data_t data;
+};
+
+template <>
+struct Walker<EmptyData> {
+    uint32_t id;
+    union {
+        step_t step;
+        EmptyData data;
+    };
+};
+
+template <typename query_data_t>
+struct stateQuery {
+    vertex_id_t src_v;
+    walker_id_t walker_idx;
+    query_data_t data;
+};
+
+template <typename response_data_t>
+struct stateResponse {
+    walker_id_t walker_idx;
+    response_data_t data;
+};
+
+namespace knightking_detail {
+
+inline void check(int rc)
+{
+    if (rc != KKW_OK) throw std::runtime_error(std::string("kkw: ") + kkw_last_error());
+}
+
+// Static weight and edge type of an edge record, per edge data type.
+template <typename T>
+struct edge_traits {
+    static const bool weighted = false, typed = false;
+    static float weight(const T &) { return 1.0f; }
+    static int meta(const T &) { return 0; }
+};
+template <>
+struct edge_traits<real_t> {
+    static const bool weighted = true, typed = false;
+    static float weight(const real_t &x) { return x; }
+    static int meta(const real_t &) { return 0; }
+};
+
+// A device job descriptor attached to a transition config by an app function.
+struct DeviceDescriptor {
+    bool present = false;
+    kkw_job job{};
+    std::vector<uint32_t> starts;
+    std::vector<uint32_t> scheme_len;
+    std::vector<uint8_t> scheme_mask;
+};
+
+struct EqualDist {
+    vertex_id_t v_num;
+    vertex_id_t operator()(walker_id_t w) const { return w % v_num; }
+};
+struct UniformDist {
+    vertex_id_t v_num;
+    vertex_id_t operator()(walker_id_t w) const { return w % v_num; }  // placeholder; drawn on the device
+};
+
+}  // namespace knightking_detail
+
+class WalkConfig {
+public:
+    bool output_file_flag;
+    std::string output_path_prefix;
+    bool print_with_head_info;
+    bool output_consumer_flag;
+    std::function<void(PathSet *)> output_consumer_func;
+    double rate;
+
+    WalkConfig()
+        : output_file_flag(false), print_with_head_info(false), output_consumer_flag(false),
+          output_consumer_func(nullptr), rate(1)
+    {
+    }
+    void set_output_file(const char *prefix, bool with_head_info = false)
+    {
+        assert(prefix != nullptr);
+        output_file_flag = true;
+        output_path_prefix = prefix;
+        print_with_head_info = with_head_info;
+    }
+    void set_output_consumer(std::function<void(PathSet *)> f)
+    {
+        assert(f != nullptr);
+        output_consumer_flag = true;
+        output_consumer_func = f;
+    }
+    void set_walk_rate(double r)
+    {
+        assert(r > 0 && r <= 1);
+        rate = r;
+    }
+};
+
+template <typename edge_data_t, typename walker_data_t>
+class WalkerConfig {
+public:
+    walker_id_t walker_num;
+    std::function<vertex_id_t(walker_id_t)> walker_init_dist_func;
+    std::function<void(Walker<walker_data_t> &, vertex_id_t)> walker_init_state_func;
+    std::function<void(Walker<walker_data_t> &, vertex_id_t, AdjUnit<edge_data_t> *)> walker_update_state_func;
+
+    WalkerConfig() : walker_num(0) {}
+    WalkerConfig(walker_id_t n,
+                 std::function<void(Walker<walker_data_t> &, vertex_id_t)> init_state = nullptr,
+                 std::function<void(Walker<walker_data_t> &, vertex_id_t, AdjUnit<edge_data_t> *)> update = nullptr,
+                 std::function<vertex_id_t(walker_id_t)> init_dist = nullptr)
+    {
+        set_walkers(n, init_state, update, init_dist);
+    }
+    void set_walkers(walker_id_t n,
+                     std::function<void(Walker<walker_data_t> &, vertex_id_t)> init_state = nullptr,
+                     std::function<void(Walker<walker_data_t> &, vertex_id_t, AdjUnit<edge_data_t> *)> update = nullptr,
+                     std::function<vertex_id_t(walker_id_t)> init_dist = nullptr)
+    {
+        walker_num = n;
+        walker_init_state_func = init_state;
+        walker_update_state_func = update;
+        walker_init_dist_func = init_dist;
+    }
+};
+
+template <typename edge_data_t, typename walker_data_t>
+class TransitionConfig {
+public:
+    std::function<real_t(Walker<walker_data_t> &, vertex_id_t)> extension_comp_func;
+    std::function<real_t(vertex_id_t, AdjUnit<edge_data_t> *)> static_comp_func;
+    std::function<real_t(Walker<walker_data_t> &, vertex_id_t, AdjUnit<edge_data_t> *)> dynamic_comp_func;
+    std::function<real_t(vertex_id_t, AdjList<edge_data_t> *)> dcomp_upperbound_func;
+    std::function<real_t(vertex_id_t, AdjList<edge_data_t> *)> dcomp_lowerbound_func;
+    std::function<void(Walker<walker_data_t> &, vertex_id_t, AdjList<edge_data_t> *, real_t &, vertex_id_t &)>
+        outlier_upperbound_func;
+    std::function<AdjUnit<edge_data_t> *(Walker<walker_data_t> &, vertex_id_t, AdjList<edge_data_t> *, vertex_id_t)>
+        outlier_search_func;
+    knightking_detail::DeviceDescriptor device;
+
+    TransitionConfig() {}
+    TransitionConfig(decltype(extension_comp_func) ext, decltype(static_comp_func) st = nullptr,
+                     decltype(dynamic_comp_func) dyn = nullptr, decltype(dcomp_upperbound_func) ub = nullptr,
+                     decltype(dcomp_lowerbound_func) lb = nullptr, decltype(outlier_upperbound_func) oub = nullptr,
+                     decltype(outlier_search_func) os = nullptr)
+    {
+        set_transition(ext, st, dyn, ub, lb, oub, os);
+    }
+    void set_transition(decltype(extension_comp_func) ext, decltype(static_comp_func) st = nullptr,
+                        decltype(dynamic_comp_func) dyn = nullptr, decltype(dcomp_upperbound_func) ub = nullptr,
+                        decltype(dcomp_lowerbound_func) lb = nullptr, decltype(outlier_upperbound_func) oub = nullptr,
+                        decltype(outlier_search_func) os = nullptr)
+    {
+        // validation of include/walk.hpp:251-261
+        if (dyn != nullptr) {
+            assert(ub != nullptr);
+        } else {
+            assert(ub == nullptr && lb == nullptr && oub == nullptr && os == nullptr);
+        }
+        assert((oub == nullptr && os == nullptr) || (oub != nullptr && os != nullptr));
+        extension_comp_func = ext;
+        static_comp_func = st;
+        dynamic_comp_func = dyn;
+        dcomp_upperbound_func = ub;
+        dcomp_lowerbound_func = lb;
+        outlier_upperbound_func = oub;
+        outlier_search_func = os;
+    }
+};
+
+template <typename edge_data_t, typename walker_data_t, typename query_data_t, typename response_data_t>
+class SecondOrderTransitionConfig {
+public:
+    std::function<real_t(Walker<walker_data_t> &, vertex_id_t)> extension_comp_func;
+    std::function<real_t(vertex_id_t, AdjUnit<edge_data_t> *)> static_comp_func;
+    std::function<void(Walker<walker_data_t> &, walker_id_t, vertex_id_t, AdjUnit<edge_data_t> *)> post_query_func;
+    std::function<void(vertex_id_t, stateQuery<query_data_t> &, AdjList<edge_data_t> *)> respond_query_func;
+    std::function<real_t(Walker<walker_data_t> &, stateResponse<response_data_t> &, vertex_id_t,
+                         AdjUnit<edge_data_t> *)>
+        dynamic_comp_func;
+    std::function<real_t(vertex_id_t, AdjList<edge_data_t> *)> dcomp_upperbound_func;
+    std::function<real_t(vertex_id_t, AdjList<edge_data_t> *)> dcomp_lowerbound_func;
+    std::function<void(Walker<walker_data_t> &, vertex_id_t, AdjList<edge_data_t> *, real_t &, vertex_id_t &)>
+        outlier_upperbound_func;
+    std::function<AdjUnit<edge_data_t> *(Walker<walker_data_t> &, vertex_id_t, AdjList<edge_data_t> *, vertex_id_t)>
+        outlier_search_func;
+    knightking_detail::DeviceDescriptor device;
+
+    SecondOrderTransitionConfig() {}
+    SecondOrderTransitionConfig(decltype(extension_comp_func) ext, decltype(static_comp_func) st,
+                                decltype(post_query_func) pq, decltype(respond_query_func) rq,
+                                decltype(dynamic_comp_func) dyn, decltype(dcomp_upperbound_func) ub,
+                                decltype(dcomp_lowerbound_func) lb = nullptr,
+                                decltype(outlier_upperbound_func) oub = nullptr,
+                                decltype(outlier_search_func) os = nullptr)
+    {
+        set_transition(ext, st, pq, rq, dyn, ub, lb, oub, os);
+    }
+    void set_transition(decltype(extension_comp_func) ext, decltype(static_comp_func) st,
+                        decltype(post_query_func) pq, decltype(respond_query_func) rq,
+                        decltype(dynamic_comp_func) dyn, decltype(dcomp_upperbound_func) ub,
+                        decltype(dcomp_lowerbound_func) lb = nullptr, decltype(outlier_upperbound_func) oub = nullptr,
+                        decltype(outlier_search_func) os = nullptr)
+    {
+        // validation of include/walk.hpp:339-351
+        if (dyn != nullptr) {
+            assert(ub != nullptr);
+        } else {
+            assert(ub == nullptr && lb == nullptr && oub == nullptr && os == nullptr);
+        }
+        assert(pq != nullptr && rq != nullptr);
+        assert((oub == nullptr && os == nullptr) || (oub != nullptr && os != nullptr));
+        extension_comp_func = ext;
+        static_comp_func = st;
+        post_query_func = pq;
+        respond_query_func = rq;
+        dynamic_comp_func = dyn;
+        dcomp_upperbound_func = ub;
+        dcomp_lowerbound_func = lb;
+        outlier_upperbound_func = oub;
+        outlier_search_func = os;
+    }
+};
+
+template <typename edge_data_t, typename walker_data_t>
+class WalkEngine {
+    kkw_engine *eng_ = nullptr;
+    vertex_id_t v_num_ = 0;
+    edge_id_t e_num_ = 0;
+    uint64_t seed_ = 1;
+    uint64_t calls_ = 0;
+
+public:
+    typedef knightking_detail::edge_traits<edge_data_t> traits;
+
+    WalkEngine()
+    {
+        kkw_engine_opts o{};
+        knightking_detail::check(kkw_engine_create(&o, &eng_));
+    }
+    ~WalkEngine() { kkw_engine_destroy(eng_); }
+    WalkEngine(const WalkEngine &) = delete;
+    WalkEngine &operator=(const WalkEngine &) = delete;
+
+    kkw_engine *native() { return eng_; }
+    void set_concurrency(int) {}          // one GPU; kept for source compatibility
+    void set_seed(uint64_t s) { seed_ = s; calls_ = 0; }
+    vertex_id_t get_vertex_num() { return v_num_; }
+    edge_id_t get_edge_num() { return e_num_; }
+
+    // GraphEngine::load_graph (include/graph.hpp:408-530), storage.hpp:35-157 formats.
+    void load_graph(vertex_id_t v_num, const char *graph_path, bool load_as_undirected = false,
+                    GraphFormat graph_format = GF_Binary)
+    {
+        std::vector<uint32_t> src, dst;
+        std::vector<float> w;
+        std::vector<uint8_t> et;
+        FILE *f = fopen(graph_path, graph_format == GF_Binary ? "rb" : "r");
+        if (!f) throw std::runtime_error(std::string("cannot open ") + graph_path);
+        if (graph_format == GF_Binary) {
+            Edge<edge_data_t> e;
+            while (fread(&e, sizeof(e), 1, f) == 1) push(e, src, dst, w, et);
+        } else {
+            read_text(f, src, dst, w);
+        }
+        fclose(f);
+        knightking_detail::check(kkw_graph_from_arrays(
+            eng_, v_num, src.data(), dst.data(), traits::weighted ? w.data() : nullptr,
+            traits::typed ? et.data() : nullptr, src.size(), load_as_undirected ? 1 : 0, 0));
+        uint32_t v = 0;
+        uint64_t e = 0;
+        knightking_detail::check(kkw_graph_info(eng_, &v, &e));
+        v_num_ = v;
+        e_num_ = e;
+    }
+
+    void sort_adjacency() { knightking_detail::check(kkw_graph_sort_adj(eng_)); }
+
+    std::function<vertex_id_t(walker_id_t)> get_equal_dist_func()
+    {
+        return knightking_detail::EqualDist{v_num_};
+    }
+    std::function<vertex_id_t(walker_id_t)> get_uniform_dist_func()
+    {
+        return knightking_detail::UniformDist{v_num_};
+    }
+
+    // WalkEngine::random_walk (include/walk.hpp:423-436, epochs 698-878).
+    template <typename transition_config_t>
+    void random_walk(WalkerConfig<edge_data_t, walker_data_t> *walker_config,
+                     transition_config_t *transition_config, WalkConfig *walk_config_param = nullptr)
+    {
+        WalkConfig local;
+        WalkConfig *wc = walk_config_param ? walk_config_param : &local;
+        knightking_detail::DeviceDescriptor &dd = transition_config->device;
+        if (!dd.present)
+            throw std::logic_error("random_walk: this transition config has no device descriptor; "
+                                   "arbitrary std::function callbacks cannot run on the GPU "
+                                   "(use node2vec, deepwalk, ppr or metapath)");
+        kkw_job job = dd.job;
+        job.seed = seed_ + calls_++;
+        const walker_id_t walker_num = walker_config->walker_num;
+        std::vector<uint32_t> host_starts;
+        auto &dist = walker_config->walker_init_dist_func;
+        if (dist && !dist.template target<knightking_detail::EqualDist>() &&
+            !dist.template target<knightking_detail::UniformDist>() && job.start_mode == KKW_START_EQUAL) {
+            // a caller-supplied start distribution: evaluate it per walker id on the host
+            host_starts.resize(walker_num);
+            for (walker_id_t i = 0; i < walker_num; i++) host_starts[i] = dist(i);
+            job.start_mode = KKW_START_LIST;
+        } else if (dist && dist.template target<knightking_detail::UniformDist>()) {
+            job.start_mode = KKW_START_UNIFORM;
+        }
+        if (job.start_mode == KKW_START_LIST) {
+            const std::vector<uint32_t> &s = host_starts.empty() ? dd.starts : host_starts;
+            job.starts = s.data();
+            job.n_starts = (uint32_t)s.size();
+        }
+        job.scheme_len = dd.scheme_len.empty() ? nullptr : dd.scheme_len.data();
+        job.scheme_mask = dd.scheme_mask.empty() ? nullptr : dd.scheme_mask.data();
+        walker_id_t per_iter = (walker_id_t)(walker_num * wc->rate);
+        if (per_iter == 0) per_iter = 1;
+        if (per_iter > walker_num) per_iter = walker_num;
+        walker_id_t remained = walker_num;
+        int iter = 0;
+        while (remained != 0) {
+            walker_id_t begin = walker_num - remained;
+            walker_id_t n = remained < per_iter ? remained : per_iter;
+            remained -= n;
+            job.walker_begin = begin;
+            job.walker_num = n;
+            kkw_paths out{};
+            knightking_detail::check(kkw_run(eng_, &job, 1, KKW_MODE_PERSISTENT, KKW_OUT_HOST, 0, &out));
+            if (wc->output_file_flag || wc->output_consumer_flag) {
+                PathSet *ps = to_pathset(out);
+                if (wc->output_file_flag) {
+                    std::string p = wc->output_path_prefix + ".0";
+                    ps->dump(p.c_str(), iter == 0 ? "w" : "a", wc->print_with_head_info);
+                }
+                if (wc->output_consumer_flag) wc->output_consumer_func(ps);
+                else delete ps;
+            }
+            kkw_paths_free(&out);
+            iter++;
+        }
+    }
+
+private:
+    template <typename T = edge_data_t>
+    void push(const Edge<T> &e, std::vector<uint32_t> &src, std::vector<uint32_t> &dst, std::vector<float> &w,
+              std::vector<uint8_t> &et)
+    {
+        src.push_back(e.src);
+        dst.push_back(e.dst);
+        if (traits::weighted) w.push_back(traits::weight(e.data));
+        if (traits::typed) et.push_back((uint8_t)traits::meta(e.data));
+    }
+    void push(const Edge<EmptyData> &e, std::vector<uint32_t> &src, std::vector<uint32_t> &dst,
+              std::vector<float> &, std::vector<uint8_t> &)
+    {
+        src.push_back(e.src);
+        dst.push_back(e.dst);
+    }
+    void read_text(FILE *f, std::vector<uint32_t> &src, std::vector<uint32_t> &dst, std::vector<float> &w)
+    {
+        // read_edge_txt (include/storage.hpp:110-122): "%u %u" or "%u %u %f"
+        static_assert(std::is_same<edge_data_t, EmptyData>::value || std::is_same<edge_data_t, real_t>::value,
+                      "text edge lists hold unweighted or real-weighted edges only");
+        unsigned a, b;
+        float x;
+        for (;;) {
+            int r = traits::weighted ? fscanf(f, "%u %u %f", &a, &b, &x) : fscanf(f, "%u %u", &a, &b);
+            if (r != (traits::weighted ? 3 : 2)) break;
+            src.push_back(a);
+            dst.push_back(b);
+            if (traits::weighted) w.push_back(x);
+        }
+    }
+    static PathSet *to_pathset(const kkw_paths &p)
+    {
+        PathSet *ps = new PathSet();
+        ps->seg_num = 1;
+        ps->path_set = new vertex_id_t *[1];
+        ps->walker_id = new walker_id_t *[1];
+        ps->path_begin = new vertex_id_t **[1];
+        ps->path_end = new vertex_id_t **[1];
+        ps->path_length = new step_t *[1];
+        ps->path_num = new walker_id_t[1];
+        uint64_t total = 0;
+        for (uint32_t i = 0; i < p.walker_num; i++) total += p.lengths[i] & 0x7fffffffu;
+        ps->path_set[0] = new vertex_id_t[total ? total : 1];
+        ps->walker_id[0] = new walker_id_t[p.walker_num ? p.walker_num : 1];
+        ps->path_begin[0] = new vertex_id_t *[p.walker_num ? p.walker_num : 1];
+        ps->path_end[0] = new vertex_id_t *[p.walker_num ? p.walker_num : 1];
+        ps->path_length[0] = new step_t[p.walker_num ? p.walker_num : 1];
+        ps->path_num[0] = p.walker_num;
+        uint64_t o = 0;
+        for (uint32_t i = 0; i < p.walker_num; i++) {
+            uint32_t len = p.lengths[i] & 0x7fffffffu;
+            memcpy(ps->path_set[0] + o, p.vertices + (uint64_t)i * p.stride, len * sizeof(vertex_id_t));
+            ps->walker_id[0][i] = p.walker_begin + i;
+            ps->path_begin[0][i] = ps->path_set[0] + o;
+            ps->path_end[0][i] = ps->path_set[0] + o + len;
+            ps->path_length[0][i] = len;
+            o += len;
+        }
+        return ps;
+    }
+};

@@ -77,7 +77,7 @@ class _ServeOpts(C.Structure):
 # Every symbol include/kkw.h declares (tests check the library exports them all).
 EXPORTED = [
     "kkw_last_error", "kkw_abi_version", "kkw_engine_create", "kkw_engine_destroy",
-    "kkw_graph_from_edges", "kkw_graph_load_file", "kkw_graph_from_csr", "kkw_graph_rmat",
+    "kkw_graph_from_edges", "kkw_graph_from_arrays", "kkw_graph_sort_adj", "kkw_graph_load_file", "kkw_graph_from_csr", "kkw_graph_rmat",
     "kkw_graph_set_edge_types", "kkw_graph_type_by_dst", "kkw_graph_info", "kkw_graph_download",
     "kkw_prepare", "kkw_alias_download", "kkw_run", "kkw_paths_free", "kkw_paths_dump",
     "kkw_last_timing", "kkw_batch_create", "kkw_batch_set_shape", "kkw_batch_launch",
@@ -104,6 +104,8 @@ def load_library(path: str = LIB_PATH):
         "kkw_engine_create": (I32, [C.POINTER(_Opts), C.POINTER(P)]),
         "kkw_engine_destroy": (I32, [P]),
         "kkw_graph_from_edges": (I32, [P, U32, P, U64, I32, I32, I32]),
+        "kkw_graph_from_arrays": (I32, [P, U32, P, P, P, P, U64, I32, I32]),
+        "kkw_graph_sort_adj": (I32, [P]),
         "kkw_graph_load_file": (I32, [P, U32, C.c_char_p, I32, I32, I32, I32]),
         "kkw_graph_from_csr": (I32, [P, U32, P, P, P]),
         "kkw_graph_rmat": (I32, [P, I32, I32, D, D, D, U64, I32]),
@@ -331,6 +333,17 @@ class Engine:
         rec["s"], rec["d"] = src, dst
         _check(_lib.kkw_graph_from_edges(self.h, v_num, _ptr(rec), n, 0 if w is None else 1,
                                          1 if make_undirected else 0, 1 if sort_adj else 0))
+
+    def graph_from_arrays(self, v_num, src, dst, w=None, etype=None, make_undirected=False, sort_adj=False):
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        w = None if w is None else np.ascontiguousarray(w, np.float32)
+        etype = None if etype is None else np.ascontiguousarray(etype, np.uint8)
+        _check(_lib.kkw_graph_from_arrays(self.h, v_num, _ptr(src), _ptr(dst), _ptr(w), _ptr(etype), len(src),
+                                          int(make_undirected), int(sort_adj)))
+
+    def sort_adj(self):
+        _check(_lib.kkw_graph_sort_adj(self.h))
 
     def load_graph(self, v_num, path, text=False, weighted=False, make_undirected=False, sort_adj=False):
         _check(_lib.kkw_graph_load_file(self.h, v_num, path.encode(), int(text), int(weighted),

@@ -659,9 +659,37 @@ int kkw_graph_from_edges(kkw_engine *eng, uint32_t v_num, const void *edges, uin
             memcpy(&dst[i], p + i * rec + 4, 4);
             if (weighted) memcpy(&w[i], p + i * rec + 8, 4);
         }
-        build_from_edges(eng->gs, v_num, src.data(), dst.data(), weighted ? w.data() : nullptr, n_edges,
-                         make_undirected != 0, sort_adj != 0, eng->stream);
+        build_from_edges(eng->gs, v_num, src.data(), dst.data(), weighted ? w.data() : nullptr, nullptr,
+                         n_edges, make_undirected != 0, sort_adj != 0, eng->stream);
         eng->timing.graph_ms = now_ms() - t0;
+        return KKW_OK;
+    });
+}
+
+int kkw_graph_from_arrays(kkw_engine *eng, uint32_t v_num, const uint32_t *src, const uint32_t *dst,
+                          const float *w, const uint8_t *etype, uint64_t n_edges, int make_undirected,
+                          int sort_adj)
+{
+    return guarded([&]() -> int {
+        if (!eng) return fail(KKW_E_INVALID, "engine is null");
+        if (n_edges && (!src || !dst)) return fail(KKW_E_INVALID, "src/dst is null");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        double t0 = now_ms();
+        build_from_edges(eng->gs, v_num, src, dst, w, etype, n_edges, make_undirected != 0, sort_adj != 0,
+                         eng->stream);
+        eng->timing.graph_ms = now_ms() - t0;
+        return KKW_OK;
+    });
+}
+
+int kkw_graph_sort_adj(kkw_engine *eng)
+{
+    return guarded([&]() -> int {
+        if (!eng) return fail(KKW_E_INVALID, "engine is null");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        sort_adjacency(eng->gs, eng->stream);
         return KKW_OK;
     });
 }

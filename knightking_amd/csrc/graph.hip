@@ -175,8 +175,8 @@ __global__ void k_entry_keys(const uint32_t *src, const uint32_t *dst, uint64_t 
 }
 
 __global__ void k_gather_entries(const uint64_t *perm, uint64_t m, const uint32_t *src,
-                                 const uint32_t *dst, const float *w, int undirected, uint32_t *col,
-                                 float *wout, uint32_t *deg_src_out)
+                                 const uint32_t *dst, const float *w, const uint8_t *et, int undirected,
+                                 uint32_t *col, float *wout, uint8_t *etout, uint32_t *deg_src_out)
 {
     uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
@@ -185,7 +185,36 @@ __global__ void k_gather_entries(const uint64_t *perm, uint64_t m, const uint32_
     bool rev = undirected && (j & 1);
     col[k] = rev ? src[i] : dst[i];
     if (wout) wout[k] = w[i];
+    if (etout) etout[k] = et[i];
     deg_src_out[k] = rev ? dst[i] : src[i];
+}
+
+// source vertex of every CSR entry
+__global__ void k_expand_src(const uint64_t *row_ptr, uint32_t V, uint32_t *srcs)
+{
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    for (uint64_t k = row_ptr[v]; k < row_ptr[v + 1]; k++) srcs[k] = (uint32_t)v;
+}
+
+__global__ void k_sort_keys_from_csr(const uint32_t *srcs, const uint32_t *col, uint64_t E, int vb,
+                                     uint64_t *keys, uint64_t *vals)
+{
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= E) return;
+    keys[k] = ((uint64_t)srcs[k] << vb) | col[k];
+    vals[k] = k;
+}
+
+__global__ void k_permute_csr(const uint64_t *perm, uint64_t E, const uint32_t *col, const float *w,
+                              const uint8_t *et, uint32_t *col2, float *w2, uint8_t *et2)
+{
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= E) return;
+    uint64_t j = perm[k];
+    col2[k] = col[j];
+    if (w2) w2[k] = w[j];
+    if (et2) et2[k] = et[j];
 }
 
 __global__ void k_rowptr_from_src(const uint32_t *srcs, uint64_t m, uint32_t V, uint64_t *row_ptr)
@@ -359,6 +388,8 @@ void check_graph(GraphStore &gs, hipStream_t s)
 
 }  // namespace
 
+static void rebuild_vmask(GraphStore &gs, hipStream_t s);
+
 void GraphStore::release()
 {
     dfree(row_ptr);
@@ -444,16 +475,19 @@ void build_rmat(GraphStore &gs, int scale, int edge_factor, uint32_t thr_a, uint
 }
 
 void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const uint32_t *h_dst,
-                      const float *h_w, uint64_t n, bool undirected, bool sort_adj, hipStream_t s)
+                      const float *h_w, const uint8_t *h_et, uint64_t n, bool undirected, bool sort_adj,
+                      hipStream_t s)
 {
     if (V == 0) throw Error(-1, "v_num must be positive");
     gs.release();
     const uint64_t m = undirected ? 2 * n : n;
     uint32_t *src = dalloc<uint32_t>(n), *dst = dalloc<uint32_t>(n);
     float *w = h_w ? dalloc<float>(n) : nullptr;
+    uint8_t *et = h_et ? dalloc<uint8_t>(n) : nullptr;
     KKW_HIP(hipMemcpyAsync(src, h_src, n * 4, hipMemcpyHostToDevice, s));
     KKW_HIP(hipMemcpyAsync(dst, h_dst, n * 4, hipMemcpyHostToDevice, s));
     if (w) KKW_HIP(hipMemcpyAsync(w, h_w, n * 4, hipMemcpyHostToDevice, s));
+    if (et) KKW_HIP(hipMemcpyAsync(et, h_et, n, hipMemcpyHostToDevice, s));
     const int vb = bits_for(V);
     uint64_t *k0 = dalloc<uint64_t>(m), *k1 = dalloc<uint64_t>(m);
     uint64_t *v0 = dalloc<uint64_t>(m), *v1 = dalloc<uint64_t>(m);
@@ -465,7 +499,7 @@ void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const u
         KKW_HIP(hipGetLastError());
     }
     if (read_counter(bad, s)) {
-        dfree(bad); dfree(k0); dfree(k1); dfree(v0); dfree(v1); dfree(src); dfree(dst); dfree(w);
+        dfree(bad); dfree(k0); dfree(k1); dfree(v0); dfree(v1); dfree(src); dfree(dst); dfree(w); dfree(et);
         throw Error(-1, "edge list holds a vertex id >= v_num");
     }
     dfree(bad);
@@ -485,10 +519,11 @@ void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const u
     gs.row_ptr = dalloc<uint64_t>((uint64_t)V + 1);
     gs.col = dalloc<uint32_t>(m);
     gs.w = w ? dalloc<float>(m) : nullptr;
+    gs.etype = et ? dalloc<uint8_t>(m) : nullptr;
     uint32_t *srcs = dalloc<uint32_t>(m);
     if (m) {
-        hipLaunchKernelGGL(k_gather_entries, dim3(grid_for(m, 256)), dim3(256), 0, s, v1, m, src, dst, w,
-                           undirected ? 1 : 0, gs.col, gs.w, srcs);
+        hipLaunchKernelGGL(k_gather_entries, dim3(grid_for(m, 256)), dim3(256), 0, s, v1, m, src, dst, w, et,
+                           undirected ? 1 : 0, gs.col, gs.w, gs.etype, srcs);
         KKW_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_rowptr_from_src, dim3(grid_for((uint64_t)V + 1, 256)), dim3(256), 0, s, srcs, m,
@@ -500,7 +535,9 @@ void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const u
     dfree(src);
     dfree(dst);
     dfree(w);
+    dfree(et);
     check_graph(gs, s);
+    if (gs.etype) rebuild_vmask(gs, s);
 }
 
 void build_from_csr(GraphStore &gs, uint32_t V, const uint64_t *h_row_ptr, const uint32_t *h_col,
@@ -564,6 +601,46 @@ void type_by_dst(GraphStore &gs, int ntypes, uint64_t seed, hipStream_t s)
     rebuild_vmask(gs, s);
     dfree(gs.bucket);
     dfree(gs.vsum);
+}
+
+// Per-list stable sort by neighbour id, in place (node2vec.hpp:122-125 sorts
+// every list before walking).  Derived tables are dropped.
+void sort_adjacency(GraphStore &gs, hipStream_t s)
+{
+    if (!gs.row_ptr) throw Error(-2, "no graph loaded");
+    if (gs.sorted) return;
+    const uint64_t E = gs.E;
+    if (E == 0) {
+        gs.sorted = true;
+        return;
+    }
+    const int vb = bits_for(gs.V);
+    uint32_t *srcs = dalloc<uint32_t>(E);
+    uint64_t *k0 = dalloc<uint64_t>(E), *k1 = dalloc<uint64_t>(E);
+    uint64_t *v0 = dalloc<uint64_t>(E), *v1 = dalloc<uint64_t>(E);
+    hipLaunchKernelGGL(k_expand_src, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V, srcs);
+    hipLaunchKernelGGL(k_sort_keys_from_csr, dim3(grid_for(E, 256)), dim3(256), 0, s, srcs, gs.col, E, vb, k0, v0);
+    KKW_HIP(hipGetLastError());
+    Temp tmp;
+    size_t tb = 0;
+    KKW_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, E, 0, 2 * vb, s));
+    tmp.ensure(tb);
+    KKW_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0, k1, v0, v1, E, 0, 2 * vb, s));
+    uint32_t *col2 = dalloc<uint32_t>(E);
+    float *w2 = gs.w ? dalloc<float>(E) : nullptr;
+    uint8_t *et2 = gs.etype ? dalloc<uint8_t>(E) : nullptr;
+    hipLaunchKernelGGL(k_permute_csr, dim3(grid_for(E, 256)), dim3(256), 0, s, v1, E, gs.col, gs.w, gs.etype,
+                       col2, w2, et2);
+    KKW_HIP(hipGetLastError());
+    KKW_HIP(hipStreamSynchronize(s));
+    dfree(srcs); dfree(k0); dfree(k1); dfree(v0); dfree(v1);
+    dfree(gs.col);
+    gs.col = col2;
+    if (w2) { dfree(gs.w); gs.w = w2; }
+    if (et2) { dfree(gs.etype); gs.etype = et2; }
+    dfree(gs.bucket);
+    dfree(gs.vsum);
+    gs.sorted = true;
 }
 
 void build_alias(GraphStore &gs, hipStream_t s)

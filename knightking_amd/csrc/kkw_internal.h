@@ -42,7 +42,9 @@ struct GraphStore {
 void build_rmat(GraphStore &gs, int scale, int edge_factor, uint32_t thr_a, uint32_t thr_ab,
                 uint32_t thr_abc, uint64_t seed, bool weighted, hipStream_t s);
 void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const uint32_t *h_dst,
-                      const float *h_w, uint64_t n, bool undirected, bool sort_adj, hipStream_t s);
+                      const float *h_w, const uint8_t *h_et, uint64_t n, bool undirected, bool sort_adj,
+                      hipStream_t s);
+void sort_adjacency(GraphStore &gs, hipStream_t s);
 void build_from_csr(GraphStore &gs, uint32_t V, const uint64_t *h_row_ptr, const uint32_t *h_col,
                     const float *h_w, hipStream_t s);
 void set_edge_types(GraphStore &gs, const uint8_t *h_etype, hipStream_t s);

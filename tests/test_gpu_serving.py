@@ -1,0 +1,144 @@
+"""Serving loop: continuous batching of walk queries on a resident graph.
+
+A query's paths are the same whether it runs alone or admitted into a mix
+with others (draws are keyed by seed and walker id, not by slot or time).
+Failure handling is forced from the host: an admission budget smaller than a
+request, a full queue, a cancelled request, malformed descriptors.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+import knightking_amd as kk
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a, b):
+    if not np.array_equal(a.lengths, b.lengths):
+        return False
+    mask = np.arange(a.vertices.shape[1])[None, :] < a.lengths[:, None]
+    return bool(np.all((a.vertices == b.vertices) | ~mask))
+
+
+@pytest.fixture()
+def served(engine):
+    print("[gpu] serving on R-MAT scale 12", flush=True)
+    engine.graph_rmat(12, 16, seed=3, weighted=True)
+    engine.type_by_dst(5, 4)
+    engine.prepare()
+    yield engine
+    engine.serve_stop()
+
+
+def mixed_queries(V, n, seed):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        k = i % 4
+        walkers = int(10 ** (2 + 2 * rng.random()))
+        if k == 0:
+            jobs.append(kk.ppr_job(walkers, 0.15, True, seed=100 + i,
+                                   start_vertices=np.array([rng.integers(0, V)], np.uint32)))
+        elif k == 1:
+            jobs.append(kk.deepwalk_job(walkers, int(rng.integers(5, 40)), bool(i % 2), seed=100 + i))
+        elif k == 2:
+            jobs.append(kk.node2vec_job(walkers, 20, 2.0, 0.5, True, seed=100 + i))
+        else:
+            jobs.append(kk.metapath_job(walkers, 15, [[0b00111, 0b11000], [0b11111]], True, seed=100 + i))
+    return jobs
+
+
+def test_results_same_alone_and_in_the_mix(served):
+    eng = served
+    V, _ = eng.info()
+    jobs = mixed_queries(V, 40, 1)
+    alone = [eng.run([j])[0] for j in jobs]
+    eng.serve_start(max_walkers_in_flight=20000)
+    tickets = [None] * len(jobs)
+
+    def submit(idx):
+        for i in idx:
+            tickets[i] = eng.submit(jobs[i])
+
+    threads = [threading.Thread(target=submit, args=(range(t, len(jobs), 4),)) for t in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for i, t in enumerate(tickets):
+        got, lat = eng.wait(t, 60.0)
+        assert lat >= 0
+        assert same(got, alone[i]), f"query {i} differs in the mix"
+
+
+def test_budget_smaller_than_request_fails_it(engine):
+    eng = kk.Engine(hbm_budget=1 << 20)
+    try:
+        eng.graph_rmat(10, 16, seed=1, weighted=False)
+        eng.serve_start()
+        big = eng.submit(kk.deepwalk_job(100000, 80, False, seed=1))   # ~32 MB of paths
+        small = eng.submit(kk.deepwalk_job(100, 10, False, seed=2))
+        with pytest.raises(kk.KkwError) as e:
+            eng.wait(big, 30.0)
+        assert e.value.code == kk.KKW_E_NOMEM
+        got, _ = eng.wait(small, 30.0)
+        assert got.lengths.shape == (100,)
+    finally:
+        eng.serve_stop()
+        eng.close()
+
+
+def test_queue_full_gives_back_pressure(served):
+    eng = served
+    eng.serve_start(max_walkers_in_flight=1 << 20, queue_capacity=3)
+    accepted, busy = [], 0
+    for i in range(60):
+        try:
+            accepted.append(eng.submit(kk.deepwalk_job(1 << 20, 80, True, seed=i)))
+        except kk.KkwError as e:
+            assert e.code == kk.KKW_E_BUSY
+            busy += 1
+    assert busy > 0
+    for t in accepted:
+        got, _ = eng.wait(t, 120.0)
+        assert got.lengths.shape == (1 << 20,)
+
+
+def test_cancel_pending_request(served):
+    eng = served
+    eng.serve_start(max_walkers_in_flight=1 << 21, queue_capacity=64)
+    first = eng.submit(kk.deepwalk_job(1 << 21, 80, True, seed=1))
+    second = eng.submit(kk.deepwalk_job(1 << 21, 80, True, seed=2))
+    third = eng.submit(kk.deepwalk_job(1000, 10, True, seed=3))
+    eng.cancel(third)
+    codes = []
+    for t in (first, second, third):
+        try:
+            eng.wait(t, 120.0)
+            codes.append(0)
+        except kk.KkwError as e:
+            codes.append(e.code)
+    assert codes[0] == 0 and codes[1] == 0
+    assert codes[2] in (0, kk.KKW_E_CANCELLED)
+
+
+@pytest.mark.parametrize("bad", ["start", "scheme", "kind", "pq", "ppr_t"])
+def test_malformed_descriptor_rejected_at_submit(served, bad):
+    eng = served
+    V, _ = eng.info()
+    eng.serve_start()
+    if bad == "start":
+        job = kk.ppr_job(10, 0.15, True, start_vertices=np.array([V + 5], np.uint32))
+    elif bad == "scheme":
+        job = kk.metapath_job(10, 10, [], True)
+    elif bad == "kind":
+        job = kk.Job(7, 10, 10)
+    elif bad == "pq":
+        job = kk.node2vec_job(10, 10, 0.0, 1.0)
+    else:
+        job = kk.ppr_job(10, 1.5, True)
+    with pytest.raises(kk.KkwError) as e:
+        eng.submit(job)
+    assert e.value.code == kk.KKW_E_INVALID
