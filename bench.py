@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, published
 LINE = 64                   # bytes charged per data-dependent random access
+CALIB_BYTES = 1 << 30       # streaming-copy calibration size (read once, written once)
 
 
 def log(*a):
@@ -162,6 +163,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--autotune", action="store_true", help="search persistent launch shapes first")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--calib", action="store_true",
+                    help="profiling pass: run the calibration copy only twice")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -220,6 +223,10 @@ def main():
         plain = steps_per_launch * args.steps / wall_l
         log(f"plain: {wall_l / args.steps * 1e3:.2f} ms/launch, {plain / 1e9:.3f} G steps/s")
         lb.close()
+
+    copy_ms = eng.stream_copy_ms(CALIB_BYTES, 2 if args.calib else 10)
+    copy_gbs = 2 * CALIB_BYTES / (copy_ms / 1e3) / 1e9
+    log(f"streaming copy of {CALIB_BYTES >> 20} MiB: {copy_ms:.3f} ms, {copy_gbs:.0f} GB/s read+write")
 
     achieved = bytes_launch / kernel_s / 1e9
     traffic = None
@@ -281,6 +288,7 @@ def main():
             "trials_per_step": round(t["trials"] / max(t["steps"], 1), 4),
             "searches_per_step": round(t["searches"] / max(t["steps"], 1), 4),
             "search_lines_per_search": round(t["search_lines"] / max(t["searches"], 1), 3),
+            "stream_copy_gbs": round(copy_gbs, 1),
         },
         "cpu_baseline": cpu,
         "sample_parity": parity,

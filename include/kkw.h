@@ -85,17 +85,20 @@ typedef struct kkw_job {
 } kkw_job;
 
 /* Paths of one job: PathSet (include/path.hpp:46-84) with one segment in
- * walker-id order.  Row i holds lengths[i] vertices (steps + 1) of walker
- * walker_begin + i; bit 31 of lengths[i] flags a path stopped by the PPR cap. */
+ * walker-id order.  Path i holds (lengths[i] & 0x7fffffff) vertices (steps + 1)
+ * of walker walker_begin + i; bit 31 of lengths[i] flags a path stopped by the
+ * PPR cap.  Rows start at vertices + i * stride, or, when offsets is not NULL
+ * (packed form, used by the serving loop), at vertices + offsets[i]. */
 typedef struct kkw_paths {
     uint32_t walker_begin;
     uint32_t walker_num;
-    uint32_t stride;         /* entries per row                                      */
+    uint32_t stride;         /* entries per row (row-major form)                     */
     uint32_t on_device;      /* 1: vertices/lengths are device pointers              */
-    uint32_t *vertices;      /* [walker_num * stride]                                */
+    uint32_t *vertices;      /* [walker_num * stride] or [offsets[walker_num]]       */
     uint32_t *lengths;       /* [walker_num]                                         */
     uint64_t steps;          /* accepted transitions (sum of lengths - 1)            */
     uint64_t truncated;      /* walkers stopped by the cap                           */
+    uint64_t *offsets;       /* [walker_num + 1] in the packed form, else NULL       */
     void *priv;
 } kkw_paths;
 
@@ -201,6 +204,12 @@ int kkw_batch_destroy(kkw_batch *b);
  * persistent kernel on this batch and keeps the fastest (cache file optional). */
 int kkw_autotune(kkw_batch *b, const char *cache_file, int *block, int *blocks_per_cu,
                  int *chunk);
+
+/* Measurement aid (no counterpart in the reference): a plain streaming copy of
+ * `bytes` (read once, written once) repeated `iters` times on the engine stream;
+ * gives the achievable HBM bandwidth and the known-traffic kernel that hardware
+ * byte counters are scaled by. */
+int kkw_diag_stream_copy(kkw_engine *eng, uint64_t bytes, int iters, double *ms_per_iter);
 
 /* Serving: a resident loop admits queries (usually PPR) by free HBM and slot
  * count, batches everything admitted into rounds of the persistent kernel, and

@@ -454,7 +454,8 @@ private:
         uint64_t o = 0;
         for (uint32_t i = 0; i < p.walker_num; i++) {
             uint32_t len = p.lengths[i] & 0x7fffffffu;
-            memcpy(ps->path_set[0] + o, p.vertices + (uint64_t)i * p.stride, len * sizeof(vertex_id_t));
+            const uint32_t *row = p.offsets ? p.vertices + p.offsets[i] : p.vertices + (uint64_t)i * p.stride;
+            memcpy(ps->path_set[0] + o, row, len * sizeof(vertex_id_t));
             ps->walker_id[0][i] = p.walker_begin + i;
             ps->path_begin[0][i] = ps->path_set[0] + o;
             ps->path_end[0][i] = ps->path_set[0] + o + len;

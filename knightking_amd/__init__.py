@@ -56,7 +56,7 @@ class _Paths(C.Structure):
         ("walker_begin", C.c_uint32), ("walker_num", C.c_uint32), ("stride", C.c_uint32),
         ("on_device", C.c_uint32), ("vertices", C.POINTER(C.c_uint32)),
         ("lengths", C.POINTER(C.c_uint32)), ("steps", C.c_uint64), ("truncated", C.c_uint64),
-        ("priv", C.c_void_p),
+        ("offsets", C.POINTER(C.c_uint64)), ("priv", C.c_void_p),
     ]
 
 
@@ -81,7 +81,7 @@ EXPORTED = [
     "kkw_graph_set_edge_types", "kkw_graph_type_by_dst", "kkw_graph_info", "kkw_graph_download",
     "kkw_prepare", "kkw_alias_download", "kkw_run", "kkw_paths_free", "kkw_paths_dump",
     "kkw_last_timing", "kkw_batch_create", "kkw_batch_set_shape", "kkw_batch_launch",
-    "kkw_batch_sync", "kkw_batch_paths", "kkw_batch_destroy", "kkw_autotune",
+    "kkw_batch_sync", "kkw_batch_paths", "kkw_batch_destroy", "kkw_autotune", "kkw_diag_stream_copy",
     "kkw_serve_start", "kkw_submit", "kkw_poll", "kkw_wait", "kkw_cancel", "kkw_serve_stop",
 ]
 
@@ -126,6 +126,7 @@ def load_library(path: str = LIB_PATH):
         "kkw_batch_paths": (I32, [P, I32, I32, C.POINTER(_Paths)]),
         "kkw_batch_destroy": (I32, [P]),
         "kkw_autotune": (I32, [P, C.c_char_p, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
+        "kkw_diag_stream_copy": (I32, [P, U64, I32, C.POINTER(D)]),
         "kkw_serve_start": (I32, [P, C.POINTER(_ServeOpts)]),
         "kkw_submit": (I32, [P, C.POINTER(_Job), C.POINTER(U64)]),
         "kkw_poll": (I32, [P, U64, C.POINTER(I32)]),
@@ -231,13 +232,20 @@ class Paths:
 
 def _take_paths(p: _Paths) -> Paths:
     n, s = p.walker_num, p.stride
-    if n:
+    raw = np.ctypeslib.as_array(p.lengths, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
+    lens = raw & 0x7FFFFFFF
+    if n and bool(p.offsets):
+        # packed form: expand to rows of `stride` entries (unused entries zero)
+        offs = np.ctypeslib.as_array(p.offsets, shape=(n + 1,)).astype(np.int64)
+        flat = np.ctypeslib.as_array(p.vertices, shape=(max(int(offs[-1]), 1),))[: int(offs[-1])]
+        verts = np.zeros((n, s), np.uint32)
+        cols = np.arange(int(offs[-1])) - np.repeat(offs[:-1], lens.astype(np.int64))
+        verts[np.repeat(np.arange(n), lens.astype(np.int64)), cols] = flat
+    elif n:
         verts = np.ctypeslib.as_array(p.vertices, shape=(n * s,)).reshape(n, s).copy()
-        raw = np.ctypeslib.as_array(p.lengths, shape=(n,)).copy()
     else:
         verts = np.zeros((0, s), np.uint32)
-        raw = np.zeros(0, np.uint32)
-    out = Paths(p.walker_begin, verts, raw & 0x7FFFFFFF, (raw >> 31).astype(bool), int(p.steps))
+    out = Paths(p.walker_begin, verts, lens, (raw >> 31).astype(bool), int(p.steps))
     _lib.kkw_paths_free(C.byref(p))
     return out
 
@@ -403,6 +411,12 @@ class Engine:
         t = _Timing()
         _check(_lib.kkw_last_timing(self.h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in _Timing._fields_}
+
+    def stream_copy_ms(self, nbytes: int, iters: int = 10) -> float:
+        """Average ms of a plain copy of nbytes (read + write) on this engine's stream."""
+        ms = C.c_double()
+        _check(_lib.kkw_diag_stream_copy(self.h, nbytes, iters, C.byref(ms)))
+        return ms.value
 
     # ---- serving ---------------------------------------------------------
     def serve_start(self, max_walkers_in_flight=1 << 22, queue_capacity=1 << 16):

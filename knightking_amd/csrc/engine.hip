@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -422,6 +423,65 @@ static void fill_paths_device(const PreparedJob &pj, hipStream_t s, kkw_paths *o
 // ==========================================================================
 // Serving
 // ==========================================================================
+// Continuous batching on a resident graph.  kkw_submit validates a query and
+// queues it; one worker thread admits queued queries into "flights" while slots
+// (walkers in flight) and memory (a device arena sized at start) allow, and
+// launches each flight as one persistent-kernel launch on one of four streams,
+// so flights admitted while others run overlap on the device.  A flight's
+// paths are packed on the device (row-major rows under a PPR length cap are
+// mostly empty) and copied back asynchronously into pinned staging memory in
+// two phases (lengths + offsets, then the payload of exactly the packed size).
+// No hipMalloc/hipFree happens per query.
+
+// First-fit sub-allocator over [0, cap) in 256-byte units.
+struct SubAlloc {
+    size_t cap = 0;
+    std::map<size_t, size_t> free_;  // offset -> length
+    static size_t up(size_t n) { return (n + 255) & ~(size_t)255; }
+    void init(size_t c)
+    {
+        cap = c & ~(size_t)255;
+        free_.clear();
+        if (cap) free_[0] = cap;
+    }
+    bool alloc(size_t n, size_t &off)
+    {
+        n = up(n ? n : 1);
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= n) {
+                off = it->first;
+                size_t rem = it->second - n;
+                free_.erase(it);
+                if (rem) free_[off + n] = rem;
+                return true;
+            }
+        return false;
+    }
+    void release(size_t off, size_t n)
+    {
+        n = up(n ? n : 1);
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t largest() const
+    {
+        size_t m = 0;
+        for (auto &kv : free_) m = std::max(m, kv.second);
+        return m;
+    }
+};
+
 struct Request {
     uint64_t ticket = 0;
     PreparedJob pj;
@@ -429,7 +489,25 @@ struct Request {
     int status = 1;  // 1 pending, 0 done, <0 error
     bool cancelled = false;
     kkw_paths result{};
-    size_t bytes = 0;
+};
+
+// Layout of one flight inside the device arena (byte offsets from its base)
+// and inside the pinned staging arena.
+struct Flight {
+    std::vector<std::shared_ptr<Request>> reqs;
+    int sidx = 0;
+    int phase = 0;                 // 1: walk + pack + D2H(lens, offs); 2: D2H(payload)
+    hipEvent_t ev = nullptr;
+    size_t dev_off = 0, dev_bytes = 0;
+    size_t pin_off = 0, pin_bytes = 0;
+    size_t pay_off = 0, pay_bytes = 0;
+    bool pay_pinned = false;
+    uint64_t walkers = 0;
+    uint64_t total = 0;            // packed vertices
+    std::vector<uint64_t> row0;    // first flight row of each request
+    size_t o_lens = 0, o_offs = 0; // device offsets of lens_all and offs (adjacent)
+    size_t o_packed = 0;
+    std::vector<uint32_t> *h_lens = nullptr;
 };
 
 struct ServeState {
@@ -443,79 +521,266 @@ struct ServeState {
     bool stop = false;
     std::thread worker;
     hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
-    uint64_t walkers_in_flight = 0;
-    size_t bytes_in_flight = 0;
+    char *dev_base = nullptr;
+    char *pin_base = nullptr;
+    SubAlloc dev, pin;
 };
 
-struct InFlight {
-    std::unique_ptr<kkw_batch> batch;
-    std::vector<std::shared_ptr<Request>> reqs;
-    int stream_idx = 0;
-    uint64_t walkers = 0;
-    size_t bytes = 0;
-};
+static size_t al256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-static size_t request_bytes(const PreparedJob &pj)
+// Device bytes one request adds to a flight: rows, packed copy, lengths,
+// counts, offsets, starts and scheme tables.
+static size_t request_dev_bytes(const PreparedJob &pj)
 {
-    return (size_t)pj.dj.walker_num * (pj.stride + 1) * 4 + pj.starts.size() * 4 + 4096;
+    const size_t n = pj.dj.walker_num;
+    return al256(n * pj.stride * 4) * 2 + al256(n * 4) + al256(n * 8) * 2 + al256(pj.starts.size() * 4) +
+           al256(pj.scheme_len.size() * 4) + al256(pj.scheme_mask.size()) + 1024;
+}
+
+static void finish_request(ServeState *S, const std::shared_ptr<Request> &r, int status)
+{
+    std::lock_guard<std::mutex> lk(S->mu);
+    r->status = status;
+    r->t_done = now_ms();
+    S->cv_done.notify_all();
+}
+
+// Build and launch one flight.  Returns false (and leaves reqs pending) when
+// the device arena has no room for them yet.
+static bool launch_flight(ServeState *S, Flight &f)
+{
+    kkw_engine *eng = S->eng;
+    hipStream_t st = S->streams[f.sidx];
+    const int nj = (int)f.reqs.size();
+    uint64_t N = 0;
+    for (auto &r : f.reqs) {
+        f.row0.push_back(N);
+        N += r->pj.dj.walker_num;
+    }
+    f.walkers = N;
+    const size_t scan_bytes = pack_scan_bytes(N ? N : 1);
+    // ---- layout: [upload: jobs | packjobs | per-job small arrays][queue][lens | offs][cnt][scan][rows...][packed]
+    size_t o = 0;
+    const size_t o_jobs = o;
+    o += al256(sizeof(DevJob) * nj);
+    const size_t o_pjobs = o;
+    o += al256(sizeof(PackJob) * nj);
+    std::vector<size_t> o_starts(nj), o_slen(nj), o_smask(nj);
+    for (int j = 0; j < nj; j++) {
+        const PreparedJob &pj = f.reqs[j]->pj;
+        o_starts[j] = o;
+        o += al256(pj.starts.size() * 4);
+        o_slen[j] = o;
+        o += al256(pj.scheme_len.size() * 4);
+        o_smask[j] = o;
+        o += al256(pj.scheme_mask.size());
+    }
+    const size_t upload_bytes = o;
+    const size_t o_queue = o;
+    o += 256;
+    f.o_lens = o;
+    const size_t lens_bytes = ((N * 4 + 7) & ~(size_t)7);
+    f.o_offs = o + lens_bytes;
+    const size_t down_bytes = lens_bytes + (N + 1) * 8;
+    o += al256(down_bytes);
+    const size_t o_cnt = o;
+    o += al256(N * 8);
+    const size_t o_scan = o;
+    o += al256(scan_bytes);
+    std::vector<size_t> o_rows(nj);
+    size_t packed_max = 0;
+    for (int j = 0; j < nj; j++) {
+        const PreparedJob &pj = f.reqs[j]->pj;
+        o_rows[j] = o;
+        o += al256((size_t)pj.dj.walker_num * pj.stride * 4);
+        packed_max += (size_t)pj.dj.walker_num * pj.stride;
+    }
+    f.o_packed = o;
+    o += al256(packed_max * 4);
+    f.dev_bytes = o;
+    f.pin_bytes = upload_bytes + al256(down_bytes);
+    if (!S->dev.alloc(f.dev_bytes, f.dev_off)) return false;
+    if (!S->pin.alloc(f.pin_bytes, f.pin_off)) {
+        S->dev.release(f.dev_off, f.dev_bytes);
+        return false;
+    }
+    char *D = S->dev_base + f.dev_off;
+    char *H = S->pin_base + f.pin_off;
+    // ---- fill the upload mirror in pinned memory
+    std::vector<DevJob> djs(nj);
+    std::vector<PackJob> pjs(nj);
+    uint64_t chunk_acc = 0;
+    const uint32_t chunk = 64;
+    for (int j = 0; j < nj; j++) {
+        const PreparedJob &pj = f.reqs[j]->pj;
+        DevJob d = pj.dj;
+        d.paths = (uint32_t *)(D + o_rows[j]);
+        d.lens = (uint32_t *)(D + f.o_lens) + f.row0[j];
+        if (!pj.starts.empty()) {
+            memcpy(H + o_starts[j], pj.starts.data(), pj.starts.size() * 4);
+            d.starts = (const uint32_t *)(D + o_starts[j]);
+        }
+        if (!pj.scheme_len.empty()) {
+            memcpy(H + o_slen[j], pj.scheme_len.data(), pj.scheme_len.size() * 4);
+            memcpy(H + o_smask[j], pj.scheme_mask.data(), pj.scheme_mask.size());
+            d.scheme_len = (const uint32_t *)(D + o_slen[j]);
+            d.scheme_mask = (const uint8_t *)(D + o_smask[j]);
+        }
+        d.chunk_begin = chunk_acc;
+        chunk_acc += (d.walker_num + chunk - 1) / chunk;
+        djs[j] = d;
+        pjs[j] = PackJob{f.row0[j], d.walker_num, pj.stride, d.paths};
+    }
+    memcpy(H + o_jobs, djs.data(), sizeof(DevJob) * nj);
+    memcpy(H + o_pjobs, pjs.data(), sizeof(PackJob) * nj);
+    // ---- enqueue: upload, walk, pack, download lengths + offsets
+    DevGraph g = eng->gs.view();
+    KKW_HIP(hipMemcpyAsync(D, H, upload_bytes, hipMemcpyHostToDevice, st));
+    if (chunk_acc) {
+        launch_fill_u64((unsigned long long *)(D + o_queue), 0ull, 1, st);
+        uint64_t blocks_needed = (chunk_acc * 64 + 255) / 256;
+        int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)eng->cu_count * 8, blocks_needed));
+        launch_walk_persistent_mixed(g, (const DevJob *)(D + o_jobs), nj, chunk_acc, chunk,
+                                     (unsigned long long *)(D + o_queue), false, nullptr, grid, 256, st);
+        launch_pack((const PackJob *)(D + o_pjobs), nj, (const uint32_t *)(D + f.o_lens), (uint64_t *)(D + o_cnt),
+                    (uint64_t *)(D + f.o_offs), N, (uint32_t *)(D + f.o_packed), D + o_scan, scan_bytes, st);
+    } else {
+        launch_fill_u64((unsigned long long *)(D + f.o_offs), 0ull, 1, st);
+    }
+    KKW_HIP(hipGetLastError());
+    KKW_HIP(hipMemcpyAsync(H + upload_bytes, D + f.o_lens, down_bytes, hipMemcpyDeviceToHost, st));
+    KKW_HIP(hipEventCreateWithFlags(&f.ev, hipEventDisableTiming));
+    KKW_HIP(hipEventRecord(f.ev, st));
+    f.phase = 1;
+    f.o_lens = upload_bytes;  // from here on: offsets into the pinned mirror
+    return true;
+}
+
+// Phase 1 done: lengths and offsets are on the host; fetch the packed payload.
+static void flight_phase2(ServeState *S, Flight &f)
+{
+    const char *H = S->pin_base + f.pin_off;
+    const uint64_t N = f.walkers;
+    const size_t lens_bytes = ((N * 4 + 7) & ~(size_t)7);
+    const uint64_t *offs = (const uint64_t *)(H + f.o_lens + lens_bytes);
+    f.total = offs[N];
+    f.pay_bytes = f.total * 4;
+    hipStream_t st = S->streams[f.sidx];
+    char *D = S->dev_base + f.dev_off;
+    if (f.pay_bytes && S->pin.alloc(f.pay_bytes, f.pay_off)) {
+        f.pay_pinned = true;
+        KKW_HIP(hipMemcpyAsync(S->pin_base + f.pay_off, D + f.o_packed, f.pay_bytes, hipMemcpyDeviceToHost, st));
+    }
+    KKW_HIP(hipEventRecord(f.ev, st));
+    f.phase = 2;
+}
+
+// Phase 2 done: hand each request its paths.
+static void flight_retire(ServeState *S, Flight &f)
+{
+    const char *H = S->pin_base + f.pin_off;
+    const uint64_t N = f.walkers;
+    const size_t lens_bytes = ((N * 4 + 7) & ~(size_t)7);
+    const uint32_t *lens = (const uint32_t *)(H + f.o_lens);
+    const uint64_t *offs = (const uint64_t *)(H + f.o_lens + lens_bytes);
+    std::vector<uint32_t> fallback;
+    const uint32_t *payload = nullptr;
+    if (f.pay_pinned) {
+        payload = (const uint32_t *)(S->pin_base + f.pay_off);
+    } else if (f.total) {
+        fallback.resize(f.total);
+        KKW_HIP(hipMemcpy(fallback.data(), S->dev_base + f.dev_off + f.o_packed, f.total * 4, hipMemcpyDeviceToHost));
+        payload = fallback.data();
+    }
+    for (size_t j = 0; j < f.reqs.size(); j++) {
+        auto &r = f.reqs[j];
+        const DevJob &d = r->pj.dj;
+        const uint64_t r0 = f.row0[j], n = d.walker_num;
+        kkw_paths &p = r->result;
+        memset(&p, 0, sizeof(p));
+        p.walker_begin = d.walker_begin;
+        p.walker_num = (uint32_t)n;
+        p.stride = r->pj.stride;
+        const uint64_t base = offs[r0], cnt = offs[r0 + n] - base;
+        p.lengths = (uint32_t *)malloc(n ? n * 4 : 4);
+        p.offsets = (uint64_t *)malloc((n + 1) * 8);
+        p.vertices = (uint32_t *)malloc(cnt ? cnt * 4 : 4);
+        int status = 0;
+        if (!p.lengths || !p.offsets || !p.vertices) {
+            free(p.lengths);
+            free(p.offsets);
+            free(p.vertices);
+            memset(&p, 0, sizeof(p));
+            status = KKW_E_NOMEM;
+        } else {
+            uint64_t steps = 0, trunc = 0;
+            for (uint64_t k = 0; k < n; k++) {
+                p.lengths[k] = lens[r0 + k];
+                p.offsets[k] = offs[r0 + k] - base;
+                steps += (lens[r0 + k] & 0x7fffffffu) - 1;
+                trunc += lens[r0 + k] >> 31;
+            }
+            p.offsets[n] = cnt;
+            if (cnt) memcpy(p.vertices, payload + base, cnt * 4);
+            p.steps = steps;
+            p.truncated = trunc;
+            p.priv = (void *)1;
+        }
+        finish_request(S, r, status);
+    }
+    S->dev.release(f.dev_off, f.dev_bytes);
+    S->pin.release(f.pin_off, f.pin_bytes);
+    if (f.pay_pinned) S->pin.release(f.pay_off, f.pay_bytes);
+    (void)hipEventDestroy(f.ev);
+    f.ev = nullptr;
 }
 
 static void serve_loop(ServeState *S)
 {
     kkw_engine *eng = S->eng;
     (void)hipSetDevice(eng->device);
-    std::vector<InFlight> flights;
+    std::list<Flight> flights;
     bool stream_busy[4] = {false, false, false, false};
+    uint64_t walkers_in_flight = 0;
     for (;;) {
-        // 1. retire finished launches
-        for (size_t i = 0; i < flights.size();) {
-            InFlight &f = flights[i];
-            if (hipEventQuery(f.batch->e1) == hipSuccess) {
-                std::vector<kkw_paths> outs(f.reqs.size());
-                int st = 0;
-                std::string err;
-                try {
-                    for (size_t k = 0; k < f.reqs.size(); k++)
-                        fill_paths_host(f.batch->jobs[k], f.batch->stream, &outs[k]);
-                } catch (const Error &e) {
-                    st = e.code;
-                    err = e.what();
-                }
-                double t = now_ms();
-                {
-                    std::lock_guard<std::mutex> lk(S->mu);
-                    for (size_t k = 0; k < f.reqs.size(); k++) {
-                        f.reqs[k]->result = outs[k];
-                        f.reqs[k]->status = st;
-                        f.reqs[k]->t_done = t;
-                    }
-                    S->walkers_in_flight -= f.walkers;
-                    S->bytes_in_flight -= f.bytes;
-                }
-                stream_busy[f.stream_idx] = false;
-                S->cv_done.notify_all();
-                flights.erase(flights.begin() + i);
-            } else {
-                i++;
+        bool progressed = false;
+        // 1. advance flights whose copies have landed
+        for (auto it = flights.begin(); it != flights.end();) {
+            Flight &f = *it;
+            if (hipEventQuery(f.ev) != hipSuccess) {
+                ++it;
+                continue;
             }
+            progressed = true;
+            try {
+                if (f.phase == 1) {
+                    flight_phase2(S, f);
+                    ++it;
+                    continue;
+                }
+                flight_retire(S, f);
+            } catch (const Error &e) {
+                for (auto &r : f.reqs) finish_request(S, r, e.code);
+            }
+            stream_busy[f.sidx] = false;
+            walkers_in_flight -= f.walkers;
+            it = flights.erase(it);
         }
-        // 2. admit pending requests into a new launch on a free stream
-        std::vector<std::shared_ptr<Request>> admit;
+        // 2. admit queued requests into a new flight on a free stream
         int sidx = -1;
         for (int k = 0; k < 4; k++)
             if (!stream_busy[k]) {
                 sidx = k;
                 break;
             }
+        std::vector<std::shared_ptr<Request>> admit;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             if (S->stop && S->pending.empty() && flights.empty()) break;
             if (sidx >= 0) {
-                size_t free_b = 0, total_b = 0;
-                (void)hipMemGetInfo(&free_b, &total_b);
-                size_t budget = eng->hbm_budget ? eng->hbm_budget : free_b;
-                size_t used = S->bytes_in_flight;
-                uint64_t walkers = S->walkers_in_flight;
+                size_t room = S->dev.largest();
+                size_t used = 4096;
+                uint64_t walkers = walkers_in_flight;
                 while (!S->pending.empty()) {
                     auto r = S->pending.front();
                     if (r->cancelled) {
@@ -525,78 +790,57 @@ static void serve_loop(ServeState *S)
                         S->cv_done.notify_all();
                         continue;
                     }
-                    bool fits_mem = eng->hbm_budget ? (used + r->bytes <= budget) : (r->bytes + (64u << 20) <= budget);
+                    size_t need = request_dev_bytes(r->pj) + 2 * pack_scan_bytes(r->pj.dj.walker_num + 1);
+                    if (need + 4096 > S->dev.cap || r->pj.dj.walker_num * (uint64_t)(r->pj.stride) * 4 > S->pin.cap) {
+                        S->pending.pop_front();   // can never fit the arenas
+                        r->status = KKW_E_NOMEM;
+                        r->t_done = now_ms();
+                        S->cv_done.notify_all();
+                        continue;
+                    }
+                    bool fits_mem = used + need <= room;
                     bool fits_slots = walkers + r->pj.dj.walker_num <= S->opts.max_walkers_in_flight ||
                                       (walkers == 0 && admit.empty());
-                    if (!fits_mem) {
-                        if (used == 0 && admit.empty()) {
-                            // can never fit: fail it instead of blocking the queue
-                            S->pending.pop_front();
-                            r->status = KKW_E_NOMEM;
-                            r->t_done = now_ms();
-                            S->cv_done.notify_all();
-                            continue;
-                        }
-                        break;
-                    }
-                    if (!fits_slots) break;
+                    if (!fits_mem || !fits_slots || admit.size() >= 1024) break;
                     S->pending.pop_front();
                     admit.push_back(r);
-                    used += r->bytes;
+                    used += need;
                     walkers += r->pj.dj.walker_num;
-                    if (!eng->hbm_budget) budget = budget > r->bytes ? budget - r->bytes : 0;
-                    if (admit.size() >= 1024) break;
-                }
-                if (!admit.empty()) {
-                    S->bytes_in_flight = used;
-                    S->walkers_in_flight = walkers;
                 }
             }
-            if (admit.empty()) {
-                if (flights.empty()) {
-                    if (S->stop && S->pending.empty()) break;
-                    S->cv_work.wait_for(lk, std::chrono::milliseconds(50));
-                    continue;
-                }
+            if (admit.empty() && flights.empty()) {
+                if (S->stop && S->pending.empty()) break;
+                S->cv_work.wait_for(lk, std::chrono::milliseconds(20));
+                continue;
             }
         }
         if (!admit.empty()) {
-            InFlight f;
-            f.stream_idx = sidx;
+            progressed = true;
+            flights.emplace_back();
+            Flight &f = flights.back();
+            f.sidx = sidx;
             f.reqs = admit;
+            bool ok = false;
             try {
-                std::lock_guard<std::recursive_mutex> elk(eng->mu);
-                auto b = std::make_unique<kkw_batch>();
-                b->eng = eng;
-                b->stream = S->streams[sidx];
-                b->mode = KKW_MODE_PERSISTENT;
-                b->chunk = 64;
-                b->block = 256;
-                std::vector<kkw_job> js;
-                for (auto &r : admit) js.push_back(r->pj.src);
-                batch_setup(b.get(), js.data(), (int)js.size());
-                batch_launch(b.get());
-                for (auto &r : admit) {
-                    f.walkers += r->pj.dj.walker_num;
-                    f.bytes += r->bytes;
-                }
-                f.batch = std::move(b);
-                stream_busy[sidx] = true;
-                flights.push_back(std::move(f));
+                ok = launch_flight(S, f);
             } catch (const Error &e) {
-                std::lock_guard<std::mutex> lk(S->mu);
-                for (auto &r : admit) {
-                    r->status = e.code;
-                    r->t_done = now_ms();
-                    S->walkers_in_flight -= r->pj.dj.walker_num;
-                    S->bytes_in_flight -= r->bytes;
+                for (auto &r : f.reqs) finish_request(S, r, e.code);
+                if (f.dev_bytes && f.phase == 0) {
                 }
-                S->cv_done.notify_all();
+                flights.pop_back();
+                continue;
             }
-            continue;
+            if (!ok) {
+                // no room after all: put them back at the head of the queue
+                std::lock_guard<std::mutex> lk(S->mu);
+                for (auto it = admit.rbegin(); it != admit.rend(); ++it) S->pending.push_front(*it);
+                flights.pop_back();
+            } else {
+                stream_busy[sidx] = true;
+                walkers_in_flight += f.walkers;
+            }
         }
-        // 3. nothing to admit: wait briefly for completions
-        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(10));
     }
 }
 
@@ -985,6 +1229,7 @@ int kkw_paths_free(kkw_paths *p)
     if (p->priv && !p->on_device) {
         free(p->vertices);
         free(p->lengths);
+        free(p->offsets);
     }
     memset(p, 0, sizeof(*p));
     return KKW_OK;
@@ -1001,7 +1246,7 @@ int kkw_paths_dump(const kkw_paths *p, const char *file, const char *mode, int w
         for (uint32_t i = 0; i < p->walker_num; i++) {
             uint32_t len = p->lengths[i] & 0x7fffffffu;
             if (with_head_info) fprintf(f, "%u %u", p->walker_begin + i, len);
-            const uint32_t *row = p->vertices + (size_t)i * p->stride;
+            const uint32_t *row = p->offsets ? p->vertices + p->offsets[i] : p->vertices + (size_t)i * p->stride;
             for (uint32_t k = 0; k < len; k++) fprintf(f, " %u", row[k]);
             fprintf(f, "\n");
         }
@@ -1015,6 +1260,32 @@ int kkw_last_timing(kkw_engine *eng, kkw_timing *t)
     if (!eng || !t) return fail(KKW_E_INVALID, "null argument");
     *t = eng->timing;
     return KKW_OK;
+}
+
+int kkw_diag_stream_copy(kkw_engine *eng, uint64_t bytes, int iters, double *ms_per_iter)
+{
+    return guarded([&]() -> int {
+        if (!eng || iters < 1 || bytes < 16) return fail(KKW_E_INVALID, "bad argument");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        bytes &= ~(uint64_t)15;
+        DevBuf a(bytes), b(bytes);
+        KKW_HIP(hipMemsetAsync(a.p, 1, bytes, eng->stream));
+        hipEvent_t e0, e1;
+        KKW_HIP(hipEventCreate(&e0));
+        KKW_HIP(hipEventCreate(&e1));
+        launch_stream_copy(a.p, b.p, bytes, eng->cu_count, eng->stream);   // warm
+        KKW_HIP(hipEventRecord(e0, eng->stream));
+        for (int i = 0; i < iters; i++) launch_stream_copy(a.p, b.p, bytes, eng->cu_count, eng->stream);
+        KKW_HIP(hipEventRecord(e1, eng->stream));
+        KKW_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        KKW_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (ms_per_iter) *ms_per_iter = ms / iters;
+        return KKW_OK;
+    });
 }
 
 int kkw_autotune(kkw_batch *b, const char *cache_file, int *block, int *blocks_per_cu, int *chunk)
@@ -1100,6 +1371,16 @@ int kkw_serve_start(kkw_engine *eng, const kkw_serve_opts *opts)
         if (S->opts.queue_capacity == 0) S->opts.queue_capacity = 1u << 16;
         KKW_HIP(hipSetDevice(eng->device));
         for (auto &s : S->streams) KKW_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        // device arena: the admission budget (hbm_budget, else half the free HBM, at most 64 GiB)
+        size_t free_b = 0, total_b = 0;
+        KKW_HIP(hipMemGetInfo(&free_b, &total_b));
+        size_t cap = eng->hbm_budget ? (size_t)eng->hbm_budget : std::min(free_b / 2, (size_t)64 << 30);
+        if (cap > free_b) cap = free_b / 2;
+        KKW_HIP(hipMalloc((void **)&S->dev_base, cap ? cap : 256));
+        S->dev.init(cap);
+        size_t pin_cap = std::min(cap, (size_t)1 << 30);
+        KKW_HIP(hipHostMalloc((void **)&S->pin_base, pin_cap ? pin_cap : 256, hipHostMallocDefault));
+        S->pin.init(pin_cap);
         ServeState *raw = S.get();
         S->worker = std::thread(serve_loop, raw);
         eng->serve = std::move(S);
@@ -1123,7 +1404,6 @@ int kkw_submit(kkw_engine *eng, const kkw_job *job, uint64_t *ticket)
         r->pj.src.starts = r->pj.starts.empty() ? nullptr : r->pj.starts.data();
         r->pj.src.scheme_len = r->pj.scheme_len.empty() ? nullptr : r->pj.scheme_len.data();
         r->pj.src.scheme_mask = r->pj.scheme_mask.empty() ? nullptr : r->pj.scheme_mask.data();
-        r->bytes = request_bytes(r->pj);
         r->t_submit = now_ms();
         {
             std::lock_guard<std::mutex> lk(S->mu);
@@ -1206,6 +1486,8 @@ int kkw_serve_stop(kkw_engine *eng)
     for (auto &kv : S->all) kkw_paths_free(&kv.second->result);
     for (auto &s : S->streams)
         if (s) (void)hipStreamDestroy(s);
+    if (S->dev_base) (void)hipFree(S->dev_base);
+    if (S->pin_base) (void)hipHostFree(S->pin_base);
     eng->serve.reset();
     return KKW_OK;
 }

@@ -248,6 +248,27 @@ __global__ void k_check_csr(const uint64_t *row_ptr, const uint32_t *col, const 
     }
 }
 
+// every (v,u) has its reverse (u,v): binary search of v in u's sorted list
+__global__ void k_check_symmetric(const uint64_t *row_ptr, const uint32_t *col, uint32_t V,
+                                  unsigned long long *missing)
+{
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    for (uint64_t k = row_ptr[v]; k < row_ptr[v + 1]; k++) {
+        const uint32_t u = col[k];
+        uint64_t lo = row_ptr[u], hi = row_ptr[u + 1];
+        while (lo < hi) {
+            uint64_t mid = (lo + hi) >> 1;
+            if (col[mid] < (uint32_t)v) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo == row_ptr[u + 1] || col[lo] != (uint32_t)v) {
+            atomicAdd(missing, 1ull);
+            return;
+        }
+    }
+}
+
 // -------------------------------------------------------------- types ----
 __global__ void k_vertex_types(uint32_t V, int ntypes, uint64_t seed, uint8_t *vtype)
 {
@@ -386,6 +407,18 @@ void check_graph(GraphStore &gs, hipStream_t s)
     gs.sorted = hs != 0;
 }
 
+void detect_symmetric(GraphStore &gs, hipStream_t s)
+{
+    gs.symmetric = false;
+    if (!gs.sorted || gs.V == 0) return;
+    unsigned long long *miss = dalloc<unsigned long long>(1);
+    KKW_HIP(hipMemsetAsync(miss, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_check_symmetric, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V, miss);
+    KKW_HIP(hipGetLastError());
+    gs.symmetric = read_counter(miss, s) == 0;
+    dfree(miss);
+}
+
 }  // namespace
 
 static void rebuild_vmask(GraphStore &gs, hipStream_t s);
@@ -402,6 +435,7 @@ void GraphStore::release()
     V = 0;
     E = 0;
     sorted = false;
+    symmetric = false;
 }
 
 DevGraph GraphStore::view() const
@@ -416,6 +450,7 @@ DevGraph GraphStore::view() const
     g.vmask = vmask;
     g.bucket = bucket;
     g.vsum = vsum;
+    g.symmetric = symmetric ? 1u : 0u;
     return g;
 }
 
@@ -462,7 +497,7 @@ void build_rmat(GraphStore &gs, int scale, int edge_factor, uint32_t thr_a, uint
     gs.V = V;
     gs.E = m;
     gs.row_ptr = dalloc<uint64_t>((uint64_t)V + 1);
-    gs.col = dalloc<uint32_t>(m);
+    gs.col = dalloc<uint32_t>(m + 4);
     gs.w = weighted ? dalloc<float>(m) : nullptr;
     hipLaunchKernelGGL(k_rowptr_from_sorted, dim3(grid_for((uint64_t)V + 1, 256)), dim3(256), 0, s, m1, m,
                        scale, V, gs.row_ptr);
@@ -472,6 +507,7 @@ void build_rmat(GraphStore &gs, int scale, int edge_factor, uint32_t thr_a, uint
     KKW_HIP(hipStreamSynchronize(s));
     dfree(m1);
     gs.sorted = true;
+    gs.symmetric = true;   // every edge was mirrored
 }
 
 void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const uint32_t *h_dst,
@@ -517,7 +553,7 @@ void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const u
     gs.V = V;
     gs.E = m;
     gs.row_ptr = dalloc<uint64_t>((uint64_t)V + 1);
-    gs.col = dalloc<uint32_t>(m);
+    gs.col = dalloc<uint32_t>(m + 4);
     gs.w = w ? dalloc<float>(m) : nullptr;
     gs.etype = et ? dalloc<uint8_t>(m) : nullptr;
     uint32_t *srcs = dalloc<uint32_t>(m);
@@ -537,6 +573,8 @@ void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const u
     dfree(w);
     dfree(et);
     check_graph(gs, s);
+    if (undirected) gs.symmetric = true;   // every edge was mirrored
+    else detect_symmetric(gs, s);
     if (gs.etype) rebuild_vmask(gs, s);
 }
 
@@ -550,13 +588,14 @@ void build_from_csr(GraphStore &gs, uint32_t V, const uint64_t *h_row_ptr, const
     gs.V = V;
     gs.E = E;
     gs.row_ptr = dalloc<uint64_t>((uint64_t)V + 1);
-    gs.col = dalloc<uint32_t>(E);
+    gs.col = dalloc<uint32_t>(E + 4);
     gs.w = h_w ? dalloc<float>(E) : nullptr;
     KKW_HIP(hipMemcpyAsync(gs.row_ptr, h_row_ptr, ((uint64_t)V + 1) * 8, hipMemcpyHostToDevice, s));
     if (E) KKW_HIP(hipMemcpyAsync(gs.col, h_col, E * 4, hipMemcpyHostToDevice, s));
     if (gs.w && E) KKW_HIP(hipMemcpyAsync(gs.w, h_w, E * 4, hipMemcpyHostToDevice, s));
     KKW_HIP(hipStreamSynchronize(s));
     check_graph(gs, s);
+    detect_symmetric(gs, s);
 }
 
 static void rebuild_vmask(GraphStore &gs, hipStream_t s)
@@ -626,7 +665,7 @@ void sort_adjacency(GraphStore &gs, hipStream_t s)
     KKW_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, E, 0, 2 * vb, s));
     tmp.ensure(tb);
     KKW_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0, k1, v0, v1, E, 0, 2 * vb, s));
-    uint32_t *col2 = dalloc<uint32_t>(E);
+    uint32_t *col2 = dalloc<uint32_t>(E + 4);
     float *w2 = gs.w ? dalloc<float>(E) : nullptr;
     uint8_t *et2 = gs.etype ? dalloc<uint8_t>(E) : nullptr;
     hipLaunchKernelGGL(k_permute_csr, dim3(grid_for(E, 256)), dim3(256), 0, s, v1, E, gs.col, gs.w, gs.etype,
@@ -641,6 +680,7 @@ void sort_adjacency(GraphStore &gs, hipStream_t s)
     dfree(gs.bucket);
     dfree(gs.vsum);
     gs.sorted = true;
+    detect_symmetric(gs, s);
 }
 
 void build_alias(GraphStore &gs, hipStream_t s)

@@ -77,6 +77,7 @@ struct DevGraph {
     const uint8_t *vmask;      // [V] or null: OR of (1 << etype) over out-edges
     const Bucket *bucket;      // [E] or null
     const float *vsum;         // [V] or null: sequential float sum of weights
+    uint32_t symmetric;        // (u,v) is an edge iff (v,u) is (lists sorted)
 };
 
 // Device form of a job (host constants precomputed in float).

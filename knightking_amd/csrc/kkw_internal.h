@@ -34,6 +34,7 @@ struct GraphStore {
     Bucket *bucket = nullptr;
     float *vsum = nullptr;
     bool sorted = false;
+    bool symmetric = false;
     void release();
     DevGraph view() const;
 };
@@ -51,6 +52,14 @@ void set_edge_types(GraphStore &gs, const uint8_t *h_etype, hipStream_t s);
 void type_by_dst(GraphStore &gs, int ntypes, uint64_t seed, hipStream_t s);
 void build_alias(GraphStore &gs, hipStream_t s);
 
+// one job's rows inside a serving flight (packed output)
+struct PackJob {
+    uint64_t row0;
+    uint32_t n;
+    uint32_t stride;
+    const uint32_t *paths;
+};
+
 // walk.hip
 void launch_walk_plain(const DevGraph &g, const DevJob &J, bool stats, DevStats *st, int block,
                        hipStream_t stream);
@@ -61,5 +70,9 @@ void launch_walk_persistent_mixed(const DevGraph &g, const DevJob *jobs, int n_j
                                   uint32_t chunk, unsigned long long *queue, bool stats, DevStats *st,
                                   int grid, int block, hipStream_t stream);
 void launch_fill_u64(unsigned long long *p, unsigned long long v, int n, hipStream_t stream);
+void launch_stream_copy(const void *src, void *dst, uint64_t bytes, int cu_count, hipStream_t stream);
+void launch_pack(const PackJob *jobs, int n_jobs, const uint32_t *lens, uint64_t *cnt, uint64_t *offs, uint64_t n,
+                 uint32_t *out, void *scan_tmp, size_t scan_bytes, hipStream_t s);
+size_t pack_scan_bytes(uint64_t n);
 
 }  // namespace kkw

@@ -16,9 +16,16 @@
 //    global work queue that spans all jobs of a batch; whenever lanes of a wave
 //    retire a walker (found by ballot), the wave refills them from its chunk, so
 //    waves stay full until the queue drains.
-#include "kkw_dev.h"
+#include "kkw_internal.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cstdlib>
+#include <cstring>
 
 namespace kkw {
+
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int n);
 
 struct Walker {
     uint32_t job;      // job index (persistent batches)
@@ -366,6 +373,335 @@ __global__ void __launch_bounds__(256) k_walk_persistent(DevGraph g, DevJob J0, 
     flush_stats<STATS>(ls, st);
 }
 
+// Plain streaming copy (16 B per lane per iteration): the known-traffic kernel
+// that HBM counters and the achievable bandwidth are calibrated against.
+__global__ void __launch_bounds__(256) k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                     uint64_t n)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+void launch_stream_copy(const void *src, void *dst, uint64_t bytes, int cu_count, hipStream_t stream)
+{
+    uint64_t n = bytes / 16;
+    hipLaunchKernelGGL(k_stream_copy, dim3(cu_count * 16), dim3(256), 0, stream, (const uint4 *)src, (uint4 *)dst, n);
+}
+
+// ----------------------------------------------------- packed output ----
+// lengths (without the cap flag) as 64-bit counts, for the offset scan
+__global__ void k_lens_to_counts(const uint32_t *__restrict__ lens, uint64_t n, uint64_t *__restrict__ cnt)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cnt[i] = lens[i] & 0x7fffffffu;
+}
+
+// Copy every walker's row into the packed array at its offset.  Rows of a
+// flight are numbered across its jobs; job j owns rows [row0, row0 + n).
+__global__ void k_pack_rows(const PackJob *__restrict__ jobs, int n_jobs, const uint32_t *__restrict__ lens,
+                            const uint64_t *__restrict__ offs, uint64_t n, uint32_t *__restrict__ out)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].row0 <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    const PackJob &J = jobs[lo];
+    const uint32_t len = lens[i] & 0x7fffffffu;
+    const uint32_t *src = J.paths + (i - J.row0) * J.stride;
+    uint32_t *dst = out + offs[i];
+    for (uint32_t k = 0; k < len; k++) dst[k] = src[k];
+}
+
+void launch_pack(const PackJob *jobs, int n_jobs, const uint32_t *lens, uint64_t *cnt, uint64_t *offs, uint64_t n,
+                 uint32_t *out, void *scan_tmp, size_t scan_bytes, hipStream_t s)
+{
+    if (n == 0) return;
+    unsigned g = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_lens_to_counts, dim3(g), dim3(256), 0, s, lens, n, cnt);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long *)offs, 0ull, 1);
+    size_t tb = scan_bytes;
+    (void)hipcub::DeviceScan::InclusiveSum(scan_tmp, tb, cnt, offs + 1, n, s);
+    hipLaunchKernelGGL(k_pack_rows, dim3(g), dim3(256), 0, s, jobs, n_jobs, lens, offs, n, out);
+}
+
+size_t pack_scan_bytes(uint64_t n)
+{
+    size_t tb = 0;
+    (void)hipcub::DeviceScan::InclusiveSum(nullptr, tb, (uint64_t *)nullptr, (uint64_t *)nullptr, n, (hipStream_t)0);
+    return tb;
+}
+
+
+// ------------------------------------------------- state-machine kernel ----
+// Persistent single-job kernel in which every loop iteration issues exactly
+// one 16-byte load per active lane: the row_ptr pair of the current vertex
+// (ROW), the alias bucket or list entry of a trial (TRIAL), or four
+// consecutive entries of the previous vertex's sorted list (SEARCH, node2vec
+// adjacency test).  Lanes at different phases share each memory instruction,
+// so a wave is not held back by its longest search; the 4-entry probe also
+// ends a search up to two levels early.  Paths are identical to advance()'s.
+enum : uint32_t { PH_ROW = 0, PH_TRIAL = 1, PH_SEARCH = 2, PH_CROW = 3 };
+
+typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+__device__ __forceinline__ u32x4 load16(const uint32_t *p)
+{
+    u32x4_a4 v = *reinterpret_cast<const u32x4_a4 *>(p);
+    return u32x4{v.x, v.y, v.z, v.w};
+}
+
+template <int KIND, bool ALIAS, bool STATS>
+__global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t n_chunks, uint32_t chunk,
+                                                 unsigned long long *queue, DevStats *st)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    LaneStats ls = {0, 0, 0, 0, 0, 0};
+    bool active = false;
+    uint32_t idx = 0, wid = 0, cur = 0, prev = 0, step = 0, trial = 0, phase = PH_ROW;
+    uint32_t scheme = 0, state = 0, vm = 0;
+    uint64_t beg = 0, end = 0, pbeg = 0, pend = 0, cbeg = 0, cend = 0;
+    // search: key in the list [sbase, sbase + slen_all), current range [lo, hi)
+    uint64_t sbase = 0;
+    uint32_t key = 0, slen_all = 0, lo = 0, hi = 0;
+    uint32_t cand = 0, rz = 0, rw = 0;
+    bool row_known = false;    // the candidate's row_ptr pair is already loaded
+    float dart = 0.0f;
+    uintptr_t last_line = 0;   // instrumented builds: distinct 64-B lines per search
+    uint32_t pool_next = 0, pool_end = 0;
+    bool drained = false;
+    uint32_t *const paths = J.paths;
+    const uint32_t stride = J.stride;
+    for (;;) {
+        uint64_t need = __ballot(!active);
+        while (need && !drained) {
+            if (pool_next >= pool_end) {
+                unsigned long long c = 0;
+                if (lane == 0) c = atomicAdd(queue, 1ull);
+                c = __shfl(c, 0, 64);
+                if (c >= n_chunks) {
+                    drained = true;
+                    break;
+                }
+                pool_next = (uint32_t)c * chunk;
+                uint32_t e = pool_next + chunk;
+                pool_end = e < J.walker_num ? e : J.walker_num;
+            }
+            uint32_t avail = pool_end - pool_next;
+            uint32_t rank = (uint32_t)__popcll(need & lt_mask);
+            if (!active && rank < avail) {
+                idx = pool_next + rank;
+                wid = J.walker_begin + idx;
+                cur = start_vertex(g, J, wid);
+                prev = 0;
+                pbeg = pend = 0;
+                step = 0;
+                trial = 0;
+                phase = PH_ROW;
+                if (KIND == KIND_METAPATH) {
+                    u32x4 r = draw(J.seed, wid, 0, 0, TAG_INIT);
+                    scheme = bounded(r.x, r.y, J.n_schemes);
+                    state = 0;
+                }
+                paths[(uint64_t)idx * stride] = cur;
+                active = true;
+            }
+            uint32_t want = (uint32_t)__popcll(need);
+            pool_next += want < avail ? want : avail;
+            need = __ballot(!active);
+        }
+        if (__ballot(active) == 0ull) break;
+        if (!active) continue;
+
+        // ---- the one load of this iteration
+        const uint32_t *addr;
+        uint64_t blk = 0;
+        if (phase == PH_ROW) {
+            addr = reinterpret_cast<const uint32_t *>(g.row_ptr + cur);
+            if (KIND == KIND_METAPATH) vm = g.vmask[cur];
+        } else if (phase == PH_TRIAL) {
+            u32x4 r = draw(J.seed, wid, step, trial, TAG_TRIAL);
+            uint32_t k = bounded(r.x, r.y, (uint32_t)(end - beg));
+            rz = r.z;
+            rw = r.w;
+            addr = ALIAS ? reinterpret_cast<const uint32_t *>(g.bucket + beg + k) : g.col + beg + k;
+            if (!ALIAS && KIND == KIND_METAPATH) vm = g.etype[beg + k];
+        } else if (phase == PH_CROW) {
+            addr = reinterpret_cast<const uint32_t *>(g.row_ptr + cand);
+        } else {
+            // aligned block of four entries holding the middle of the range
+            blk = (sbase + lo + ((hi - lo) >> 1)) & ~(uint64_t)3;
+            addr = g.col + blk;
+        }
+        const u32x4 v = load16(addr);
+
+        // ---- consume it
+        bool moved = false, rejected = false;
+        if (phase == PH_CROW) {
+            cbeg = (uint64_t)v.x | ((uint64_t)v.y << 32);
+            cend = (uint64_t)v.z | ((uint64_t)v.w << 32);
+            row_known = true;
+            ls.rows++;
+            // prev's list holds cand iff cand's list holds prev (symmetric
+            // graph): search the shorter one
+            const uint64_t pl = pend - pbeg, cl = cend - cbeg;
+            if (cl < pl) {
+                sbase = cbeg;
+                slen_all = (uint32_t)cl;
+                key = prev;
+            } else {
+                sbase = pbeg;
+                slen_all = (uint32_t)pl;
+                key = cand;
+            }
+            lo = 0;
+            hi = slen_all;
+            if (hi == 0) {   // cand has no neighbours: prev is not one of them
+                moved = dart <= J.inv_q;
+                rejected = !moved;
+            } else {
+                phase = PH_SEARCH;
+                ls.searches++;
+                last_line = ~(uintptr_t)0;
+                continue;
+            }
+        } else if (phase == PH_ROW) {
+            beg = (uint64_t)v.x | ((uint64_t)v.y << 32);
+            end = (uint64_t)v.z | ((uint64_t)v.w << 32);
+            ls.rows++;
+        } else if (phase == PH_TRIAL) {
+            ls.trials++;
+            uint32_t et = 0;
+            row_known = false;
+            if (ALIAS) {
+                bool prim = u01(rz) < __uint_as_float(v.x);
+                cand = prim ? v.y : v.z;
+                et = prim ? (v.w & 0xffu) : ((v.w >> 8) & 0xffu);
+            } else {
+                cand = v.x;
+                et = vm;
+            }
+            if (KIND == KIND_NODE2VEC) {
+                dart = u01(rw) * J.upper;
+                if (dart <= J.lower || step == 0) {
+                    moved = (dart <= J.lower) || (dart <= J.upper);
+                    rejected = !moved;
+                } else if (cand == prev) {
+                    moved = dart <= J.inv_p;
+                    rejected = !moved;
+                } else if (dart <= J.acc_lo) {
+                    moved = true;
+                } else if (dart > J.acc_hi) {
+                    rejected = true;
+                } else if (g.symmetric) {
+                    phase = PH_CROW;
+                    continue;
+                } else {
+                    sbase = pbeg;
+                    slen_all = (uint32_t)(pend - pbeg);
+                    key = cand;
+                    lo = 0;
+                    hi = slen_all;
+                    phase = PH_SEARCH;
+                    ls.searches++;
+                    last_line = ~(uintptr_t)0;
+                    continue;
+                }
+            } else if (KIND == KIND_METAPATH) {
+                float d = u01(rw) * 1.0f;
+                float pd = ((J.scheme_mask[scheme * 8 + state] >> et) & 1u) ? 1.0f : 0.0f;
+                moved = d <= pd;
+                rejected = !moved;
+            } else {
+                moved = true;
+            }
+        } else {
+            // PH_SEARCH: entries blk..blk+3 (absolute); those inside the list
+            // narrow [lo, hi) from both sides; the middle one is among them
+            if (STATS) {
+                ls.probes++;
+                uintptr_t line = (uintptr_t)(g.col + blk) >> 6;
+                ls.lines += line != last_line;
+                last_line = line;
+            }
+            const uint32_t e4[4] = {v.x, v.y, v.z, v.w};
+            bool found = false;
+            uint32_t nlo = lo, nhi = hi;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const uint64_t a = blk + (uint64_t)t;
+                if (a >= sbase && a < sbase + slen_all) {
+                    const uint32_t r = (uint32_t)(a - sbase);
+                    if (e4[t] == key) found = true;
+                    else if (e4[t] < key) nlo = nlo > r + 1 ? nlo : r + 1;
+                    else nhi = nhi < r ? nhi : r;
+                }
+            }
+            if (!found) {
+                lo = nlo;
+                hi = nhi;
+                if (lo < hi) continue;
+            }
+            moved = dart <= (found ? 1.0f : J.inv_q);
+            rejected = !moved;
+        }
+
+        if (rejected) {
+            trial++;
+            phase = PH_TRIAL;
+            if (trial >= MAX_TRIALS) {
+                J.lens[idx] = (step + 1) | 0x80000000u;
+                active = false;
+            }
+            continue;
+        }
+        if (moved) {
+            step++;
+            paths[(uint64_t)idx * stride + step] = cand;
+            if (KIND == KIND_METAPATH) state = (state + 1) % J.scheme_len[scheme];
+            prev = cur;
+            pbeg = beg;
+            pend = end;
+            cur = cand;
+            trial = 0;
+            ls.steps++;
+            if (!row_known) {
+                phase = PH_ROW;
+                continue;
+            }
+            beg = cbeg;
+            end = cend;
+            row_known = false;
+        }
+        // row of cur known: termination test of the new step (walk.hpp:956-973)
+        bool stop = end == beg;
+        bool capped = false;
+        if (!stop) {
+            if (KIND == KIND_PPR) {
+                if (J.ppr_ext == 0.0f) stop = true;
+                else if (J.ppr_ext < 1.0f) stop = u01(draw(J.seed, wid, step, 0, TAG_EXT).x) > J.ppr_ext;
+                if (!stop && step >= J.walk_length) stop = capped = true;
+            } else if (KIND == KIND_METAPATH) {
+                stop = step >= J.walk_length || !(vm & J.scheme_mask[scheme * 8 + state]);
+            } else {
+                stop = step >= J.walk_length;
+            }
+        }
+        if (stop) {
+            J.lens[idx] = (step + 1) | (capped ? 0x80000000u : 0u);
+            active = false;
+            continue;
+        }
+        phase = PH_TRIAL;
+    }
+    flush_stats<STATS>(ls, st);
+}
+
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -414,6 +750,22 @@ static void launch_single_t(const DevGraph &g, const DevJob &J, uint64_t n_chunk
                             unsigned long long *queue, bool stats, DevStats *st, int grid, int block,
                             hipStream_t stream)
 {
+    // The state-machine kernel, unless the walk needs node2vec's outlier appendix
+    // (handled by advance()) or KKW_WALK_KERNEL=trial selects the trial-loop form.
+    static const bool trial_form = [] {
+        const char *e = getenv("KKW_WALK_KERNEL");
+        return e && strcmp(e, "trial") == 0;
+    }();
+    const bool sm = !trial_form && !(KIND == KIND_NODE2VEC && J.overflow > 0.0f);
+    if (sm) {
+        if (stats)
+            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, true>), dim3(grid), dim3(block), 0, stream, g, J, n_chunks,
+                               chunk, queue, st);
+        else
+            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, false>), dim3(grid), dim3(block), 0, stream, g, J, n_chunks,
+                               chunk, queue, st);
+        return;
+    }
     if (stats)
         hipLaunchKernelGGL((k_walk_persistent<KIND, true, true, ALIAS, true>), dim3(grid), dim3(block), 0,
                            stream, g, J, (const DevJob *)nullptr, 1, n_chunks, chunk, queue, st);

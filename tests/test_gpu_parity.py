@@ -224,3 +224,23 @@ def test_large_node2vec_properties(engine):
     (pl,) = engine.run([job], mode=kk.MODE_PLAIN)
     assert np.array_equal(pl.lengths, pp.lengths)
     assert np.array_equal(pl.vertices[sample], pp.vertices[sample])
+
+
+@pytest.mark.parametrize("p,q", [(2.0, 0.5), (0.5, 2.0), (1.0, 0.25)])
+@pytest.mark.parametrize("mode", [kk.MODE_PLAIN, kk.MODE_PERSISTENT], ids=["plain", "persistent"])
+def test_node2vec_directed_graph(engine, p, q, mode):
+    """An asymmetric graph: the adjacency test must search the previous vertex's
+    list (no shortcut through the candidate's list); where the return edge is
+    missing the outlier appendix is empty on both sides."""
+    rng = np.random.default_rng(7)
+    V, n = 300, 4000
+    src = rng.integers(0, V, n).astype(np.uint32)
+    dst = rng.integers(0, V, n).astype(np.uint32)
+    w = (1 + 4 * rng.random(n)).astype(np.float32)
+    engine.graph_from_edges(V, src, dst, w, make_undirected=False, sort_adj=True)
+    rp, col, wd, _ = engine.download()
+    g = O.Graph(rp, col, wd)
+    print(f"[gpu] directed node2vec p={p} q={q} mode={mode}", flush=True)
+    (gpu,) = engine.run([kk.node2vec_job(2000, 30, p, q, True, seed=9)], mode=mode)
+    op, ol, ot, st = O.walk(g, O.NODE2VEC, 2000, 30, True, p=p, q=q, seed=9)
+    assert_paths_equal(gpu, op, ol, ot, "directed")
