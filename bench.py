@@ -31,15 +31,22 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(t, walkers, weighted):
-    """Bytes one launch needs: one 64 B line per data-dependent access (row_ptr
-    pair, alias bucket or col entry per trial, each distinct line a binary
-    search touches) plus the exact path and length bytes written."""
-    rows = t["row_loads"]
-    trials = t["trials"]
-    lines = t["search_lines"]
-    path_bytes = 4 * (t["steps"] + walkers) + 4 * walkers
-    return LINE * (rows + trials + lines) + path_bytes
+def reference_model_bytes(t, weighted):
+    """Algorithmic bytes of one launch under the per-step model of KnightKing's
+    node2vec loop: per step 64 B (row_ptr pair) + 4 B (path entry); per trial
+    64 B alias bucket + 64 B neighbour entry (weighted) or 64 B neighbour entry
+    (unweighted); per adjacency query the reference would send, 64 B times
+    max(1, ceil(log2(deg(prev)+1)) - 3) binary-search lines.  T, f and L_b are
+    the counts measured by the instrumented run of the same launch."""
+    per_trial = 2 * LINE if weighted else LINE
+    return (LINE + 4) * t["steps"] + per_trial * t["trials"] + LINE * t["ref_query_lines"]
+
+
+def touched_bytes(t, walkers):
+    """Bytes this engine's kernel touches: one 64 B line per row_ptr pair,
+    alias bucket (or list entry) and distinct line of an adjacency probe, plus
+    path and length bytes."""
+    return LINE * (t["row_loads"] + t["trials"] + t["search_lines"]) + 4 * (t["steps"] + walkers) + 4 * walkers
 
 
 def time_batch(b, steps, warmup):
@@ -182,12 +189,11 @@ def main():
     graph_s = time.perf_counter() - t0
     V, E = eng.info()
     log(f"R-MAT-{args.scale}: V={V} E={E} built in {graph_s:.2f}s")
-    alias_s = 0.0
-    if weighted:
-        t0 = time.perf_counter()
-        eng.prepare()
-        alias_s = time.perf_counter() - t0
-        log(f"alias tables in {alias_s:.2f}s")
+    eng.prepare()   # alias tables (weighted) and neighbour hash sets
+    tp = eng.timing()
+    alias_s = tp["alias_ms"] / 1e3 if weighted else 0.0
+    index_s = tp["index_ms"] / 1e3
+    log(f"alias tables in {alias_s:.2f}s, neighbour hash sets in {index_s:.2f}s")
 
     job_args = dict(seed=1)
     job = kk.node2vec_job(V, 80, 2.0, 0.5, weighted, seed=1)
@@ -198,9 +204,12 @@ def main():
     sb.sync()
     t = eng.timing()
     sb.close()
-    bytes_launch = algorithmic_bytes(t, V, weighted)
-    log(f"stats: steps={t['steps']} trials={t['trials']} searches={t['searches']} "
-        f"lines={t['search_lines']} rows={t['row_loads']} bytes/step={bytes_launch / max(t['steps'], 1):.1f}")
+    ref_bytes = reference_model_bytes(t, weighted)
+    touched = touched_bytes(t, V)
+    bytes_launch = touched   # this kernel's algorithmic bytes (64 B per data-dependent access)
+    log(f"stats: steps={t['steps']} trials={t['trials']} searches={t['searches']} lines={t['search_lines']} "
+        f"rows={t['row_loads']} ref_queries={t['ref_queries']} ref_lines={t['ref_query_lines']} "
+        f"reference model bytes/step={ref_bytes / max(t['steps'], 1):.1f} touched/step={touched / max(t['steps'], 1):.1f}")
 
     pb = eng.batch([job], kk.MODE_PERSISTENT, kk.BATCH_GRAPH)
     shape = (256, 8, 64)
@@ -244,6 +253,21 @@ def main():
         log(f"cpu baseline {rate / 1e6:.2f} M steps/s on {threads} threads; sample parity {parity}")
     pb.close()
 
+    variants = None
+    if not args.no_plain:
+        # the same two launch forms with the binary search over sorted lists in
+        # place of the neighbour hash sets
+        eng.adj_index(False)
+        variants = {}
+        for name, mode in (("persistent_binary_search", kk.MODE_PERSISTENT), ("plain_binary_search", kk.MODE_PLAIN)):
+            vb = eng.batch([job], mode, 0)
+            wall_v, _ = time_batch(vb, args.steps, args.warmup)
+            assert vb.steps(0) == steps_per_launch
+            variants[name] = round(steps_per_launch * args.steps / wall_v, 1)
+            vb.close()
+            log(f"{name}: {variants[name] / 1e9:.3f} G steps/s")
+        eng.adj_index(True)
+
     serving = None
     if not args.no_serve:
         rp, _, _, _ = eng.download(weights=False)
@@ -273,8 +297,10 @@ def main():
         },
         "plain_steps_per_s": None if plain is None else round(plain, 1),
         "persistent_over_plain": None if plain is None else round(value / plain, 3),
+        "variants_steps_per_s": variants,
         "graph_build_s": round(graph_s, 3),
         "alias_build_s": round(alias_s, 3),
+        "neighbour_set_build_s": round(index_s, 3),
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -284,6 +310,8 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes_per_launch": int(bytes_launch),
             "algorithmic_bytes_per_step": round(bytes_launch / max(steps_per_launch, 1), 1),
+            "reference_model_bytes_per_step": round(ref_bytes / max(steps_per_launch, 1), 1),
+            "reference_model_gbs": round(ref_bytes / kernel_s / 1e9, 1),
             "kernel_ms": round(kernel_s * 1e3, 3),
             "trials_per_step": round(t["trials"] / max(t["steps"], 1), 4),
             "searches_per_step": round(t["searches"] / max(t["steps"], 1), 4),

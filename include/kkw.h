@@ -107,10 +107,14 @@ typedef struct kkw_timing {
     double total_ms;         /* host wall time of the last kkw_run / batch launch     */
     double alias_ms;         /* last kkw_prepare                                     */
     double graph_ms;         /* last graph build                                     */
+    double index_ms;         /* last neighbour hash-set build                        */
     /* instrumented runs (KKW_FLAG_STATS / KKW_BATCH_STATS): accepted steps, trials,
      * adjacency searches, elements they read, distinct 64-B lines they read,
-     * row_ptr pairs read */
+     * row_ptr pairs read; and, for node2vec, the adjacency queries KnightKing's loop
+     * would send (step >= 1, dart above the lower bound, candidate not the previous
+     * vertex) with the sum over them of max(1, ceil(log2(deg(prev)+1)) - 3) */
     uint64_t steps, trials, searches, search_probes, row_loads, search_lines;
+    uint64_t ref_queries, ref_query_lines;
 } kkw_timing;
 
 enum kkw_flags { KKW_FLAG_STATS = 1 };
@@ -167,8 +171,15 @@ int kkw_graph_download(kkw_engine *eng, uint64_t *row_ptr, uint32_t *col, float 
                        uint8_t *etype);
 
 /* Static component.  Replaces init_alias_tables (include/walk.hpp:538-652), kept
- * resident across runs instead of being rebuilt per random_walk call. */
+ * resident across runs instead of being rebuilt per random_walk call.  Also
+ * builds the neighbour hash sets node2vec's adjacency test uses (unless turned
+ * off below); both are otherwise built on first use. */
 int kkw_prepare(kkw_engine *eng);
+/* Neighbour hash sets for node2vec's "is the candidate a neighbour of the
+ * previous vertex" test (the respond_query of src/apps/node2vec.hpp:158-166,
+ * there a std::binary_search): 1 builds them now, 0 drops them and makes the
+ * test use the binary search over sorted lists.  Same answers either way. */
+int kkw_graph_adj_index(kkw_engine *eng, int enable);
 /* Copy the alias buckets back (creation order per vertex) and the per-vertex
  * sequential weight sums; any pointer may be NULL. */
 int kkw_alias_download(kkw_engine *eng, float *p, uint32_t *primary, uint32_t *alias,

@@ -63,9 +63,9 @@ class _Paths(C.Structure):
 class _Timing(C.Structure):
     _fields_ = [
         ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("alias_ms", C.c_double),
-        ("graph_ms", C.c_double), ("steps", C.c_uint64), ("trials", C.c_uint64),
+        ("graph_ms", C.c_double), ("index_ms", C.c_double), ("steps", C.c_uint64), ("trials", C.c_uint64),
         ("searches", C.c_uint64), ("search_probes", C.c_uint64), ("row_loads", C.c_uint64),
-        ("search_lines", C.c_uint64),
+        ("search_lines", C.c_uint64), ("ref_queries", C.c_uint64), ("ref_query_lines", C.c_uint64),
     ]
 
 
@@ -79,7 +79,7 @@ EXPORTED = [
     "kkw_last_error", "kkw_abi_version", "kkw_engine_create", "kkw_engine_destroy",
     "kkw_graph_from_edges", "kkw_graph_from_arrays", "kkw_graph_sort_adj", "kkw_graph_load_file", "kkw_graph_from_csr", "kkw_graph_rmat",
     "kkw_graph_set_edge_types", "kkw_graph_type_by_dst", "kkw_graph_info", "kkw_graph_download",
-    "kkw_prepare", "kkw_alias_download", "kkw_run", "kkw_paths_free", "kkw_paths_dump",
+    "kkw_prepare", "kkw_graph_adj_index", "kkw_alias_download", "kkw_run", "kkw_paths_free", "kkw_paths_dump",
     "kkw_last_timing", "kkw_batch_create", "kkw_batch_set_shape", "kkw_batch_launch",
     "kkw_batch_sync", "kkw_batch_paths", "kkw_batch_destroy", "kkw_autotune", "kkw_diag_stream_copy",
     "kkw_serve_start", "kkw_submit", "kkw_poll", "kkw_wait", "kkw_cancel", "kkw_serve_stop",
@@ -114,6 +114,7 @@ def load_library(path: str = LIB_PATH):
         "kkw_graph_info": (I32, [P, C.POINTER(U32), C.POINTER(U64)]),
         "kkw_graph_download": (I32, [P, P, P, P, P]),
         "kkw_prepare": (I32, [P]),
+        "kkw_graph_adj_index": (I32, [P, I32]),
         "kkw_alias_download": (I32, [P, P, P, P, P, P]),
         "kkw_run": (I32, [P, C.POINTER(_Job), I32, I32, I32, I32, C.POINTER(_Paths)]),
         "kkw_paths_free": (I32, [C.POINTER(_Paths)]),
@@ -386,6 +387,10 @@ class Engine:
 
     def prepare(self):
         _check(_lib.kkw_prepare(self.h))
+
+    def adj_index(self, enable: bool = True):
+        """Build (True) or drop (False) the neighbour hash sets of node2vec's adjacency test."""
+        _check(_lib.kkw_graph_adj_index(self.h, 1 if enable else 0))
 
     def alias_download(self):
         V, E = self.info()

@@ -95,6 +95,10 @@ struct kkw_engine {
     std::recursive_mutex mu;
     GraphStore gs;
     kkw_timing timing{};
+    bool adj_index_auto = true;
+    // bumped whenever device graph storage is reallocated; batches compare it
+    // before replaying a captured launch
+    uint64_t graph_version = 1;
     std::unique_ptr<ServeState> serve;
 };
 
@@ -159,10 +163,19 @@ static PreparedJob prepare_job(kkw_engine *eng, const kkw_job &j)
             throw Error(KKW_E_INVALID, "node2vec needs adjacency lists sorted by neighbour "
                                        "(load with sort_adj=1, as node2vec.hpp:122-125 sorts)");
         node2vec_constants(j.p, j.q, d);
+        // neighbour hash sets for the adjacency test, unless the caller turned
+        // them off (kkw_graph_adj_index(eng, 0): binary search of sorted lists)
+        if (eng->adj_index_auto && !gs.adjset) {
+            eng->graph_version++;
+            double t0 = now_ms();
+            build_adj_index(gs, eng->stream);
+            eng->timing.index_ms = now_ms() - t0;
+        }
     }
     if (j.use_weights) {
         if (!gs.w) throw Error(KKW_E_INVALID, "use_weights on a graph without weights");
         if (!gs.bucket) {
+            eng->graph_version++;
             double t0 = now_ms();
             build_alias(gs, eng->stream);
             eng->timing.alias_ms = now_ms() - t0;
@@ -210,6 +223,9 @@ struct kkw_batch {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool launched = false;
     double host_t0 = 0.0;
+    uint32_t graph_V = 0;          // graph the jobs were validated against
+    uint64_t graph_E = 0;
+    uint64_t captured_version = 0; // graph version baked into gexec
 
     ~kkw_batch()
     {
@@ -230,6 +246,8 @@ static void batch_setup(kkw_batch *b, const kkw_job *jobs, int n_jobs)
     kkw_engine *eng = b->eng;
     if (n_jobs < 0 || (n_jobs > 0 && !jobs)) throw Error(KKW_E_INVALID, "bad job array");
     for (int i = 0; i < n_jobs; i++) b->jobs.push_back(prepare_job(eng, jobs[i]));
+    b->graph_V = eng->gs.V;
+    b->graph_E = eng->gs.E;
     KKW_HIP(hipEventCreate(&b->e0));
     KKW_HIP(hipEventCreate(&b->e1));
     b->d_queue = (unsigned long long *)b->alloc(sizeof(unsigned long long) * 2);
@@ -297,7 +315,7 @@ static void batch_enqueue(kkw_batch *b)
 {
     DevGraph g = b->eng->gs.view();
     hipStream_t s = b->stream;
-    if (b->stats) launch_fill_u64((unsigned long long *)b->d_stats, 0ull, 6, s);
+    if (b->stats) launch_fill_u64((unsigned long long *)b->d_stats, 0ull, 8, s);
     if (b->mode == KKW_MODE_PLAIN) {
         for (auto &pj : b->jobs) launch_walk_plain(g, pj.dj, b->stats, b->d_stats, b->block, s);
     } else {
@@ -319,10 +337,23 @@ static void batch_enqueue(kkw_batch *b)
 
 static void batch_launch(kkw_batch *b)
 {
+    kkw_engine *eng = b->eng;
+    if (!eng->gs.row_ptr || eng->gs.V != b->graph_V || eng->gs.E != b->graph_E)
+        throw Error(KKW_E_INVALID, "the graph changed since this batch was created; create a new batch");
+    for (auto &pj : b->jobs)   // tables a job needs may have been dropped since
+        if ((pj.dj.use_alias && !eng->gs.bucket) || (pj.dj.kind == KKW_METAPATH && !eng->gs.vmask))
+            throw Error(KKW_E_INVALID, "the graph's alias tables or edge types changed; create a new batch");
+    if (b->gexec && b->captured_version != eng->graph_version) {   // stale pointers in the capture
+        (void)hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
     b->host_t0 = now_ms();
     KKW_HIP(hipEventRecord(b->e0, b->stream));
     if (b->use_graph && b->mode == KKW_MODE_PERSISTENT) {
         if (!b->gexec) {
+            b->captured_version = eng->graph_version;
             KKW_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             try {
                 batch_enqueue(b);
@@ -361,6 +392,8 @@ static double batch_sync(kkw_batch *b)
         t.search_probes = h.probes;
         t.row_loads = h.row_loads;
         t.search_lines = h.search_lines;
+        t.ref_queries = h.ref_queries;
+        t.ref_query_lines = h.ref_lines;
     }
     return ms;
 }
@@ -903,6 +936,7 @@ int kkw_graph_from_edges(kkw_engine *eng, uint32_t v_num, const void *edges, uin
             memcpy(&dst[i], p + i * rec + 4, 4);
             if (weighted) memcpy(&w[i], p + i * rec + 8, 4);
         }
+        eng->graph_version++;
         build_from_edges(eng->gs, v_num, src.data(), dst.data(), weighted ? w.data() : nullptr, nullptr,
                          n_edges, make_undirected != 0, sort_adj != 0, eng->stream);
         eng->timing.graph_ms = now_ms() - t0;
@@ -920,6 +954,7 @@ int kkw_graph_from_arrays(kkw_engine *eng, uint32_t v_num, const uint32_t *src, 
         std::lock_guard<std::recursive_mutex> lk(eng->mu);
         KKW_HIP(hipSetDevice(eng->device));
         double t0 = now_ms();
+        eng->graph_version++;
         build_from_edges(eng->gs, v_num, src, dst, w, etype, n_edges, make_undirected != 0, sort_adj != 0,
                          eng->stream);
         eng->timing.graph_ms = now_ms() - t0;
@@ -933,6 +968,7 @@ int kkw_graph_sort_adj(kkw_engine *eng)
         if (!eng) return fail(KKW_E_INVALID, "engine is null");
         std::lock_guard<std::recursive_mutex> lk(eng->mu);
         KKW_HIP(hipSetDevice(eng->device));
+        eng->graph_version++;
         sort_adjacency(eng->gs, eng->stream);
         return KKW_OK;
     });
@@ -997,6 +1033,7 @@ int kkw_graph_from_csr(kkw_engine *eng, uint32_t v_num, const uint64_t *row_ptr,
         double t0 = now_ms();
         for (uint32_t v = 0; v < v_num; v++)
             if (row_ptr[v + 1] < row_ptr[v]) return fail(KKW_E_INVALID, "row_ptr must be non-decreasing");
+        eng->graph_version++;
         build_from_csr(eng->gs, v_num, row_ptr, col, w, eng->stream);
         eng->timing.graph_ms = now_ms() - t0;
         return KKW_OK;
@@ -1017,6 +1054,7 @@ int kkw_graph_rmat(kkw_engine *eng, int scale, int edge_factor, double a, double
             double v = std::floor(x * 4294967296.0);
             return v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)v;
         };
+        eng->graph_version++;
         build_rmat(eng->gs, scale, edge_factor, thr(a), thr(a + b), thr(a + b + c), seed, weighted != 0,
                    eng->stream);
         eng->timing.graph_ms = now_ms() - t0;
@@ -1030,6 +1068,7 @@ int kkw_graph_set_edge_types(kkw_engine *eng, const uint8_t *etype)
         if (!eng || !etype) return fail(KKW_E_INVALID, "null argument");
         std::lock_guard<std::recursive_mutex> lk(eng->mu);
         KKW_HIP(hipSetDevice(eng->device));
+        eng->graph_version++;
         set_edge_types(eng->gs, etype, eng->stream);
         return KKW_OK;
     });
@@ -1041,6 +1080,7 @@ int kkw_graph_type_by_dst(kkw_engine *eng, int ntypes, uint64_t seed)
         if (!eng) return fail(KKW_E_INVALID, "engine is null");
         std::lock_guard<std::recursive_mutex> lk(eng->mu);
         KKW_HIP(hipSetDevice(eng->device));
+        eng->graph_version++;
         type_by_dst(eng->gs, ntypes, seed, eng->stream);
         return KKW_OK;
     });
@@ -1085,9 +1125,39 @@ int kkw_prepare(kkw_engine *eng)
         if (!eng) return fail(KKW_E_INVALID, "engine is null");
         std::lock_guard<std::recursive_mutex> lk(eng->mu);
         KKW_HIP(hipSetDevice(eng->device));
+        if (!eng->gs.row_ptr) return fail(KKW_E_NOGRAPH, "no graph loaded");
+        eng->graph_version++;
+        if (eng->gs.w) {
+            double t0 = now_ms();
+            build_alias(eng->gs, eng->stream);
+            eng->timing.alias_ms = now_ms() - t0;
+        }
+        if (eng->adj_index_auto && !eng->gs.adjset) {
+            double t0 = now_ms();
+            build_adj_index(eng->gs, eng->stream);
+            eng->timing.index_ms = now_ms() - t0;
+        }
+        return KKW_OK;
+    });
+}
+
+int kkw_graph_adj_index(kkw_engine *eng, int enable)
+{
+    return guarded([&]() -> int {
+        if (!eng) return fail(KKW_E_INVALID, "engine is null");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        eng->adj_index_auto = enable != 0;
+        eng->graph_version++;
+        if (!enable) {
+            if (eng->gs.adjset) (void)hipFree(eng->gs.adjset);
+            eng->gs.adjset = nullptr;
+            return KKW_OK;
+        }
+        if (!eng->gs.row_ptr) return fail(KKW_E_NOGRAPH, "no graph loaded");
         double t0 = now_ms();
-        build_alias(eng->gs, eng->stream);
-        eng->timing.alias_ms = now_ms() - t0;
+        build_adj_index(eng->gs, eng->stream);
+        eng->timing.index_ms = now_ms() - t0;
         return KKW_OK;
     });
 }

@@ -269,6 +269,36 @@ __global__ void k_check_symmetric(const uint64_t *row_ptr, const uint32_t *col, 
     }
 }
 
+// neighbour hash sets (layout in kkw_dev.h)
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_adjset_insert(const uint64_t *row_ptr, const uint32_t *col, uint32_t V, uint64_t E, uint32_t *t)
+{
+    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    uint32_t lo = 0, hi = V - 1;   // source: last v with row_ptr[v] <= e
+    while (lo < hi) {
+        uint32_t mid = lo + ((hi - lo + 1) >> 1);
+        if (row_ptr[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    const uint32_t v = lo;
+    const uint64_t b = row_ptr[v];
+    const uint32_t size = (uint32_t)(2 * (row_ptr[v + 1] - b) + 4);
+    uint32_t *tab = t + 2 * b + 4 * (uint64_t)v;
+    const uint32_t u = col[e];
+    uint32_t slot = adj_slot(u, size);
+    for (uint32_t n = 0; n < size; n++) {
+        uint32_t old = atomicCAS(tab + slot, ADJ_EMPTY, u);
+        if (old == ADJ_EMPTY || old == u) return;
+        slot = slot + 1 == size ? 0 : slot + 1;
+    }
+}
+
 // -------------------------------------------------------------- types ----
 __global__ void k_vertex_types(uint32_t V, int ntypes, uint64_t seed, uint8_t *vtype)
 {
@@ -432,6 +462,7 @@ void GraphStore::release()
     dfree(vmask);
     dfree(bucket);
     dfree(vsum);
+    dfree(adjset);
     V = 0;
     E = 0;
     sorted = false;
@@ -451,6 +482,7 @@ DevGraph GraphStore::view() const
     g.bucket = bucket;
     g.vsum = vsum;
     g.symmetric = symmetric ? 1u : 0u;
+    g.adjset = adjset;
     return g;
 }
 
@@ -681,6 +713,20 @@ void sort_adjacency(GraphStore &gs, hipStream_t s)
     dfree(gs.vsum);
     gs.sorted = true;
     detect_symmetric(gs, s);
+}
+
+void build_adj_index(GraphStore &gs, hipStream_t s)
+{
+    if (!gs.row_ptr) throw Error(-2, "no graph loaded");
+    if (gs.adjset) return;
+    const uint64_t n = 2 * gs.E + 4 * (uint64_t)gs.V + 4;
+    gs.adjset = dalloc<uint32_t>(n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, gs.adjset, ADJ_EMPTY, n);
+    if (gs.E)
+        hipLaunchKernelGGL(k_adjset_insert, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V,
+                           gs.E, gs.adjset);
+    KKW_HIP(hipGetLastError());
+    KKW_HIP(hipStreamSynchronize(s));
 }
 
 void build_alias(GraphStore &gs, hipStream_t s)

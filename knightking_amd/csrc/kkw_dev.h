@@ -78,7 +78,19 @@ struct DevGraph {
     const Bucket *bucket;      // [E] or null
     const float *vsum;         // [V] or null: sequential float sum of weights
     uint32_t symmetric;        // (u,v) is an edge iff (v,u) is (lists sorted)
+    const uint32_t *adjset;    // null, or per-vertex neighbour hash sets (below)
 };
+
+// Neighbour hash sets: vertex v owns slots [2*row_ptr[v] + 4v, +2*deg(v) + 4) of
+// adjset, an open-addressing table (linear probing, load <= 1/2) of its
+// neighbour ids; empty slots hold ADJ_EMPTY.  The table of v is found from
+// row_ptr[v] alone, and a membership test usually reads one 16-byte block.
+static constexpr uint32_t ADJ_EMPTY = 0xFFFFFFFFu;
+
+__host__ __device__ __forceinline__ uint32_t adj_slot(uint32_t key, uint32_t size)
+{
+    return (uint32_t)(((uint64_t)(key * 0x9E3779B1u) * size) >> 32);
+}
 
 // Device form of a job (host constants precomputed in float).
 struct DevJob {
@@ -105,7 +117,7 @@ struct DevJob {
 };
 
 struct DevStats {
-    unsigned long long steps, trials, searches, probes, row_loads, search_lines;
+    unsigned long long steps, trials, searches, probes, row_loads, search_lines, ref_queries, ref_lines;
 };
 
 }  // namespace kkw

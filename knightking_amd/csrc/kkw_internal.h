@@ -33,6 +33,7 @@ struct GraphStore {
     uint8_t *vmask = nullptr;
     Bucket *bucket = nullptr;
     float *vsum = nullptr;
+    uint32_t *adjset = nullptr;
     bool sorted = false;
     bool symmetric = false;
     void release();
@@ -51,6 +52,7 @@ void build_from_csr(GraphStore &gs, uint32_t V, const uint64_t *h_row_ptr, const
 void set_edge_types(GraphStore &gs, const uint8_t *h_etype, hipStream_t s);
 void type_by_dst(GraphStore &gs, int ntypes, uint64_t seed, hipStream_t s);
 void build_alias(GraphStore &gs, hipStream_t s);
+void build_adj_index(GraphStore &gs, hipStream_t s);
 
 // one job's rows inside a serving flight (packed output)
 struct PackJob {

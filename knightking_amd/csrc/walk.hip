@@ -40,7 +40,16 @@ struct Walker {
 
 struct LaneStats {
     uint32_t steps, trials, searches, probes, rows, lines;
+    uint32_t ref_queries, ref_lines;   // reference byte model: queries KnightKing would send, sum of L_b
 };
+
+// Lines the reference's binary search over a list of d entries is charged:
+// max(1, ceil(log2(d+1)) - 3) (the last four levels share one 64-byte line).
+__device__ __forceinline__ uint32_t ref_search_lines(uint64_t d)
+{
+    uint32_t bits = d ? 64u - (uint32_t)__clzll(d) : 0u;
+    return bits > 4 ? bits - 3 : 1u;
+}
 
 static constexpr uint32_t MAX_TRIALS = 1u << 20;  // per step; guards degenerate inputs
 
@@ -69,6 +78,31 @@ __device__ __forceinline__ bool adj_contains(const uint32_t *__restrict__ col, u
         }
     }
     return false;
+}
+
+// Membership of x among prev's neighbours: the neighbour hash set when the
+// graph has one, else the binary search over prev's sorted list.
+__device__ __forceinline__ bool adj_member(const DevGraph &g, uint32_t prev, uint64_t pbeg, uint64_t pend,
+                                           uint32_t x, uint32_t &probes, uint32_t &lines)
+{
+    if (g.adjset) {
+        const uint32_t *tab = g.adjset + 2 * pbeg + 4 * (uint64_t)prev;
+        const uint32_t size = (uint32_t)(2 * (pend - pbeg) + 4);
+        uint32_t s = adj_slot(x, size);
+        uintptr_t last = ~(uintptr_t)0;
+        for (uint32_t n = 0; n < size; n++) {   // load <= 1/2: an empty slot ends it
+            const uint32_t e = tab[s];
+            probes++;
+            uintptr_t line = (uintptr_t)(tab + s) >> 6;
+            lines += line != last;
+            last = line;
+            if (e == x) return true;
+            if (e == ADJ_EMPTY) return false;
+            s = s + 1 == size ? 0 : s + 1;
+        }
+        return false;
+    }
+    return adj_contains(g.col, pbeg, pend, x, probes, lines);
 }
 
 __device__ __forceinline__ uint32_t start_vertex(const DevGraph &g, const DevJob &J, uint32_t wid)
@@ -195,6 +229,10 @@ __device__ __forceinline__ bool advance(const DevGraph &g, const DevJob &J, Walk
             dart = u01(r.w) * J.upper;            // walk.hpp:1225
             pre = dart <= J.lower;                // walk.hpp:1226-1229
         }
+        if (!pre && s.step > 0 && cand != s.prev && !have) {
+            ls.ref_queries++;
+            ls.ref_lines += ref_search_lines(s.pend - s.pbeg);
+        }
         bool accept;
         if (pre) {
             accept = true;
@@ -208,7 +246,7 @@ __device__ __forceinline__ bool advance(const DevGraph &g, const DevJob &J, Walk
             accept = false;                       // both < dart
         } else {
             uint32_t pr = 0, ln = 0;
-            bool found = adj_contains(g.col, s.pbeg, s.pend, cand, pr, ln);
+            bool found = adj_member(g, s.prev, s.pbeg, s.pend, cand, pr, ln);
             ls.searches++;
             ls.probes += pr;
             ls.lines += ln;
@@ -277,9 +315,10 @@ __device__ __forceinline__ void flush_stats(const LaneStats &ls, DevStats *st)
 {
     if (!STATS) return;
     // wave reduction, one atomic per counter per wave
-    unsigned long long v[6] = {ls.steps, ls.trials, ls.searches, ls.probes, ls.rows, ls.lines};
+    unsigned long long v[8] = {ls.steps, ls.trials, ls.searches, ls.probes, ls.rows, ls.lines,
+                               ls.ref_queries, ls.ref_lines};
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
+    for (int k = 0; k < 8; k++) {
         unsigned long long x = v[k];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
         v[k] = x;
@@ -291,6 +330,8 @@ __device__ __forceinline__ void flush_stats(const LaneStats &ls, DevStats *st)
         atomicAdd(&st->probes, v[3]);
         atomicAdd(&st->row_loads, v[4]);
         atomicAdd(&st->search_lines, v[5]);
+        atomicAdd(&st->ref_queries, v[6]);
+        atomicAdd(&st->ref_lines, v[7]);
     }
 }
 
@@ -299,7 +340,7 @@ template <int KIND, bool ALIAS, bool STATS>
 __global__ void __launch_bounds__(256) k_walk_plain(DevGraph g, DevJob J, DevStats *st)
 {
     uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    LaneStats ls = {0, 0, 0, 0, 0, 0};
+    LaneStats ls = {0, 0, 0, 0, 0, 0, 0, 0};
     if (idx < J.walker_num) {
         Walker s;
         init_walker<KIND>(g, J, 0, idx, s, ls);
@@ -329,7 +370,7 @@ __global__ void __launch_bounds__(256) k_walk_persistent(DevGraph g, DevJob J0, 
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    LaneStats ls = {0, 0, 0, 0, 0, 0};
+    LaneStats ls = {0, 0, 0, 0, 0, 0, 0, 0};
     Walker s;
     bool active = false;
     // wave-uniform pool of walker rows [pool_next, pool_end) of job pool_job
@@ -445,7 +486,7 @@ size_t pack_scan_bytes(uint64_t n)
 // adjacency test).  Lanes at different phases share each memory instruction,
 // so a wave is not held back by its longest search; the 4-entry probe also
 // ends a search up to two levels early.  Paths are identical to advance()'s.
-enum : uint32_t { PH_ROW = 0, PH_TRIAL = 1, PH_SEARCH = 2, PH_CROW = 3 };
+enum : uint32_t { PH_ROW = 0, PH_TRIAL = 1, PH_SEARCH = 2, PH_CROW = 3, PH_HASH = 4 };
 
 typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
 
@@ -461,7 +502,7 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    LaneStats ls = {0, 0, 0, 0, 0, 0};
+    LaneStats ls = {0, 0, 0, 0, 0, 0, 0, 0};
     bool active = false;
     uint32_t idx = 0, wid = 0, cur = 0, prev = 0, step = 0, trial = 0, phase = PH_ROW;
     uint32_t scheme = 0, state = 0, vm = 0;
@@ -470,6 +511,8 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
     uint64_t sbase = 0;
     uint32_t key = 0, slen_all = 0, lo = 0, hi = 0;
     uint32_t cand = 0, rz = 0, rw = 0;
+    uint64_t toff = 0;         // hash probe: table of prev, its size, current slot
+    uint32_t tsize = 0, hslot = 0;
     bool row_known = false;    // the candidate's row_ptr pair is already loaded
     float dart = 0.0f;
     uintptr_t last_line = 0;   // instrumented builds: distinct 64-B lines per search
@@ -533,6 +576,9 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             if (!ALIAS && KIND == KIND_METAPATH) vm = g.etype[beg + k];
         } else if (phase == PH_CROW) {
             addr = reinterpret_cast<const uint32_t *>(g.row_ptr + cand);
+        } else if (phase == PH_HASH) {
+            blk = (toff + hslot) & ~(uint64_t)3;
+            addr = g.adjset + blk;
         } else {
             // aligned block of four entries holding the middle of the range
             blk = (sbase + lo + ((hi - lo) >> 1)) & ~(uint64_t)3;
@@ -588,6 +634,10 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             }
             if (KIND == KIND_NODE2VEC) {
                 dart = u01(rw) * J.upper;
+                if (STATS && step > 0 && !(dart <= J.lower) && cand != prev) {
+                    ls.ref_queries++;
+                    ls.ref_lines += ref_search_lines(pend - pbeg);
+                }
                 if (dart <= J.lower || step == 0) {
                     moved = (dart <= J.lower) || (dart <= J.upper);
                     rejected = !moved;
@@ -598,6 +648,14 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
                     moved = true;
                 } else if (dart > J.acc_hi) {
                     rejected = true;
+                } else if (g.adjset) {
+                    toff = 2 * pbeg + 4 * (uint64_t)prev;
+                    tsize = (uint32_t)(2 * (pend - pbeg) + 4);
+                    hslot = adj_slot(cand, tsize);
+                    phase = PH_HASH;
+                    ls.searches++;
+                    last_line = ~(uintptr_t)0;
+                    continue;
                 } else if (g.symmetric) {
                     phase = PH_CROW;
                     continue;
@@ -620,6 +678,32 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             } else {
                 moved = true;
             }
+        } else if (phase == PH_HASH) {
+            // slots from hslot to the end of this aligned block, in probe order
+            if (STATS) {
+                ls.probes++;
+                uintptr_t line = (uintptr_t)(g.adjset + blk) >> 6;
+                ls.lines += line != last_line;
+                last_line = line;
+            }
+            const uint32_t e4[4] = {v.x, v.y, v.z, v.w};
+            const uint64_t first = toff + hslot, lim = toff + tsize;
+            int res = -1;   // -1 undecided, 0 absent, 1 present
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const uint64_t a = blk + (uint64_t)t;
+                if (res < 0 && a >= first && a < lim) {
+                    if (e4[t] == cand) res = 1;
+                    else if (e4[t] == ADJ_EMPTY) res = 0;
+                }
+            }
+            if (res < 0) {
+                const uint64_t nx = blk + 4;
+                hslot = nx >= lim ? 0u : (uint32_t)(nx - toff);
+                continue;
+            }
+            moved = dart <= (res ? 1.0f : J.inv_q);
+            rejected = !moved;
         } else {
             // PH_SEARCH: entries blk..blk+3 (absolute); those inside the list
             // narrow [lo, hi) from both sides; the middle one is among them

@@ -244,3 +244,40 @@ def test_node2vec_directed_graph(engine, p, q, mode):
     (gpu,) = engine.run([kk.node2vec_job(2000, 30, p, q, True, seed=9)], mode=mode)
     op, ol, ot, st = O.walk(g, O.NODE2VEC, 2000, 30, True, p=p, q=q, seed=9)
     assert_paths_equal(gpu, op, ol, ot, "directed")
+
+
+def test_captured_batch_follows_graph_changes(engine):
+    """A hipGraph-captured batch re-captures after the neighbour hash sets are
+    dropped or rebuilt (same paths), and refuses to run on a replaced graph."""
+    load_rmat(engine, 10)
+    job = kk.node2vec_job(2048, 20, 2.0, 0.5, True, seed=51)
+    (ref,) = engine.run([job], mode=kk.MODE_PLAIN)
+    b = engine.batch([job], kk.MODE_PERSISTENT, kk.BATCH_GRAPH)
+    for enable in (True, False, True):
+        engine.adj_index(enable)
+        b.launch()
+        b.sync()
+        got = b.paths(0)
+        assert np.array_equal(got.lengths, ref.lengths)
+        mask = np.arange(got.vertices.shape[1])[None, :] < got.lengths[:, None]
+        assert np.array_equal(np.where(mask, got.vertices, 0), np.where(mask, ref.vertices, 0))
+    load_rmat(engine, 9)
+    with pytest.raises(kk.KkwError):
+        b.launch()
+    b.close()
+
+
+@pytest.mark.parametrize("index", [True, False])
+def test_node2vec_with_and_without_hash_sets(engine, index):
+    """The adjacency test answers the same through the neighbour hash sets and
+    through the binary search (both launch forms, weighted and unweighted)."""
+    load_rmat(engine, 12)
+    engine.adj_index(index)
+    rp, col, w = O.rmat_graph(12, 16, seed=1, weighted=True)
+    g = O.Graph(rp, col, w)
+    for weighted in (True, False):
+        op, ol, ot, st = O.walk(g, O.NODE2VEC, 4096, 40, weighted, p=2.0, q=0.5, seed=61)
+        for mode in (kk.MODE_PLAIN, kk.MODE_PERSISTENT):
+            (gpu,) = engine.run([kk.node2vec_job(4096, 40, 2.0, 0.5, weighted, seed=61)], mode=mode)
+            assert_paths_equal(gpu, op, ol, ot, f"index={index} weighted={weighted} mode={mode}")
+    engine.adj_index(True)
