@@ -154,7 +154,7 @@ static PreparedJob prepare_job(kkw_engine *eng, const kkw_job &j)
     }
     if (L > (1u << 24)) throw Error(KKW_E_INVALID, "walk_length above 2^24");
     d.walk_length = L;
-    pj.stride = L + 1;
+    pj.stride = (L + 1 + 3) & ~3u;   // rows of whole 16-byte groups (path write-out)
     d.stride = pj.stride;
     if (j.kind == KKW_NODE2VEC) {
         if (!(j.p > 0.0f && std::isfinite(j.p) && j.q > 0.0f && std::isfinite(j.q)))
@@ -171,6 +171,10 @@ static PreparedJob prepare_job(kkw_engine *eng, const kkw_job &j)
             build_adj_index(gs, eng->stream);
             eng->timing.index_ms = now_ms() - t0;
         }
+    }
+    if (!j.use_weights && !gs.ubucket) {   // uniform draws read uniform buckets
+        eng->graph_version++;
+        build_uniform_buckets(gs, eng->stream);
     }
     if (j.use_weights) {
         if (!gs.w) throw Error(KKW_E_INVALID, "use_weights on a graph without weights");
@@ -341,7 +345,8 @@ static void batch_launch(kkw_batch *b)
     if (!eng->gs.row_ptr || eng->gs.V != b->graph_V || eng->gs.E != b->graph_E)
         throw Error(KKW_E_INVALID, "the graph changed since this batch was created; create a new batch");
     for (auto &pj : b->jobs)   // tables a job needs may have been dropped since
-        if ((pj.dj.use_alias && !eng->gs.bucket) || (pj.dj.kind == KKW_METAPATH && !eng->gs.vmask))
+        if ((pj.dj.use_alias && !eng->gs.bucket) || (!pj.dj.use_alias && !eng->gs.ubucket) ||
+            (pj.dj.kind == KKW_METAPATH && !eng->gs.vmask))
             throw Error(KKW_E_INVALID, "the graph's alias tables or edge types changed; create a new batch");
     if (b->gexec && b->captured_version != eng->graph_version) {   // stale pointers in the capture
         (void)hipGraphExecDestroy(b->gexec);

@@ -377,6 +377,8 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
         uint2 se = S[sh++];
         float ps = __uint_as_float(se.y);
         Bucket bk;
+        bk.prow = 0;
+        bk.arow = 0;
         if (lh != lt) {
             uint2 le = L[lh++];
             float pl = __uint_as_float(le.y);
@@ -400,12 +402,49 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
     while (lh != lt) {
         uint2 le = L[lh++];
         Bucket bk;
+        bk.prow = 0;
+        bk.arow = 0;
         bk.p = 1.0f;
         bk.primary = cv[le.x];
         bk.alias = bk.primary;
         bk.meta = tv ? (uint32_t)tv[le.x] * 0x101u : 0u;
         *o++ = bk;
     }
+}
+
+// rows of each bucket's candidates (a separate pass keeps the per-vertex
+// construction above free of random reads)
+__global__ void k_bucket_rows(Bucket *b, uint64_t E, const uint64_t *row_ptr)
+{
+    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const uint32_t p = b[e].primary, a = b[e].alias;
+    b[e].prow = pack_row(row_ptr[p], row_ptr[p + 1]);
+    b[e].arow = p == a ? b[e].prow : pack_row(row_ptr[a], row_ptr[a + 1]);
+}
+
+// uniform candidates as buckets: p = 1, alias = primary = col[e]
+__global__ void k_uniform_buckets(const uint32_t *col, const uint8_t *etype, const uint64_t *row_ptr, uint64_t E,
+                                  int rows, Bucket *out)
+{
+    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const uint32_t u = col[e];
+    Bucket b;
+    b.p = 1.0f;
+    b.primary = u;
+    b.alias = u;
+    b.meta = etype ? (uint32_t)etype[e] * 0x101u : 0u;
+    b.prow = rows ? pack_row(row_ptr[u], row_ptr[u + 1]) : 0;
+    b.arow = b.prow;
+    out[e] = b;
+}
+
+__global__ void k_max_degree(const uint64_t *row_ptr, uint32_t V, unsigned long long *out)
+{
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    atomicMax(out, (unsigned long long)(row_ptr[v + 1] - row_ptr[v]));
 }
 
 unsigned long long read_counter(unsigned long long *d, hipStream_t s)
@@ -437,6 +476,19 @@ void check_graph(GraphStore &gs, hipStream_t s)
     gs.sorted = hs != 0;
 }
 
+// rows pack into 40 + 24 bits when E < 2^40 and every degree < 2^24
+void compute_packable(GraphStore &gs, hipStream_t s)
+{
+    gs.packable = false;
+    if (gs.V == 0 || gs.E >= (1ull << ROW_DEG_SHIFT)) return;
+    unsigned long long *mx = dalloc<unsigned long long>(1);
+    KKW_HIP(hipMemsetAsync(mx, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_max_degree, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V, mx);
+    KKW_HIP(hipGetLastError());
+    gs.packable = read_counter(mx, s) < (1ull << (64 - ROW_DEG_SHIFT));
+    dfree(mx);
+}
+
 void detect_symmetric(GraphStore &gs, hipStream_t s)
 {
     gs.symmetric = false;
@@ -461,12 +513,14 @@ void GraphStore::release()
     dfree(etype);
     dfree(vmask);
     dfree(bucket);
+    dfree(ubucket);
     dfree(vsum);
     dfree(adjset);
     V = 0;
     E = 0;
     sorted = false;
     symmetric = false;
+    packable = false;
 }
 
 DevGraph GraphStore::view() const
@@ -480,6 +534,8 @@ DevGraph GraphStore::view() const
     g.etype = etype;
     g.vmask = vmask;
     g.bucket = bucket;
+    g.ubucket = ubucket;
+    g.packed_rows = packable ? 1u : 0u;
     g.vsum = vsum;
     g.symmetric = symmetric ? 1u : 0u;
     g.adjset = adjset;
@@ -540,6 +596,7 @@ void build_rmat(GraphStore &gs, int scale, int edge_factor, uint32_t thr_a, uint
     dfree(m1);
     gs.sorted = true;
     gs.symmetric = true;   // every edge was mirrored
+    compute_packable(gs, s);
 }
 
 void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const uint32_t *h_dst,
@@ -607,6 +664,7 @@ void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const u
     check_graph(gs, s);
     if (undirected) gs.symmetric = true;   // every edge was mirrored
     else detect_symmetric(gs, s);
+    compute_packable(gs, s);
     if (gs.etype) rebuild_vmask(gs, s);
 }
 
@@ -628,6 +686,7 @@ void build_from_csr(GraphStore &gs, uint32_t V, const uint64_t *h_row_ptr, const
     KKW_HIP(hipStreamSynchronize(s));
     check_graph(gs, s);
     detect_symmetric(gs, s);
+    compute_packable(gs, s);
 }
 
 static void rebuild_vmask(GraphStore &gs, hipStream_t s)
@@ -652,6 +711,7 @@ void set_edge_types(GraphStore &gs, const uint8_t *h_etype, hipStream_t s)
     if (gs.E) KKW_HIP(hipMemcpyAsync(gs.etype, h_etype, gs.E, hipMemcpyHostToDevice, s));
     rebuild_vmask(gs, s);
     dfree(gs.bucket);  // bucket meta carries types: rebuild on next prepare
+    dfree(gs.ubucket);
     dfree(gs.vsum);
 }
 
@@ -671,6 +731,7 @@ void type_by_dst(GraphStore &gs, int ntypes, uint64_t seed, hipStream_t s)
     dfree(vt);
     rebuild_vmask(gs, s);
     dfree(gs.bucket);
+    dfree(gs.ubucket);
     dfree(gs.vsum);
 }
 
@@ -710,6 +771,7 @@ void sort_adjacency(GraphStore &gs, hipStream_t s)
     if (w2) { dfree(gs.w); gs.w = w2; }
     if (et2) { dfree(gs.etype); gs.etype = et2; }
     dfree(gs.bucket);
+    dfree(gs.ubucket);
     dfree(gs.vsum);
     gs.sorted = true;
     detect_symmetric(gs, s);
@@ -729,6 +791,18 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
     KKW_HIP(hipStreamSynchronize(s));
 }
 
+void build_uniform_buckets(GraphStore &gs, hipStream_t s)
+{
+    if (!gs.row_ptr) throw Error(-2, "no graph loaded");
+    if (gs.ubucket) return;
+    gs.ubucket = dalloc<Bucket>(gs.E);
+    if (gs.E)
+        hipLaunchKernelGGL(k_uniform_buckets, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.col, gs.etype, gs.row_ptr,
+                           gs.E, gs.packable ? 1 : 0, gs.ubucket);
+    KKW_HIP(hipGetLastError());
+    KKW_HIP(hipStreamSynchronize(s));
+}
+
 void build_alias(GraphStore &gs, hipStream_t s)
 {
     if (!gs.row_ptr) throw Error(-2, "no graph loaded");
@@ -740,6 +814,8 @@ void build_alias(GraphStore &gs, hipStream_t s)
     uint2 *qs = dalloc<uint2>(gs.E), *ql = dalloc<uint2>(gs.E);
     hipLaunchKernelGGL(k_alias, dim3(grid_for(gs.V, 64)), dim3(64), 0, s, gs.row_ptr, gs.col, gs.w,
                        gs.etype, gs.V, gs.bucket, gs.vsum, qs, ql);
+    if (gs.packable && gs.E)
+        hipLaunchKernelGGL(k_bucket_rows, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.bucket, gs.E, gs.row_ptr);
     KKW_HIP(hipGetLastError());
     KKW_HIP(hipStreamSynchronize(s));
     dfree(qs);

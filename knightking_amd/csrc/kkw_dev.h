@@ -57,15 +57,27 @@ __host__ __device__ __forceinline__ uint32_t bounded(uint32_t lo, uint32_t hi, u
     return (uint32_t)((a + (b >> 32)) >> 32);
 }
 
-// One alias bucket (16 B, never straddles a cache line): the candidate's vertex
-// ids are stored directly, so a weighted draw costs one random access.
-// meta = edge type of primary | edge type of alias << 8.
-struct __align__(16) Bucket {
+// One alias bucket (32 B, half a 64-byte line): the candidates' vertex ids and
+// their packed rows (list begin | degree << 40) are stored directly, so a draw
+// and the move it leads to cost one random access.  meta = edge type of
+// primary | edge type of alias << 8.  Rows are 0 when the graph cannot pack
+// them (2^40 or more edges, or a degree of 2^24 or more).
+struct __align__(32) Bucket {
     float p;
     uint32_t primary;
     uint32_t alias;
     uint32_t meta;
+    uint64_t prow;
+    uint64_t arow;
 };
+
+static constexpr int ROW_DEG_SHIFT = 40;
+static constexpr uint64_t ROW_BEG_MASK = (1ull << ROW_DEG_SHIFT) - 1;
+
+__host__ __device__ __forceinline__ uint64_t pack_row(uint64_t beg, uint64_t end)
+{
+    return beg | ((end - beg) << ROW_DEG_SHIFT);
+}
 
 struct DevGraph {
     uint32_t V;
@@ -75,9 +87,11 @@ struct DevGraph {
     const float *w;            // [E] or null
     const uint8_t *etype;      // [E] or null
     const uint8_t *vmask;      // [V] or null: OR of (1 << etype) over out-edges
-    const Bucket *bucket;      // [E] or null
+    const Bucket *bucket;      // [E] or null: alias tables of the edge weights
+    const Bucket *ubucket;     // [E] or null: uniform buckets (p = 1, alias = primary)
     const float *vsum;         // [V] or null: sequential float sum of weights
     uint32_t symmetric;        // (u,v) is an edge iff (v,u) is (lists sorted)
+    uint32_t packed_rows;      // buckets carry the rows of their candidates
     const uint32_t *adjset;    // null, or per-vertex neighbour hash sets (below)
 };
 

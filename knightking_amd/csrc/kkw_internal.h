@@ -32,10 +32,12 @@ struct GraphStore {
     uint8_t *etype = nullptr;
     uint8_t *vmask = nullptr;
     Bucket *bucket = nullptr;
+    Bucket *ubucket = nullptr;
     float *vsum = nullptr;
     uint32_t *adjset = nullptr;
     bool sorted = false;
     bool symmetric = false;
+    bool packable = false;   // every row fits the packed form
     void release();
     DevGraph view() const;
 };
@@ -53,6 +55,7 @@ void set_edge_types(GraphStore &gs, const uint8_t *h_etype, hipStream_t s);
 void type_by_dst(GraphStore &gs, int ntypes, uint64_t seed, hipStream_t s);
 void build_alias(GraphStore &gs, hipStream_t s);
 void build_adj_index(GraphStore &gs, hipStream_t s);
+void build_uniform_buckets(GraphStore &gs, hipStream_t s);
 
 // one job's rows inside a serving flight (packed output)
 struct PackJob {

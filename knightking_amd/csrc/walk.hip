@@ -27,6 +27,26 @@ namespace kkw {
 
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int n);
 
+// Path write-out: each lane keeps the current group of four path entries in
+// registers and stores it as one aligned 16-byte write when the group fills or
+// the walk ends (rows hold a multiple of four entries; entries past a path's
+// length are left unspecified).
+struct PathGroup {
+    uint32_t q0, q1, q2, q3;
+    __device__ __forceinline__ void set(uint32_t i, uint32_t v)
+    {
+        const uint32_t k = i & 3u;
+        q0 = k == 0 ? v : q0;
+        q1 = k == 1 ? v : q1;
+        q2 = k == 2 ? v : q2;
+        q3 = k == 3 ? v : q3;
+    }
+    __device__ __forceinline__ void store(uint32_t *row, uint32_t i) const
+    {
+        *reinterpret_cast<uint4 *>(row + (i & ~3u)) = make_uint4(q0, q1, q2, q3);
+    }
+};
+
 struct Walker {
     uint32_t job;      // job index (persistent batches)
     uint32_t idx;      // row within the job
@@ -36,6 +56,7 @@ struct Walker {
     uint64_t pbeg, pend; // list of prev
     uint32_t step, trial;
     uint32_t scheme, state;
+    PathGroup pg;
 };
 
 struct LaneStats {
@@ -137,43 +158,59 @@ __device__ __forceinline__ void init_walker(const DevGraph &g, const DevJob &J, 
     s.beg = g.row_ptr[s.cur];
     s.end = g.row_ptr[s.cur + 1];
     ls.rows++;
-    J.paths[(uint64_t)idx * J.stride] = s.cur;
+    s.pg.q0 = s.pg.q1 = s.pg.q2 = s.pg.q3 = s.cur;
 }
 
 __device__ __forceinline__ void finish_walker(const DevJob &J, const Walker &s, bool truncated)
 {
+    if ((s.step & 3u) != 3u) s.pg.store(J.paths + (uint64_t)s.idx * J.stride, s.step);
     J.lens[s.idx] = (s.step + 1) | (truncated ? 0x80000000u : 0u);
 }
 
+// Move to cand; its row comes from the bucket when row_ok, else from row_ptr.
 __device__ __forceinline__ void move_to(const DevGraph &g, const DevJob &J, Walker &s,
-                                        uint32_t cand, LaneStats &ls)
+                                        uint32_t cand, LaneStats &ls, bool row_ok = false, uint64_t row = 0)
 {
     s.step++;
-    J.paths[(uint64_t)s.idx * J.stride + s.step] = cand;
+    s.pg.set(s.step, cand);
+    if ((s.step & 3u) == 3u) s.pg.store(J.paths + (uint64_t)s.idx * J.stride, s.step);
     s.prev = s.cur;
     s.pbeg = s.beg;
     s.pend = s.end;
     s.cur = cand;
-    s.beg = g.row_ptr[cand];
-    s.end = g.row_ptr[cand + 1];
+    if (row_ok) {
+        s.beg = row & ROW_BEG_MASK;
+        s.end = s.beg + (row >> ROW_DEG_SHIFT);
+    } else {
+        s.beg = g.row_ptr[cand];
+        s.end = g.row_ptr[cand + 1];
+        ls.rows++;
+    }
     s.trial = 0;
     ls.steps++;
-    ls.rows++;
 }
 
 // Candidate by alias bucket or uniform index (include/walk.hpp:1016-1029).
+// Uniform draws read the uniform buckets (p = 1: always the primary, which is
+// col[beg + k]) when they exist.  row_ok/row: the candidate's packed row.
 template <bool ALIAS_KNOWN, bool ALIAS>
 __device__ __forceinline__ uint32_t pick(const DevGraph &g, const DevJob &J, const Walker &s,
-                                         uint32_t deg, const u32x4 &r, uint32_t *etype)
+                                         uint32_t deg, const u32x4 &r, uint32_t *etype, bool &row_ok,
+                                         uint64_t &row)
 {
     uint32_t k = bounded(r.x, r.y, deg);
     const bool use_alias = ALIAS_KNOWN ? ALIAS : (J.use_alias != 0);
-    if (use_alias) {
-        Bucket b = g.bucket[s.beg + k];
+    const Bucket *tab = use_alias ? g.bucket : g.ubucket;
+    if (tab) {
+        Bucket b = tab[s.beg + k];
         bool prim = u01(r.z) < b.p;
         if (etype) *etype = prim ? (b.meta & 0xffu) : ((b.meta >> 8) & 0xffu);
+        row_ok = g.packed_rows != 0;
+        row = prim ? b.prow : b.arow;
         return prim ? b.primary : b.alias;
     }
+    row_ok = false;
+    row = 0;
     if (etype) *etype = g.etype ? g.etype[s.beg + k] : 0u;
     return g.col[s.beg + k];
 }
@@ -196,6 +233,8 @@ __device__ __forceinline__ bool advance(const DevGraph &g, const DevJob &J, Walk
         uint32_t cand = 0;
         float dart = 0.0f;
         bool have = false, pre = false;
+        bool row_ok = false;
+        uint64_t crow = 0;
         if (J.overflow > 0.0f && s.step > 0) {
             // Outlier appendix (walk.hpp:1181-1208, node2vec.hpp:47-106).
             const uint32_t *base = g.col + s.beg;
@@ -225,7 +264,7 @@ __device__ __forceinline__ bool advance(const DevGraph &g, const DevJob &J, Walk
         }
         if (!have) {
             u32x4 r = draw(J.seed, s.wid, s.step, s.trial, TAG_TRIAL);
-            cand = pick<ALIAS_KNOWN, ALIAS>(g, J, s, deg, r, nullptr);
+            cand = pick<ALIAS_KNOWN, ALIAS>(g, J, s, deg, r, nullptr, row_ok, crow);
             dart = u01(r.w) * J.upper;            // walk.hpp:1225
             pre = dart <= J.lower;                // walk.hpp:1226-1229
         }
@@ -252,7 +291,7 @@ __device__ __forceinline__ bool advance(const DevGraph &g, const DevJob &J, Walk
             ls.lines += ln;
             accept = dart <= (found ? 1.0f : J.inv_q);   // node2vec.hpp:187-191
         }
-        if (accept) move_to(g, J, s, cand, ls);
+        if (accept) move_to(g, J, s, cand, ls, row_ok, crow);
         else s.trial++;
         return false;
     }
@@ -291,22 +330,24 @@ __device__ __forceinline__ bool advance(const DevGraph &g, const DevJob &J, Walk
     }
     ls.trials++;
     u32x4 r = draw(J.seed, s.wid, s.step, s.trial, TAG_TRIAL);
+    bool row_ok = false;
+    uint64_t crow = 0;
     if (kind == KIND_METAPATH) {
         uint32_t et = 0;
-        uint32_t cand = pick<ALIAS_KNOWN, ALIAS>(g, J, s, deg, r, &et);
+        uint32_t cand = pick<ALIAS_KNOWN, ALIAS>(g, J, s, deg, r, &et, row_ok, crow);
         float dart = u01(r.w) * 1.0f;
         float pd = ((J.scheme_mask[s.scheme * 8 + s.state] >> et) & 1u) ? 1.0f : 0.0f;
         if (dart <= pd) {                                          // walk.hpp:1030-1036
             uint32_t slen = J.scheme_len[s.scheme];
-            move_to(g, J, s, cand, ls);
+            move_to(g, J, s, cand, ls, row_ok, crow);
             s.state = (s.state + 1) % slen;                        // metapath.hpp:96-99
         } else {
             s.trial++;
         }
         return false;
     }
-    uint32_t cand = pick<ALIAS_KNOWN, ALIAS>(g, J, s, deg, r, nullptr);
-    move_to(g, J, s, cand, ls);
+    uint32_t cand = pick<ALIAS_KNOWN, ALIAS>(g, J, s, deg, r, nullptr, row_ok, crow);
+    move_to(g, J, s, cand, ls, row_ok, crow);
     return false;
 }
 
@@ -502,22 +543,30 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
+    // candidates come from buckets: the alias tables, or the uniform buckets
+    // (both 32 B, both carrying the candidates' rows when the graph packs them)
+    const Bucket *const tab = ALIAS ? g.bucket : g.ubucket;
+    const bool rows_in_buckets = g.packed_rows != 0 && KIND != KIND_METAPATH;
     LaneStats ls = {0, 0, 0, 0, 0, 0, 0, 0};
     bool active = false;
     uint32_t idx = 0, wid = 0, cur = 0, prev = 0, step = 0, trial = 0, phase = PH_ROW;
     uint32_t scheme = 0, state = 0, vm = 0;
     uint64_t beg = 0, end = 0, pbeg = 0, pend = 0, cbeg = 0, cend = 0;
-    // search: key in the list [sbase, sbase + slen_all), current range [lo, hi)
+    // A search and a hash probe are never live together, so they share state:
+    // sorted-list search: key in [sbase, sbase + slen_all), current range [lo, hi);
+    // hash probe: table of prev at toff (= sbase), tsize (= slen_all) slots, at
+    // slot hslot (= lo).
     uint64_t sbase = 0;
     uint32_t key = 0, slen_all = 0, lo = 0, hi = 0;
+    uint64_t &toff = sbase;
+    uint32_t &tsize = slen_all, &hslot = lo;
     uint32_t cand = 0, rz = 0, rw = 0;
-    uint64_t toff = 0;         // hash probe: table of prev, its size, current slot
-    uint32_t tsize = 0, hslot = 0;
-    bool row_known = false;    // the candidate's row_ptr pair is already loaded
+    bool row_known = false;    // the candidate's row is already in cbeg/cend
     float dart = 0.0f;
     uintptr_t last_line = 0;   // instrumented builds: distinct 64-B lines per search
     uint32_t pool_next = 0, pool_end = 0;
     bool drained = false;
+    PathGroup pg = {0u, 0u, 0u, 0u};
     uint32_t *const paths = J.paths;
     const uint32_t stride = J.stride;
     for (;;) {
@@ -551,7 +600,7 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
                     scheme = bounded(r.x, r.y, J.n_schemes);
                     state = 0;
                 }
-                paths[(uint64_t)idx * stride] = cur;
+                pg.q0 = pg.q1 = pg.q2 = pg.q3 = cur;
                 active = true;
             }
             uint32_t want = (uint32_t)__popcll(need);
@@ -561,7 +610,7 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
         if (__ballot(active) == 0ull) break;
         if (!active) continue;
 
-        // ---- the one load of this iteration
+        // ---- the load of this iteration (a trial reads both halves of its bucket)
         const uint32_t *addr;
         uint64_t blk = 0;
         if (phase == PH_ROW) {
@@ -572,8 +621,7 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             uint32_t k = bounded(r.x, r.y, (uint32_t)(end - beg));
             rz = r.z;
             rw = r.w;
-            addr = ALIAS ? reinterpret_cast<const uint32_t *>(g.bucket + beg + k) : g.col + beg + k;
-            if (!ALIAS && KIND == KIND_METAPATH) vm = g.etype[beg + k];
+            addr = reinterpret_cast<const uint32_t *>(tab + beg + k);
         } else if (phase == PH_CROW) {
             addr = reinterpret_cast<const uint32_t *>(g.row_ptr + cand);
         } else if (phase == PH_HASH) {
@@ -585,52 +633,51 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             addr = g.col + blk;
         }
         const u32x4 v = load16(addr);
+        u32x4 v2 = {0u, 0u, 0u, 0u};
+        if (phase == PH_TRIAL && rows_in_buckets) v2 = load16(addr + 4);
 
         // ---- consume it
         bool moved = false, rejected = false;
-        if (phase == PH_CROW) {
-            cbeg = (uint64_t)v.x | ((uint64_t)v.y << 32);
-            cend = (uint64_t)v.z | ((uint64_t)v.w << 32);
-            row_known = true;
+        if (phase == PH_ROW || phase == PH_CROW) {
+            const uint64_t b0 = (uint64_t)v.x | ((uint64_t)v.y << 32);
+            const uint64_t e0 = (uint64_t)v.z | ((uint64_t)v.w << 32);
             ls.rows++;
-            // prev's list holds cand iff cand's list holds prev (symmetric
-            // graph): search the shorter one
-            const uint64_t pl = pend - pbeg, cl = cend - cbeg;
-            if (cl < pl) {
-                sbase = cbeg;
-                slen_all = (uint32_t)cl;
-                key = prev;
+            if (phase == PH_ROW) {
+                beg = b0;
+                end = e0;
             } else {
-                sbase = pbeg;
-                slen_all = (uint32_t)pl;
-                key = cand;
+                cbeg = b0;
+                cend = e0;
+                row_known = true;
+                // prev's list holds cand iff cand's list holds prev (symmetric
+                // graph): search the shorter one
+                const uint64_t pl = pend - pbeg, cl = cend - cbeg;
+                sbase = cl < pl ? cbeg : pbeg;
+                slen_all = (uint32_t)(cl < pl ? cl : pl);
+                key = cl < pl ? prev : cand;
+                lo = 0;
+                hi = slen_all;
+                if (hi == 0) {   // an empty list holds nothing
+                    moved = dart <= J.inv_q;
+                    rejected = !moved;
+                } else {
+                    phase = PH_SEARCH;
+                    ls.searches++;
+                    last_line = ~(uintptr_t)0;
+                    continue;
+                }
             }
-            lo = 0;
-            hi = slen_all;
-            if (hi == 0) {   // cand has no neighbours: prev is not one of them
-                moved = dart <= J.inv_q;
-                rejected = !moved;
-            } else {
-                phase = PH_SEARCH;
-                ls.searches++;
-                last_line = ~(uintptr_t)0;
-                continue;
-            }
-        } else if (phase == PH_ROW) {
-            beg = (uint64_t)v.x | ((uint64_t)v.y << 32);
-            end = (uint64_t)v.z | ((uint64_t)v.w << 32);
-            ls.rows++;
         } else if (phase == PH_TRIAL) {
             ls.trials++;
-            uint32_t et = 0;
-            row_known = false;
-            if (ALIAS) {
-                bool prim = u01(rz) < __uint_as_float(v.x);
-                cand = prim ? v.y : v.z;
-                et = prim ? (v.w & 0xffu) : ((v.w >> 8) & 0xffu);
-            } else {
-                cand = v.x;
-                et = vm;
+            const bool prim = u01(rz) < __uint_as_float(v.x);
+            cand = prim ? v.y : v.z;
+            const uint32_t et = prim ? (v.w & 0xffu) : ((v.w >> 8) & 0xffu);
+            row_known = rows_in_buckets;
+            if (rows_in_buckets) {
+                const uint64_t row = prim ? ((uint64_t)v2.x | ((uint64_t)v2.y << 32))
+                                          : ((uint64_t)v2.z | ((uint64_t)v2.w << 32));
+                cbeg = row & ROW_BEG_MASK;
+                cend = cbeg + (row >> ROW_DEG_SHIFT);
             }
             if (KIND == KIND_NODE2VEC) {
                 dart = u01(rw) * J.upper;
@@ -657,8 +704,25 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
                     last_line = ~(uintptr_t)0;
                     continue;
                 } else if (g.symmetric) {
-                    phase = PH_CROW;
-                    continue;
+                    if (!row_known) {
+                        phase = PH_CROW;
+                        continue;
+                    }
+                    const uint64_t pl = pend - pbeg, cl = cend - cbeg;
+                    sbase = cl < pl ? cbeg : pbeg;
+                    slen_all = (uint32_t)(cl < pl ? cl : pl);
+                    key = cl < pl ? prev : cand;
+                    lo = 0;
+                    hi = slen_all;
+                    if (hi == 0) {
+                        moved = dart <= J.inv_q;
+                        rejected = !moved;
+                    } else {
+                        phase = PH_SEARCH;
+                        ls.searches++;
+                        last_line = ~(uintptr_t)0;
+                        continue;
+                    }
                 } else {
                     sbase = pbeg;
                     slen_all = (uint32_t)(pend - pbeg);
@@ -739,6 +803,7 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             trial++;
             phase = PH_TRIAL;
             if (trial >= MAX_TRIALS) {
+                if ((step & 3u) != 3u) pg.store(paths + (uint64_t)idx * stride, step);
                 J.lens[idx] = (step + 1) | 0x80000000u;
                 active = false;
             }
@@ -746,7 +811,8 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
         }
         if (moved) {
             step++;
-            paths[(uint64_t)idx * stride + step] = cand;
+            pg.set(step, cand);
+            if ((step & 3u) == 3u) pg.store(paths + (uint64_t)idx * stride, step);
             if (KIND == KIND_METAPATH) state = (state + 1) % J.scheme_len[scheme];
             prev = cur;
             pbeg = beg;
@@ -760,7 +826,6 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             }
             beg = cbeg;
             end = cend;
-            row_known = false;
         }
         // row of cur known: termination test of the new step (walk.hpp:956-973)
         bool stop = end == beg;
@@ -777,6 +842,7 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             }
         }
         if (stop) {
+            if ((step & 3u) != 3u) pg.store(paths + (uint64_t)idx * stride, step);
             J.lens[idx] = (step + 1) | (capped ? 0x80000000u : 0u);
             active = false;
             continue;
