@@ -23,7 +23,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, published
-LINE = 64                   # bytes charged per data-dependent random access
+LINE = 64                   # bytes per random access in the reference's per-step model
+L2_LINE = 128               # bytes one L2 miss fetches on MI355X (TCC_EA0_RDREQ are 128 B)
 CALIB_BYTES = 1 << 30       # streaming-copy calibration size (read once, written once)
 
 
@@ -42,11 +43,18 @@ def reference_model_bytes(t, weighted):
     return (LINE + 4) * t["steps"] + per_trial * t["trials"] + LINE * t["ref_query_lines"]
 
 
-def touched_bytes(t, walkers):
-    """Bytes this engine's kernel touches: one 64 B line per row_ptr pair,
-    alias bucket (or list entry) and distinct line of an adjacency probe, plus
-    path and length bytes."""
-    return LINE * (t["row_loads"] + t["trials"] + t["search_lines"]) + 4 * (t["steps"] + walkers) + 4 * walkers
+def path_bytes(t, walkers, stride=84):
+    """Path rows (whole 16-byte groups actually written) plus lengths."""
+    groups = (t["steps"] + walkers + 3 * walkers) // 4   # upper bound: one partial group per walker
+    return min(16 * groups, 4 * stride * walkers) + 4 * walkers
+
+
+def kernel_bytes(t, walkers, line):
+    """Bytes this engine's kernel must move: one `line` per data-dependent
+    random access (row_ptr pairs read at walker starts, alias buckets -- a
+    32-byte bucket lies in one line -- and each distinct block of an adjacency
+    probe), plus the path and length bytes written."""
+    return line * (t["row_loads"] + t["trials"] + t["search_lines"]) + path_bytes(t, walkers)
 
 
 def time_batch(b, steps, warmup):
@@ -205,11 +213,12 @@ def main():
     t = eng.timing()
     sb.close()
     ref_bytes = reference_model_bytes(t, weighted)
-    touched = touched_bytes(t, V)
-    bytes_launch = touched   # this kernel's algorithmic bytes (64 B per data-dependent access)
+    touched = kernel_bytes(t, V, LINE)
+    bytes_launch = kernel_bytes(t, V, L2_LINE)   # algorithmic bytes: one L2 line per random access
     log(f"stats: steps={t['steps']} trials={t['trials']} searches={t['searches']} lines={t['search_lines']} "
         f"rows={t['row_loads']} ref_queries={t['ref_queries']} ref_lines={t['ref_query_lines']} "
-        f"reference model bytes/step={ref_bytes / max(t['steps'], 1):.1f} touched/step={touched / max(t['steps'], 1):.1f}")
+        f"reference model bytes/step={ref_bytes / max(t['steps'], 1):.1f} "
+        f"kernel bytes/step={bytes_launch / max(t['steps'], 1):.1f} (64 B lines: {touched / max(t['steps'], 1):.1f})")
 
     pb = eng.batch([job], kk.MODE_PERSISTENT, kk.BATCH_GRAPH)
     shape = (256, 8, 64)
@@ -310,8 +319,11 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes_per_launch": int(bytes_launch),
             "algorithmic_bytes_per_step": round(bytes_launch / max(steps_per_launch, 1), 1),
+            "bytes_model": "one 128 B L2 line per data-dependent random access + path/length bytes",
+            "bytes_per_step_64B_lines": round(touched / max(steps_per_launch, 1), 1),
             "reference_model_bytes_per_step": round(ref_bytes / max(steps_per_launch, 1), 1),
             "reference_model_gbs": round(ref_bytes / kernel_s / 1e9, 1),
+            "traffic_gbs": None if traffic is None else round(traffic / kernel_s / 1e9, 1),
             "kernel_ms": round(kernel_s * 1e3, 3),
             "trials_per_step": round(t["trials"] / max(t["steps"], 1), 4),
             "searches_per_step": round(t["searches"] / max(t["steps"], 1), 4),
