@@ -70,7 +70,7 @@ def time_batch(b, steps, warmup):
     return wall, kms
 
 
-def cpu_baseline(eng, job_args, gpu_batch, V, seconds=15.0):
+def cpu_baseline(eng, job_args, gpu_batch, V, seconds=20.0):
     """Oracle on host cores over a strided sample of the walkers; returns
     (steps/s, threads, sample description, parity summary)."""
     from oracle import oracle as O
@@ -107,6 +107,75 @@ def cpu_baseline(eng, job_args, gpu_batch, V, seconds=15.0):
               f"(k < {n}), L=80, {st.steps} steps in {dt:.1f}s, {threads} threads")
     parity = {"walkers": int(n), "bit_exact": bool(same)}
     return rate, threads, sample, parity
+
+
+def mixed_jobs(kk, n_jobs=64, seed=4, n_types=5):
+    """64 jobs: kind uniform over PPR (t = 1/80), DeepWalk (L in [40,120]) and
+    metapath (schemes of 3 to 5 states, each a non-empty subset of the types,
+    L in [40,120]); walkers round(10^(3+3U)); every job has its own seed."""
+    rng = np.random.default_rng(seed)
+    jobs, kinds = [], []
+    for i in range(n_jobs):
+        walkers = int(round(10 ** (3 + 3 * rng.random())))
+        kind = int(rng.integers(0, 3))
+        L = int(rng.integers(40, 121))
+        if kind == 0:
+            jobs.append(kk.ppr_job(walkers, 1.0 / 80, True, seed=100 + i))
+        elif kind == 1:
+            jobs.append(kk.deepwalk_job(walkers, L, True, seed=100 + i))
+        else:
+            schemes = [[int(rng.integers(1, 1 << n_types)) for _ in range(int(rng.integers(3, 6)))]
+                       for _ in range(4)]
+            jobs.append(kk.metapath_job(walkers, L, schemes, True, seed=100 + i))
+        kinds.append(("ppr", "deepwalk", "metapath")[kind])
+    return jobs, kinds
+
+
+def batched_vs_plain(kk, steps, warmup, scale=23):
+    """The 64 mixed jobs in one persistent launch vs one plain launch per job."""
+    eng = kk.Engine()
+    try:
+        eng.graph_rmat(scale, 16, 0.57, 0.19, 0.19, seed=2, weighted=True)
+        eng.type_by_dst(5, 3)
+        eng.prepare()
+        jobs, kinds = mixed_jobs(kk)
+        out = {"graph": f"R-MAT scale {scale}, 5 vertex types (edge type = type of its destination)",
+               "jobs": len(jobs), "walkers": int(sum(j.walker_num for j in jobs)),
+               "kinds": {k: kinds.count(k) for k in set(kinds)}}
+        for name, mode, flags in (("persistent", kk.MODE_PERSISTENT, kk.BATCH_GRAPH), ("plain", kk.MODE_PLAIN, 0)):
+            b = eng.batch(jobs, mode, flags)
+            wall, kms = time_batch(b, steps, warmup)
+            total = sum(b.steps(i) for i in range(len(jobs)))
+            out[name + "_steps_per_s"] = round(total * steps / wall, 1)
+            out[name + "_ms"] = round(float(np.mean(kms)), 3)
+            out["steps_per_launch"] = int(total)
+            b.close()
+        out["persistent_over_plain"] = round(out["persistent_steps_per_s"] / out["plain_steps_per_s"], 3)
+        log(f"batched mixed jobs: {out}")
+        return out
+    finally:
+        eng.close()
+
+
+def deepwalk_line(kk, steps, warmup, scale=22):
+    """DeepWalk with weighted alias sampling on R-MAT-22, 4,194,304 walkers, L=80."""
+    eng = kk.Engine()
+    try:
+        eng.graph_rmat(scale, 16, 0.57, 0.19, 0.19, seed=1, weighted=True)
+        eng.prepare()
+        job = kk.deepwalk_job(1 << scale, 80, True, seed=1)
+        out = {}
+        for name, mode, flags in (("persistent", kk.MODE_PERSISTENT, kk.BATCH_GRAPH), ("plain", kk.MODE_PLAIN, 0)):
+            b = eng.batch([job], mode, flags)
+            wall, kms = time_batch(b, steps, warmup)
+            s = b.steps(0)
+            out[name + "_steps_per_s"] = round(s * steps / wall, 1)
+            out["steps_per_launch"] = int(s)
+            b.close()
+        log(f"deepwalk R-MAT-{scale}: {out}")
+        return out
+    finally:
+        eng.close()
 
 
 def serve_latency(eng, rp, rates, seconds, seed=7):
@@ -176,7 +245,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--autotune", action="store_true", help="search persistent launch shapes first")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default persistent launch shape")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched-mixed and DeepWalk lines")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--calib", action="store_true",
                     help="profiling pass: run the calibration copy only twice")
@@ -222,8 +292,8 @@ def main():
 
     pb = eng.batch([job], kk.MODE_PERSISTENT, kk.BATCH_GRAPH)
     shape = (256, 8, 64)
-    if args.autotune:
-        shape = pb.autotune(os.path.join(ROOT, "profiles", "autotune_cache.txt"))
+    if not args.no_autotune:
+        shape = pb.autotune(None)
         log(f"autotuned shape block/blocks_per_cu/chunk = {shape}")
     wall_p, kms_p = time_batch(pb, args.steps, args.warmup)
     steps_per_launch = pb.steps(0)
@@ -276,6 +346,15 @@ def main():
             vb.close()
             log(f"{name}: {variants[name] / 1e9:.3f} G steps/s")
         eng.adj_index(True)
+
+    extra_batched = extra_deepwalk = None
+    if not args.no_extra and weighted and args.scale == 24:
+        eng.close()
+        extra_batched = batched_vs_plain(kk, args.steps, args.warmup)
+        extra_deepwalk = deepwalk_line(kk, args.steps, args.warmup)
+        eng = kk.Engine()
+        eng.graph_rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=1, weighted=True)
+        eng.prepare()
 
     serving = None
     if not args.no_serve:
@@ -332,6 +411,8 @@ def main():
         },
         "cpu_baseline": cpu,
         "sample_parity": parity,
+        "batched_mixed_jobs": extra_batched,
+        "deepwalk_rmat22": extra_deepwalk,
         "ppr_serving": serving,
     }
     print(json.dumps(line), flush=True)
