@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
                                               const float *__restrict__ w,
                                               const uint8_t *__restrict__ etype, uint32_t V,
                                               Bucket *__restrict__ out, float *__restrict__ vsum,
-                                              uint2 *__restrict__ qs, uint2 *__restrict__ ql)
+                                              uint2 *__restrict__ qs, uint2 *__restrict__ ql, uint32_t big_threshold)
 {
     uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
@@ -355,6 +355,7 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
         vsum[v] = 0.0f;
         return;
     }
+    if (d >= big_threshold) return;   // k_alias_wave's
     const float *wv = w + b;
     float sum = 0.0f;
     for (uint32_t e = 0; e < d; e++) sum += wv[e];
@@ -410,6 +411,196 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
         bk.meta = tv ? (uint32_t)tv[le.x] * 0x101u : 0u;
         *o++ = bk;
     }
+}
+
+
+// ------------------------------------------------ alias, longer lists ----
+// The same construction for lists of AW_MIN or more entries, one wave per
+// vertex.  The float sum stays sequential (lane 0, in edge order, from weights
+// staged in LDS); the split into small/large queues keeps edge order (ballot +
+// prefix popcount).  The pairing runs in batches of up to 64: while the queues
+// hold k entries ahead of both heads, the next k pairings take small entries
+// S[sh..sh+k) and large entries L[lh..lh+k) -- each pairing pops a different
+// large entry, so the k pairings are independent -- and their re-queued large
+// entries go to the tails in pairing order (ballot again).  This is exactly the
+// sequential FIFO run k steps at a time.  Queue windows of AW_WIN entries live
+// in LDS; lists longer than that also keep their queues in global memory and
+// refill the windows from there.  Buckets first hold local edge indices; the
+// last pass turns them into neighbour ids and edge types.
+constexpr uint32_t AW_MIN = 32;
+constexpr uint32_t AW_WIN = 512;
+constexpr int AW_WAVES = 4;
+constexpr uint32_t AW_SUM_CHUNK = 256;
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__restrict__ list, uint32_t n_list,
+                                                            const uint64_t *__restrict__ row_ptr,
+                                                            const uint32_t *__restrict__ col,
+                                                            const float *__restrict__ w,
+                                                            const uint8_t *__restrict__ etype,
+                                                            Bucket *__restrict__ out, float *__restrict__ vsum,
+                                                            uint2 *__restrict__ qs, uint2 *__restrict__ ql)
+{
+    __shared__ uint2 win_s[AW_WAVES][AW_WIN], win_l[AW_WAVES][AW_WIN];
+    __shared__ float stage[AW_WAVES][AW_SUM_CHUNK];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t idx = blockIdx.x * AW_WAVES + wave;
+    if (idx >= n_list) return;   // whole wave; nothing below synchronises across waves
+    const uint32_t v = list[idx];
+    const uint64_t b = row_ptr[v];
+    const uint32_t d = (uint32_t)(row_ptr[v + 1] - b);
+    const float *wv = w + b;
+    uint2 *S = qs + b, *L = ql + b;
+    uint2 *sS = win_s[wave], *sL = win_l[wave];
+    float *sw = stage[wave];
+    Bucket *o = out + b;
+    const bool spill = d > AW_WIN;   // queues also kept in global memory
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+
+    // 1. sequential float sum in edge order (walk.hpp:591-595)
+    float nx[AW_SUM_CHUNK / 64];
+#pragma unroll
+    for (uint32_t j = 0; j < AW_SUM_CHUNK / 64; j++) nx[j] = j * 64 + lane < d ? wv[j * 64 + lane] : 0.0f;
+    float sum = 0.0f;
+    for (uint32_t base = 0; base < d; base += AW_SUM_CHUNK) {
+#pragma unroll
+        for (uint32_t j = 0; j < AW_SUM_CHUNK / 64; j++) sw[j * 64 + lane] = nx[j];
+        wave_sync();
+        const uint32_t nb = base + AW_SUM_CHUNK;
+#pragma unroll
+        for (uint32_t j = 0; j < AW_SUM_CHUNK / 64; j++) nx[j] = nb + j * 64 + lane < d ? wv[nb + j * 64 + lane] : 0.0f;
+        if (lane == 0) {
+            const uint32_t n = d - base < AW_SUM_CHUNK ? d - base : AW_SUM_CHUNK;
+            for (uint32_t i = 0; i < n; i++) sum += sw[i];
+        }
+        wave_sync();
+    }
+    sum = __shfl(sum, 0);
+    if (lane == 0) vsum[v] = sum;
+    const float ave = sum / (float)d;
+
+    // 2. small / large queues in edge order (walk.hpp:601-615)
+    uint32_t st = 0, lt = 0;
+    for (uint32_t base = 0; base < d; base += 64) {
+        const uint32_t e = base + lane;
+        const bool valid = e < d;
+        const float p = valid ? wv[e] : 0.0f;
+        const bool small = valid && p < ave, large = valid && !(p < ave);
+        const uint64_t ms = __ballot(small), ml = __ballot(large);
+        const uint2 ent = make_uint2(e, __float_as_uint(p));
+        if (small) {
+            const uint32_t pos = st + (uint32_t)__popcll(ms & lt_mask);
+            if (spill) S[pos] = ent;
+            if (pos < AW_WIN) sS[pos] = ent;
+        }
+        if (large) {
+            const uint32_t pos = lt + (uint32_t)__popcll(ml & lt_mask);
+            if (spill) L[pos] = ent;
+            if (pos < AW_WIN) sL[pos] = ent;
+        }
+        st += (uint32_t)__popcll(ms);
+        lt += (uint32_t)__popcll(ml);
+    }
+    wave_sync();
+
+    // 3. FIFO pairing, k steps at a time (walk.hpp:617-634)
+    uint32_t sh = 0, lh = 0, ws = 0, wl = 0, ob = 0;
+    while (sh != st) {
+        const bool paired = lh != lt;
+        const uint32_t av_s = (st < ws + AW_WIN ? st : ws + AW_WIN) - sh;
+        const uint32_t av_l = paired ? (lt < wl + AW_WIN ? lt : wl + AW_WIN) - lh : 64u;
+        if ((av_s < 64 && st > ws + AW_WIN) || (av_l < 64 && paired && lt > wl + AW_WIN)) {
+            // refill both windows from the global queues at the current heads
+            for (uint32_t i = lane; i < AW_WIN; i += 64) {
+                if (sh + i < st) sS[i] = S[sh + i];
+                if (lh + i < lt) sL[i] = L[lh + i];
+            }
+            ws = sh;
+            wl = lh;
+            wave_sync();
+            continue;
+        }
+        uint32_t k = av_s < av_l ? av_s : av_l;
+        k = k < 64 ? k : 64;
+        const bool act = lane < k;
+        Bucket bk;
+        bk.meta = 0;
+        bk.prow = 0;
+        bk.arow = 0;
+        bool to_s = false, to_l = false;
+        uint2 ne = make_uint2(0, 0);
+        if (act) {
+            const uint2 se = sS[sh - ws + lane];
+            const float ps = __uint_as_float(se.y);
+            bk.primary = se.x;
+            if (paired) {
+                const uint2 le = sL[lh - wl + lane];
+                float pl = __uint_as_float(le.y);
+                bk.p = ps / ave;
+                bk.alias = le.x;
+                const float diff = ave - ps;
+                pl -= diff;
+                ne = make_uint2(le.x, __float_as_uint(pl));
+                to_s = pl < ave;
+                to_l = !to_s;
+            } else {
+                bk.p = 1.0f;
+                bk.alias = se.x;
+            }
+            o[ob + lane] = bk;
+        }
+        const uint64_t ms = __ballot(to_s), ml = __ballot(to_l);
+        if (to_s) {
+            const uint32_t pos = st + (uint32_t)__popcll(ms & lt_mask);
+            if (spill) S[pos] = ne;
+            if (pos < ws + AW_WIN) sS[pos - ws] = ne;
+        }
+        if (to_l) {
+            const uint32_t pos = lt + (uint32_t)__popcll(ml & lt_mask);
+            if (spill) L[pos] = ne;
+            if (pos < wl + AW_WIN) sL[pos - wl] = ne;
+        }
+        st += (uint32_t)__popcll(ms);
+        lt += (uint32_t)__popcll(ml);
+        sh += k;
+        if (paired) lh += k;
+        ob += k;
+        wave_sync();
+    }
+    // 4. leftover large entries (walk.hpp:635-647), in queue order
+    for (uint32_t i = lane; lh + i < lt; i += 64) {
+        const uint32_t pos = lh + i;
+        const uint32_t e = pos < wl + AW_WIN ? sL[pos - wl].x : L[pos].x;
+        Bucket bk;
+        bk.p = 1.0f;
+        bk.primary = e;
+        bk.alias = e;
+        bk.meta = 0;
+        bk.prow = 0;
+        bk.arow = 0;
+        o[ob + i] = bk;
+    }
+    wave_sync();
+    // 5. local edge indices -> neighbour ids and edge types
+    for (uint32_t i = lane; i < d; i += 64) {
+        const uint32_t pe = o[i].primary, ae = o[i].alias;
+        o[i].meta = etype ? ((uint32_t)etype[b + pe] | ((uint32_t)etype[b + ae] << 8)) : 0u;
+        o[i].primary = col[b + pe];
+        o[i].alias = col[b + ae];
+    }
+}
+
+__global__ void k_list_big(const uint64_t *row_ptr, uint32_t V, uint32_t *big, unsigned int *n_big)
+{
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    if (row_ptr[v + 1] - row_ptr[v] >= AW_MIN) big[atomicAdd(n_big, 1u)] = (uint32_t)v;
 }
 
 // rows of each bucket's candidates (a separate pass keeps the per-vertex
@@ -812,14 +1003,27 @@ void build_alias(GraphStore &gs, hipStream_t s)
     gs.bucket = dalloc<Bucket>(gs.E);
     gs.vsum = dalloc<float>(gs.V);
     uint2 *qs = dalloc<uint2>(gs.E), *ql = dalloc<uint2>(gs.E);
+    // lists of AW_MIN or more entries: one wave each; the rest: one lane each
+    uint32_t *big = dalloc<uint32_t>(gs.V);
+    unsigned int *n_big_d = dalloc<unsigned int>(1);
+    KKW_HIP(hipMemsetAsync(n_big_d, 0, sizeof(unsigned int), s));
+    hipLaunchKernelGGL(k_list_big, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V, big, n_big_d);
+    unsigned int n_big = 0;
+    KKW_HIP(hipMemcpyAsync(&n_big, n_big_d, sizeof(n_big), hipMemcpyDeviceToHost, s));
+    KKW_HIP(hipStreamSynchronize(s));
+    if (n_big)
+        hipLaunchKernelGGL(k_alias_wave, dim3(grid_for(n_big, AW_WAVES)), dim3(64 * AW_WAVES), 0, s, big, n_big,
+                           gs.row_ptr, gs.col, gs.w, gs.etype, gs.bucket, gs.vsum, qs, ql);
     hipLaunchKernelGGL(k_alias, dim3(grid_for(gs.V, 64)), dim3(64), 0, s, gs.row_ptr, gs.col, gs.w,
-                       gs.etype, gs.V, gs.bucket, gs.vsum, qs, ql);
+                       gs.etype, gs.V, gs.bucket, gs.vsum, qs, ql, AW_MIN);
     if (gs.packable && gs.E)
         hipLaunchKernelGGL(k_bucket_rows, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.bucket, gs.E, gs.row_ptr);
     KKW_HIP(hipGetLastError());
     KKW_HIP(hipStreamSynchronize(s));
     dfree(qs);
     dfree(ql);
+    dfree(big);
+    dfree(n_big_d);
 }
 
 }  // namespace kkw

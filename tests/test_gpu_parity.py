@@ -79,6 +79,49 @@ def test_alias_matches_oracle(engine, scale):
     assert np.array_equal(vs.view(np.uint32), vs_o.view(np.uint32))
 
 
+def _hub_graph(seed):
+    """CSR whose long lists cover the workgroup-per-vertex alias build: lists
+    just below, at and above 1024 entries and up to 300k, with weight shapes
+    that drive the small/large queues differently (uniform, heavy-tailed, all
+    equal, one dominant weight, one tiny weight)."""
+    rng = np.random.default_rng(seed)
+    degs = [3, 1023, 1024, 1025, 4096, 70000, 300000, 5000, 5000, 5000, 2048, 17]
+    V = len(degs) + 64
+    degs = degs + [0] * (V - len(degs))
+    rp = np.zeros(V + 1, np.uint64)
+    rp[1:] = np.cumsum(degs)
+    E = int(rp[-1])
+    col = rng.integers(0, V, E).astype(np.uint32)
+    w = (1.0 + 4.0 * rng.random(E)).astype(np.float32)
+    def seg(i):
+        return slice(int(rp[i]), int(rp[i + 1]))
+    w[seg(5)] = (rng.exponential(1.0, degs[5]) ** 3 + 1e-3).astype(np.float32)
+    w[seg(7)] = 2.5
+    w[seg(8)] = 1.0
+    w[int(rp[8])] = 1e4
+    w[seg(9)] = 1.0
+    w[int(rp[9]) + 17] = 1e-6
+    w[seg(10)] = np.where(rng.random(degs[10]) < 0.05, 50.0, 0.5).astype(np.float32)
+    return rp, col, w
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_alias_matches_oracle_long_lists(engine, seed):
+    rp, col, w = _hub_graph(seed)
+    print(f"[gpu] alias on long lists, seed {seed}, E={int(rp[-1])}", flush=True)
+    engine.graph_from_csr(rp, col, w)
+    et = np.random.default_rng(seed + 7).integers(0, 5, len(col)).astype(np.uint8)
+    engine.set_edge_types(et)
+    engine.prepare()
+    p, prim, al, meta, vs = engine.alias_download()
+    p_o, prim_o, al_o, meta_o, vs_o = O.alias_build(rp, col, w, et)
+    assert np.array_equal(vs.view(np.uint32), vs_o.view(np.uint32))
+    assert np.array_equal(p.view(np.uint32), p_o.view(np.uint32))
+    assert np.array_equal(prim, prim_o)
+    assert np.array_equal(al, al_o)
+    assert np.array_equal(meta, meta_o)
+
+
 def _walk_both(engine, g, mode, **job):
     kind = job.pop("kind")
     n = job.pop("walker_num")
