@@ -57,19 +57,23 @@ __host__ __device__ __forceinline__ uint32_t bounded(uint32_t lo, uint32_t hi, u
     return (uint32_t)((a + (b >> 32)) >> 32);
 }
 
-// One alias bucket (32 B, half a 64-byte line): the candidates' vertex ids and
-// their packed rows (list begin | degree << 40) are stored directly, so a draw
-// and the move it leads to cost one random access.  meta = edge type of
-// primary | edge type of alias << 8.  Rows are 0 when the graph cannot pack
-// them (2^40 or more edges, or a degree of 2^24 or more).
+// One alias bucket (32 B, a quarter of a 128-byte line) in two 16-byte
+// halves: {p, primary, primary's packed row} and {alias, meta, alias's packed
+// row}.  A draw whose coin picks the primary -- always, for uniform buckets --
+// reads only the first half; the vertex id and the row (list begin | degree <<
+// 40) of the candidate are both there, so a draw and the move it leads to
+// cost one 16-byte random access.  meta = edge type of primary | edge type of
+// alias << 8.  Rows are 0 when the graph cannot pack them (2^40 or more edges,
+// or a degree of 2^24 or more).
 struct __align__(32) Bucket {
     float p;
     uint32_t primary;
+    uint64_t prow;
     uint32_t alias;
     uint32_t meta;
-    uint64_t prow;
     uint64_t arow;
 };
+static_assert(sizeof(Bucket) == 32, "bucket is 32 bytes");
 
 static constexpr int ROW_DEG_SHIFT = 40;
 static constexpr uint64_t ROW_BEG_MASK = (1ull << ROW_DEG_SHIFT) - 1;

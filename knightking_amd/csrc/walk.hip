@@ -190,6 +190,15 @@ __device__ __forceinline__ void move_to(const DevGraph &g, const DevJob &J, Walk
     ls.steps++;
 }
 
+// 16-byte load from a 4-byte aligned address (one global_load_dwordx4)
+typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+__device__ __forceinline__ u32x4 load16(const uint32_t *p)
+{
+    u32x4_a4 v = *reinterpret_cast<const u32x4_a4 *>(p);
+    return u32x4{v.x, v.y, v.z, v.w};
+}
+
 // Candidate by alias bucket or uniform index (include/walk.hpp:1016-1029).
 // Uniform draws read the uniform buckets (p = 1: always the primary, which is
 // col[beg + k]) when they exist.  row_ok/row: the candidate's packed row.
@@ -202,12 +211,20 @@ __device__ __forceinline__ uint32_t pick(const DevGraph &g, const DevJob &J, con
     const bool use_alias = ALIAS_KNOWN ? ALIAS : (J.use_alias != 0);
     const Bucket *tab = use_alias ? g.bucket : g.ubucket;
     if (tab) {
-        Bucket b = tab[s.beg + k];
-        bool prim = u01(r.z) < b.p;
-        if (etype) *etype = prim ? (b.meta & 0xffu) : ((b.meta >> 8) & 0xffu);
+        // first half {p, primary, prow}; the second {alias, meta, arow} only
+        // when the coin picks the alias or the edge type is wanted
+        const uint32_t *bp = reinterpret_cast<const uint32_t *>(tab + s.beg + k);
+        const u32x4 a = load16(bp);
+        const bool prim = u01(r.z) < __uint_as_float(a.x);
         row_ok = g.packed_rows != 0;
-        row = prim ? b.prow : b.arow;
-        return prim ? b.primary : b.alias;
+        if (prim && !etype) {
+            row = (uint64_t)a.z | ((uint64_t)a.w << 32);
+            return a.y;
+        }
+        const u32x4 h = load16(bp + 4);
+        if (etype) *etype = prim ? (h.y & 0xffu) : ((h.y >> 8) & 0xffu);
+        row = prim ? ((uint64_t)a.z | ((uint64_t)a.w << 32)) : ((uint64_t)h.z | ((uint64_t)h.w << 32));
+        return prim ? a.y : h.x;
     }
     row_ok = false;
     row = 0;
@@ -522,20 +539,13 @@ size_t pack_scan_bytes(uint64_t n)
 // ------------------------------------------------- state-machine kernel ----
 // Persistent single-job kernel in which every loop iteration issues exactly
 // one 16-byte load per active lane: the row_ptr pair of the current vertex
-// (ROW), the alias bucket or list entry of a trial (TRIAL), or four
-// consecutive entries of the previous vertex's sorted list (SEARCH, node2vec
-// adjacency test).  Lanes at different phases share each memory instruction,
+// (ROW), the first half of a trial's alias bucket (TRIAL) and, when its coin
+// picks the alias, the second half (ALIAS), or four consecutive entries of a
+// sorted list or hash set (SEARCH, HASH: node2vec adjacency test).  Lanes at different phases share each memory instruction,
 // so a wave is not held back by its longest search; the 4-entry probe also
 // ends a search up to two levels early.  Paths are identical to advance()'s.
-enum : uint32_t { PH_ROW = 0, PH_TRIAL = 1, PH_SEARCH = 2, PH_CROW = 3, PH_HASH = 4 };
+enum : uint32_t { PH_ROW = 0, PH_TRIAL = 1, PH_SEARCH = 2, PH_CROW = 3, PH_HASH = 4, PH_ALIAS = 5 };
 
-typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
-
-__device__ __forceinline__ u32x4 load16(const uint32_t *p)
-{
-    u32x4_a4 v = *reinterpret_cast<const u32x4_a4 *>(p);
-    return u32x4{v.x, v.y, v.z, v.w};
-}
 
 template <int KIND, bool ALIAS, bool STATS>
 __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t n_chunks, uint32_t chunk,
@@ -555,7 +565,8 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
     // A search and a hash probe are never live together, so they share state:
     // sorted-list search: key in [sbase, sbase + slen_all), current range [lo, hi);
     // hash probe: table of prev at toff (= sbase), tsize (= slen_all) slots, at
-    // slot hslot (= lo).
+    // slot hslot (= lo).  Between a trial and its ALIAS load, sbase holds the
+    // bucket's index.
     uint64_t sbase = 0;
     uint32_t key = 0, slen_all = 0, lo = 0, hi = 0;
     uint64_t &toff = sbase;
@@ -610,7 +621,8 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
         if (__ballot(active) == 0ull) break;
         if (!active) continue;
 
-        // ---- the load of this iteration (a trial reads both halves of its bucket)
+        // ---- the load of this iteration (a trial without rows in its buckets
+        // reads both halves of the bucket at once)
         const uint32_t *addr;
         uint64_t blk = 0;
         if (phase == PH_ROW) {
@@ -621,7 +633,10 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             uint32_t k = bounded(r.x, r.y, (uint32_t)(end - beg));
             rz = r.z;
             rw = r.w;
-            addr = reinterpret_cast<const uint32_t *>(tab + beg + k);
+            sbase = beg + k;
+            addr = reinterpret_cast<const uint32_t *>(tab + sbase);
+        } else if (phase == PH_ALIAS) {
+            addr = reinterpret_cast<const uint32_t *>(tab + sbase) + 4;
         } else if (phase == PH_CROW) {
             addr = reinterpret_cast<const uint32_t *>(g.row_ptr + cand);
         } else if (phase == PH_HASH) {
@@ -634,7 +649,7 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
         }
         const u32x4 v = load16(addr);
         u32x4 v2 = {0u, 0u, 0u, 0u};
-        if (phase == PH_TRIAL && rows_in_buckets) v2 = load16(addr + 4);
+        if (phase == PH_TRIAL && !rows_in_buckets) v2 = load16(addr + 4);
 
         // ---- consume it
         bool moved = false, rejected = false;
@@ -667,15 +682,29 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
                     continue;
                 }
             }
-        } else if (phase == PH_TRIAL) {
-            ls.trials++;
-            const bool prim = u01(rz) < __uint_as_float(v.x);
-            cand = prim ? v.y : v.z;
-            const uint32_t et = prim ? (v.w & 0xffu) : ((v.w >> 8) & 0xffu);
+        } else if (phase == PH_TRIAL || phase == PH_ALIAS) {
+            // bucket halves: v = {p, primary, prow} (TRIAL) or {alias, meta, arow}
+            // (ALIAS); without rows in the buckets v2 is the second half
+            uint32_t et = 0;
+            if (phase == PH_TRIAL) {
+                ls.trials++;
+                const bool prim = u01(rz) < __uint_as_float(v.x);
+                if (rows_in_buckets) {
+                    if (!prim) {
+                        phase = PH_ALIAS;
+                        continue;
+                    }
+                    cand = v.y;
+                } else {
+                    cand = prim ? v.y : v2.x;
+                    et = prim ? (v2.y & 0xffu) : ((v2.y >> 8) & 0xffu);
+                }
+            } else {
+                cand = v.x;
+            }
             row_known = rows_in_buckets;
             if (rows_in_buckets) {
-                const uint64_t row = prim ? ((uint64_t)v2.x | ((uint64_t)v2.y << 32))
-                                          : ((uint64_t)v2.z | ((uint64_t)v2.w << 32));
+                const uint64_t row = (uint64_t)v.z | ((uint64_t)v.w << 32);
                 cbeg = row & ROW_BEG_MASK;
                 cend = cbeg + (row >> ROW_DEG_SHIFT);
             }
