@@ -26,6 +26,7 @@ HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, published
 LINE = 64                   # bytes per random access in the reference's per-step model
 L2_LINE = 128               # bytes one L2 miss fetches on MI355X (TCC_EA0_RDREQ are 128 B)
 CALIB_BYTES = 1 << 30       # streaming-copy calibration size (read once, written once)
+GATHER_BYTES = 16 << 30     # random-gather ceiling: buffer about the size of the R-MAT-24 alias tables
 
 
 def log(*a):
@@ -316,6 +317,16 @@ def main():
     copy_gbs = 2 * CALIB_BYTES / (copy_ms / 1e3) / 1e9
     log(f"streaming copy of {CALIB_BYTES >> 20} MiB: {copy_ms:.3f} ms, {copy_gbs:.0f} GB/s read+write")
 
+    # ceiling for independent random 16-byte reads (one 128-byte line each),
+    # against which the kernel's own rate of data-dependent random reads is set
+    gather_lps = None
+    if not args.calib:
+        gather_lps = eng.random_gather_lines_per_s(GATHER_BYTES)
+        log(f"random 16-byte reads over {GATHER_BYTES >> 30} GiB: {gather_lps / 1e9:.2f} G lines/s "
+            f"({gather_lps * L2_LINE / 1e9:.0f} GB/s of 128-byte lines)")
+    accesses = t["row_loads"] + t["trials"] + t["search_lines"]
+    access_rate = accesses / kernel_s
+
     achieved = bytes_launch / kernel_s / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "node2vec_rmat24_hbm_traffic.json")
@@ -408,6 +419,10 @@ def main():
             "searches_per_step": round(t["searches"] / max(t["steps"], 1), 4),
             "search_lines_per_search": round(t["search_lines"] / max(t["searches"], 1), 3),
             "stream_copy_gbs": round(copy_gbs, 1),
+            "random_accesses_per_s": round(access_rate, 1),
+            "random_gather_lines_per_s": None if gather_lps is None else round(gather_lps, 1),
+            "random_gather_gbs": None if gather_lps is None else round(gather_lps * L2_LINE / 1e9, 1),
+            "frac_of_random_gather": None if gather_lps is None else round(access_rate / gather_lps, 4),
         },
         "cpu_baseline": cpu,
         "sample_parity": parity,

@@ -222,6 +222,14 @@ int kkw_autotune(kkw_batch *b, const char *cache_file, int *block, int *blocks_p
  * byte counters are scaled by. */
 int kkw_diag_stream_copy(kkw_engine *eng, uint64_t bytes, int iters, double *ms_per_iter);
 
+/* Measurement aid (no counterpart in the reference): independent random
+ * 16-byte reads, each from its own 128-byte line of a `bytes` buffer,
+ * `reads` per thread over enough threads to fill the device, repeated `iters`
+ * times; gives the rate (lines per second) at which the memory system serves
+ * such reads -- the ceiling of a kernel whose every access is a data-dependent
+ * 16-byte read, as a random walk's are. */
+int kkw_diag_random_gather(kkw_engine *eng, uint64_t bytes, int reads, int iters, double *lines_per_s);
+
 /* Serving: a resident loop admits queries (usually PPR) by free HBM and slot
  * count, batches everything admitted into rounds of the persistent kernel, and
  * retires queries as their last walker finishes. */

@@ -82,6 +82,7 @@ EXPORTED = [
     "kkw_prepare", "kkw_graph_adj_index", "kkw_alias_download", "kkw_run", "kkw_paths_free", "kkw_paths_dump",
     "kkw_last_timing", "kkw_batch_create", "kkw_batch_set_shape", "kkw_batch_launch",
     "kkw_batch_sync", "kkw_batch_paths", "kkw_batch_destroy", "kkw_autotune", "kkw_diag_stream_copy",
+    "kkw_diag_random_gather",
     "kkw_serve_start", "kkw_submit", "kkw_poll", "kkw_wait", "kkw_cancel", "kkw_serve_stop",
 ]
 
@@ -128,6 +129,7 @@ def load_library(path: str = LIB_PATH):
         "kkw_batch_destroy": (I32, [P]),
         "kkw_autotune": (I32, [P, C.c_char_p, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
         "kkw_diag_stream_copy": (I32, [P, U64, I32, C.POINTER(D)]),
+        "kkw_diag_random_gather": (I32, [P, U64, I32, I32, C.POINTER(D)]),
         "kkw_serve_start": (I32, [P, C.POINTER(_ServeOpts)]),
         "kkw_submit": (I32, [P, C.POINTER(_Job), C.POINTER(U64)]),
         "kkw_poll": (I32, [P, U64, C.POINTER(I32)]),
@@ -422,6 +424,12 @@ class Engine:
         ms = C.c_double()
         _check(_lib.kkw_diag_stream_copy(self.h, nbytes, iters, C.byref(ms)))
         return ms.value
+
+    def random_gather_lines_per_s(self, nbytes: int, reads: int = 64, iters: int = 5) -> float:
+        """Independent random 16-byte reads served per second, one 128-byte line each."""
+        r = C.c_double()
+        _check(_lib.kkw_diag_random_gather(self.h, nbytes, reads, iters, C.byref(r)))
+        return r.value
 
     # ---- serving ---------------------------------------------------------
     def serve_start(self, max_walkers_in_flight=1 << 22, queue_capacity=1 << 16):

@@ -1363,6 +1363,35 @@ int kkw_diag_stream_copy(kkw_engine *eng, uint64_t bytes, int iters, double *ms_
     });
 }
 
+int kkw_diag_random_gather(kkw_engine *eng, uint64_t bytes, int reads, int iters, double *lines_per_s)
+{
+    return guarded([&]() -> int {
+        if (!eng || iters < 1 || reads < 1 || bytes < 128) return fail(KKW_E_INVALID, "bad argument");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        bytes &= ~(uint64_t)127;
+        DevBuf a(bytes), sink(256);
+        KKW_HIP(hipMemsetAsync(a.p, 1, bytes, eng->stream));
+        const uint32_t threads = (uint32_t)eng->cu_count * 32u * 256u;
+        hipEvent_t e0, e1;
+        KKW_HIP(hipEventCreate(&e0));
+        KKW_HIP(hipEventCreate(&e1));
+        launch_random_gather(a.p, bytes, threads, (uint32_t)reads, 1, (uint32_t *)sink.p, eng->stream);   // warm
+        KKW_HIP(hipEventRecord(e0, eng->stream));
+        for (int i = 0; i < iters; i++)
+            launch_random_gather(a.p, bytes, threads, (uint32_t)reads, 2 + (uint64_t)i, (uint32_t *)sink.p, eng->stream);
+        KKW_HIP(hipEventRecord(e1, eng->stream));
+        KKW_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        KKW_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        KKW_HIP(hipGetLastError());
+        if (lines_per_s) *lines_per_s = (double)threads * (double)reads * iters / (ms * 1e-3);
+        return KKW_OK;
+    });
+}
+
 int kkw_autotune(kkw_batch *b, const char *cache_file, int *block, int *blocks_per_cu, int *chunk)
 {
     return guarded([&]() -> int {
@@ -1396,28 +1425,39 @@ int kkw_autotune(kkw_batch *b, const char *cache_file, int *block, int *blocks_p
                 fclose(f);
             }
         }
+        // one launch per shape, then the four fastest again, three launches
+        // each, ranked by their median (a single launch is too noisy to rank
+        // shapes within a few percent of each other)
         const int blocks[] = {64, 128, 256};
         const int bpcs[] = {4, 8, 16, 32};
         const int chunks[] = {16, 64, 256};
-        double best = 1e300;
-        int bb = b->block, bbp = b->blocks_per_cu, bc = b->chunk;
+        struct Cand {
+            double ms;
+            int bl, bp, ch;
+        };
+        std::vector<Cand> cands;
+        auto time_shape = [&](int bl, int bp, int ch) -> double {
+            int rc = kkw_batch_set_shape(b, bl, bp, ch);
+            if (rc) throw Error(rc, kkw_last_error());
+            std::lock_guard<std::recursive_mutex> lk(b->eng->mu);
+            batch_launch(b);
+            return batch_sync(b);
+        };
         for (int bl : blocks)
             for (int bp : bpcs) {
                 if (bl * bp > 2048) continue;  // at most 32 waves per CU
-                for (int ch : chunks) {
-                    int rc = kkw_batch_set_shape(b, bl, bp, ch);
-                    if (rc) return rc;
-                    std::lock_guard<std::recursive_mutex> lk(b->eng->mu);
-                    batch_launch(b);
-                    double ms = batch_sync(b);
-                    if (ms < best) {
-                        best = ms;
-                        bb = bl;
-                        bbp = bp;
-                        bc = ch;
-                    }
-                }
+                for (int ch : chunks) cands.push_back({time_shape(bl, bp, ch), bl, bp, ch});
             }
+        std::sort(cands.begin(), cands.end(), [](const Cand &x, const Cand &y) { return x.ms < y.ms; });
+        if (cands.size() > 4) cands.resize(4);
+        for (Cand &c : cands) {
+            double t[3];
+            for (double &x : t) x = time_shape(c.bl, c.bp, c.ch);
+            std::sort(t, t + 3);
+            c.ms = t[1];
+        }
+        std::sort(cands.begin(), cands.end(), [](const Cand &x, const Cand &y) { return x.ms < y.ms; });
+        const int bb = cands[0].bl, bbp = cands[0].bp, bc = cands[0].ch;
         int rc = kkw_batch_set_shape(b, bb, bbp, bc);
         if (rc) return rc;
         if (cache_file) {

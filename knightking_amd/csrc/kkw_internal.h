@@ -76,6 +76,8 @@ void launch_walk_persistent_mixed(const DevGraph &g, const DevJob *jobs, int n_j
                                   int grid, int block, hipStream_t stream);
 void launch_fill_u64(unsigned long long *p, unsigned long long v, int n, hipStream_t stream);
 void launch_stream_copy(const void *src, void *dst, uint64_t bytes, int cu_count, hipStream_t stream);
+void launch_random_gather(const void *buf, uint64_t bytes, uint32_t threads, uint32_t reads, uint64_t seed,
+                          uint32_t *sink, hipStream_t stream);
 void launch_pack(const PackJob *jobs, int n_jobs, const uint32_t *lens, uint64_t *cnt, uint64_t *offs, uint64_t n,
                  uint32_t *out, void *scan_tmp, size_t scan_bytes, hipStream_t s);
 size_t pack_scan_bytes(uint64_t n);

@@ -474,18 +474,51 @@ __global__ void __launch_bounds__(256) k_walk_persistent(DevGraph g, DevJob J0, 
 
 // Plain streaming copy (16 B per lane per iteration): the known-traffic kernel
 // that HBM counters and the achievable bandwidth are calibrated against.
+// four 16-byte loads in flight per lane, then their stores
 __global__ void __launch_bounds__(256) k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
                                                      uint64_t n)
 {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
     for (; i < n; i += stride) dst[i] = src[i];
 }
 
 void launch_stream_copy(const void *src, void *dst, uint64_t bytes, int cu_count, hipStream_t stream)
 {
     uint64_t n = bytes / 16;
-    hipLaunchKernelGGL(k_stream_copy, dim3(cu_count * 16), dim3(256), 0, stream, (const uint4 *)src, (uint4 *)dst, n);
+    hipLaunchKernelGGL(k_stream_copy, dim3(cu_count * 64), dim3(256), 0, stream, (const uint4 *)src, (uint4 *)dst, n);
+}
+
+// Independent random 16-byte reads, each from its own 128-byte line of a
+// buffer of `lines` lines: the rate the memory system serves such reads at is
+// the ceiling of a kernel whose accesses are all data-dependent 16-byte reads.
+__global__ void __launch_bounds__(256) k_random_gather(const uint4 *__restrict__ buf, uint64_t lines,
+                                                       uint32_t reads, uint64_t seed, uint32_t *__restrict__ sink)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t acc = 0;
+    for (uint32_t i = 0; i < reads; i++) {
+        const u32x4 r = philox(t, i, 0u, 0x200u, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const uint64_t h = (uint64_t)r.x | ((uint64_t)r.y << 32);
+        const uint64_t line = __umul64hi(h, lines);   // < lines
+        const uint4 v = buf[line * 8 + (r.z & 7u)];
+        acc += v.x ^ v.w;
+    }
+    if (acc == 0x9E3779B9u) sink[t & 63] = acc;   // keeps the reads; practically never taken
+}
+
+void launch_random_gather(const void *buf, uint64_t bytes, uint32_t threads, uint32_t reads, uint64_t seed,
+                          uint32_t *sink, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_random_gather, dim3(threads / 256), dim3(256), 0, stream, (const uint4 *)buf, bytes / 128,
+                       reads, seed, sink);
 }
 
 // ----------------------------------------------------- packed output ----
