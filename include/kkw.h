@@ -216,6 +216,44 @@ int kkw_batch_destroy(kkw_batch *b);
 int kkw_autotune(kkw_batch *b, const char *cache_file, int *block, int *blocks_per_cu,
                  int *chunk);
 
+/* Walk programs: a first-order TransitionConfig + WalkerConfig of arbitrary
+ * user logic (include/walk.hpp:147-271, the generic random_walk of
+ * walk.hpp:423-436 with internal_walk_epoch 891-1080), written as device
+ * functions of fixed names (include/kkw_program.h) and compiled at run time
+ * for the engine's GPU.  The static component is the graph's edge weights
+ * (alias tables) or uniform; everything dynamic is the program's. */
+typedef struct kkw_program kkw_program;
+
+typedef struct kkw_prog_job {
+    uint32_t walker_begin;   /* walker ids [walker_begin, +walker_num)                      */
+    uint32_t walker_num;
+    uint32_t max_length;     /* path capacity in steps; a walk still going there is cut     */
+    int32_t use_weights;     /* static component: 1 edge weights (alias), 0 uniform         */
+    uint64_t seed;
+    int32_t start_mode;      /* enum kkw_start (get_equal_dist_func / list / uniform)      */
+    uint32_t n_starts;
+    const uint32_t *starts;  /* host array, KKW_START_LIST                                  */
+    int32_t second_order;    /* 1: extension re-tested on every retry (walk.hpp:1161-1173)  */
+    uint32_t reserved;
+    float param[8];          /* kkw_ctx.param                                               */
+} kkw_prog_job;
+
+/* Compile `source` (C++ device code defining the functions kkw_program.h
+ * lists) for the engine's GPU; `options` adds compiler options (e.g.
+ * "-DTAGS=4"), may be NULL.  A compile error returns KKW_E_INVALID with the
+ * compiler's log in kkw_last_error(). */
+int kkw_program_create(kkw_engine *eng, const char *source, const char *options, kkw_program **out);
+/* Destroy every program (and batch) of an engine before the engine. */
+int kkw_program_destroy(kkw_program *prog);
+/* Compiler log of the last kkw_program_create of this program. */
+const char *kkw_program_log(const kkw_program *prog);
+/* Copy `bytes` of caller data to the device as kkw_ctx.array[slot] (0-7),
+ * replacing what the slot held; bytes = 0 clears it.  The vertex_tag-like
+ * arrays user closures capture by reference. */
+int kkw_program_set_array(kkw_program *prog, int slot, const void *data, uint64_t bytes);
+/* Run one job to completion; host paths as kkw_run gives them. */
+int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out);
+
 /* Measurement aid (no counterpart in the reference): a plain streaming copy of
  * `bytes` (read once, written once) repeated `iters` times on the engine stream;
  * gives the achievable HBM bandwidth and the known-traffic kernel that hardware

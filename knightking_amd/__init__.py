@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
@@ -60,6 +61,15 @@ class _Paths(C.Structure):
     ]
 
 
+class _ProgJob(C.Structure):
+    _fields_ = [
+        ("walker_begin", C.c_uint32), ("walker_num", C.c_uint32), ("max_length", C.c_uint32),
+        ("use_weights", C.c_int32), ("seed", C.c_uint64), ("start_mode", C.c_int32), ("n_starts", C.c_uint32),
+        ("starts", C.POINTER(C.c_uint32)), ("second_order", C.c_int32), ("reserved", C.c_uint32),
+        ("param", C.c_float * 8),
+    ]
+
+
 class _Timing(C.Structure):
     _fields_ = [
         ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("alias_ms", C.c_double),
@@ -83,6 +93,7 @@ EXPORTED = [
     "kkw_last_timing", "kkw_batch_create", "kkw_batch_set_shape", "kkw_batch_launch",
     "kkw_batch_sync", "kkw_batch_paths", "kkw_batch_destroy", "kkw_autotune", "kkw_diag_stream_copy",
     "kkw_diag_random_gather",
+    "kkw_program_create", "kkw_program_destroy", "kkw_program_log", "kkw_program_set_array", "kkw_program_run",
     "kkw_serve_start", "kkw_submit", "kkw_poll", "kkw_wait", "kkw_cancel", "kkw_serve_stop",
 ]
 
@@ -130,6 +141,11 @@ def load_library(path: str = LIB_PATH):
         "kkw_autotune": (I32, [P, C.c_char_p, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
         "kkw_diag_stream_copy": (I32, [P, U64, I32, C.POINTER(D)]),
         "kkw_diag_random_gather": (I32, [P, U64, I32, I32, C.POINTER(D)]),
+        "kkw_program_create": (I32, [P, C.c_char_p, C.c_char_p, C.POINTER(P)]),
+        "kkw_program_destroy": (I32, [P]),
+        "kkw_program_log": (C.c_char_p, [P]),
+        "kkw_program_set_array": (I32, [P, I32, P, U64]),
+        "kkw_program_run": (I32, [P, C.POINTER(_ProgJob), C.POINTER(_Paths)]),
         "kkw_serve_start": (I32, [P, C.POINTER(_ServeOpts)]),
         "kkw_submit": (I32, [P, C.POINTER(_Job), C.POINTER(U64)]),
         "kkw_poll": (I32, [P, U64, C.POINTER(I32)]),
@@ -253,6 +269,67 @@ def _take_paths(p: _Paths) -> Paths:
     return out
 
 
+class Program:
+    """A walk program (kkw_program): a first-order transition written as device
+    functions (include/kkw_program.h), compiled at run time for the engine's GPU."""
+
+    def __init__(self, eng: "Engine", source: str, options: Optional[str] = None):
+        self.eng = eng
+        h = C.c_void_p()
+        _check(_lib.kkw_program_create(eng.h, source.encode(), None if options is None else options.encode(),
+                                       C.byref(h)))
+        self.h = h
+        self._arrays = {}
+        eng._children.add(self)
+
+    @property
+    def log(self) -> str:
+        return _lib.kkw_program_log(self.h).decode(errors="replace")
+
+    def set_array(self, slot: int, data: Optional[np.ndarray]):
+        """kkw_ctx.array[slot] = a device copy of `data` (None clears it)."""
+        if data is None:
+            _check(_lib.kkw_program_set_array(self.h, slot, None, 0))
+            self._arrays.pop(slot, None)
+            return
+        data = np.ascontiguousarray(data)
+        _check(_lib.kkw_program_set_array(self.h, slot, _ptr(data), data.nbytes))
+        self._arrays[slot] = data
+
+    def run(self, walker_num: int, max_length: int, use_weights: bool = False, seed: int = 1,
+            walker_begin: int = 0, start_vertices: Optional[np.ndarray] = None, uniform_start: bool = False,
+            second_order: bool = False, params: Sequence[float] = ()) -> Paths:
+        j = _ProgJob()
+        j.walker_begin, j.walker_num, j.max_length = walker_begin, walker_num, max_length
+        j.use_weights, j.seed = int(use_weights), seed
+        starts = None
+        if start_vertices is not None:
+            starts = np.ascontiguousarray(start_vertices, np.uint32)
+            j.start_mode, j.n_starts = START_LIST, len(starts)
+            j.starts = starts.ctypes.data_as(C.POINTER(C.c_uint32))
+        else:
+            j.start_mode = START_UNIFORM if uniform_start else START_EQUAL
+        j.second_order = int(second_order)
+        if len(params) > 8:
+            raise ValueError("at most 8 parameters")
+        for k, v in enumerate(params):
+            j.param[k] = float(np.float32(v))
+        out = _Paths()
+        _check(_lib.kkw_program_run(self.h, C.byref(j), C.byref(out)))
+        return _take_paths(out)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.kkw_program_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Batch:
     """Resident batch (kkw_batch): job table, path buffers, optional hipGraph."""
 
@@ -263,6 +340,7 @@ class Batch:
         h = C.c_void_p()
         _check(_lib.kkw_batch_create(eng.h, self._cj, len(jobs), mode, flags, C.byref(h)))
         self.h = h
+        eng._children.add(self)
 
     def set_shape(self, block=256, blocks_per_cu=8, chunk=64):
         _check(_lib.kkw_batch_set_shape(self.h, block, blocks_per_cu, chunk))
@@ -292,7 +370,7 @@ class Batch:
         return b.value, bp.value, ch.value
 
     def close(self):
-        if self.h:
+        if getattr(self, "h", None):
             _lib.kkw_batch_destroy(self.h)
             self.h = None
 
@@ -312,9 +390,13 @@ class Engine:
         h = C.c_void_p()
         _check(_lib.kkw_engine_create(C.byref(o), C.byref(h)))
         self.h = h
+        # batches and programs hold the engine's handle: closed before it is
+        self._children = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "h", None):
+            for child in list(getattr(self, "_children", ())):
+                child.close()
             _lib.kkw_engine_destroy(self.h)
             self.h = None
 
@@ -413,6 +495,10 @@ class Engine:
 
     def batch(self, jobs: Sequence[Job], mode: int = MODE_PERSISTENT, flags: int = 0) -> Batch:
         return Batch(self, jobs, mode, flags)
+
+    def program(self, source: str, options: Optional[str] = None) -> Program:
+        """Compile a walk program (source text) for this engine's GPU."""
+        return Program(self, source, options)
 
     def timing(self) -> dict:
         t = _Timing()

@@ -7,8 +7,15 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["graph.hip", "walk.hip", "engine.hip"]
+SOURCES = ["graph.hip", "walk.hip", "engine.hip", "program.hip"]
 OUT = os.path.join(HERE, "libkkw.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+# headers a walk program is compiled with at run time, embedded as text
+PROGRAM_HEADERS = [("kkw_rng.h", os.path.join(CSRC, "kkw_rng.h")),
+                   ("kkw_program.h", os.path.join(INCLUDE, "kkw_program.h")),
+                   ("program_args.h", os.path.join(CSRC, "program_args.h")),
+                   ("program_walk.h", os.path.join(CSRC, "program_walk.h"))]
+PROGRAM_INC = os.path.join(CSRC, "program_sources.inc")
 
 # -ffp-contract=off: every float operation rounds on its own, as in the CPU
 # replay, so the accept/reject comparisons agree bit for bit.
@@ -28,8 +35,30 @@ def hipcc() -> str:
     return "hipcc"
 
 
+def write_program_sources() -> None:
+    """program_sources.inc: the run-time compiled headers as string literals
+    (name, text) for hiprtcCreateProgram."""
+    parts = ["// generated by knightking_amd/build.py from the headers it names; do not edit",
+             "static const char *const kProgramHeaderNames[] = {" +
+             ", ".join(f'"{n}"' for n, _ in PROGRAM_HEADERS) + "};",
+             "static const char *const kProgramHeaderTexts[] = {"]
+    for _, path in PROGRAM_HEADERS:
+        with open(path) as f:
+            text = f.read()
+        if ")KKWSRC\"" in text:
+            raise RuntimeError(f"{path} holds the raw-string delimiter")
+        parts.append('R"KKWSRC(' + text + ')KKWSRC",')
+    parts.append("};")
+    parts.append(f"static const int kProgramHeaderCount = {len(PROGRAM_HEADERS)};")
+    with open(PROGRAM_INC + ".tmp", "w") as f:
+        f.write("\n".join(parts) + "\n")
+    os.replace(PROGRAM_INC + ".tmp", PROGRAM_INC)
+
+
 def build_lib(verbose: bool = False) -> str:
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", OUT + ".tmp", *[os.path.join(CSRC, s) for s in SOURCES]]
+    write_program_sources()
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-o", OUT + ".tmp", *[os.path.join(CSRC, s) for s in SOURCES],
+           "-lhiprtc"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

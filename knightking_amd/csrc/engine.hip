@@ -1,8 +1,7 @@
 // engine.hip -- the C ABI of libkkw (include/kkw.h): engine state, graph
 // loading, job validation, resident batches, kkw_run, path output, the
 // autotuner and the serving loop.
-#include "kkw_internal.h"
-#include "../../include/kkw.h"
+#include "engine_state.h"
 
 #include <algorithm>
 #include <atomic>
@@ -24,55 +23,6 @@ using namespace kkw;
 
 namespace {
 
-thread_local std::string g_last_error;
-
-int fail(int code, const std::string &msg)
-{
-    g_last_error = msg;
-    return code;
-}
-
-template <typename F>
-int guarded(F &&f)
-{
-    try {
-        return f();
-    } catch (const Error &e) {
-        return fail(e.code, e.what());
-    } catch (const std::bad_alloc &) {
-        return fail(KKW_E_NOMEM, "host allocation failed");
-    } catch (const std::exception &e) {
-        return fail(KKW_E_INVALID, e.what());
-    }
-}
-
-double now_ms()
-{
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
-        .count();
-}
-
-// Device allocation owned by a batch / request.
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    explicit DevBuf(size_t n)
-    {
-        bytes = n;
-        hipError_t e = hipMalloc(&p, n ? n : 1);
-        if (e != hipSuccess)
-            throw Error(KKW_E_NOMEM, "hipMalloc of " + std::to_string(n) + " bytes: " + hipGetErrorString(e));
-    }
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; }
-    ~DevBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-};
-
 uint32_t ppr_default_cap(float t)
 {
     // P(walk longer than cap) = (1-t)^cap <= 1e-12
@@ -86,21 +36,6 @@ uint32_t ppr_default_cap(float t)
 // --------------------------------------------------------------------------
 struct ServeState;
 
-struct kkw_engine {
-    int device = 0;
-    int verbose = 0;
-    uint64_t hbm_budget = 0;
-    int cu_count = 256;
-    hipStream_t stream = nullptr;
-    std::recursive_mutex mu;
-    GraphStore gs;
-    kkw_timing timing{};
-    bool adj_index_auto = true;
-    // bumped whenever device graph storage is reallocated; batches compare it
-    // before replaying a captured launch
-    uint64_t graph_version = 1;
-    std::unique_ptr<ServeState> serve;
-};
 
 // Host copy of a validated job plus its device-side arrays.
 struct PreparedJob {

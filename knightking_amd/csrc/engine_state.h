@@ -1,0 +1,84 @@
+// engine_state.h -- engine state and the helpers every C-ABI translation
+// unit of libkkw shares: thread-local last error, exception-to-status
+// translation, owned device buffers.
+#pragma once
+
+#include "kkw_internal.h"
+#include "../../include/kkw.h"
+
+#include <chrono>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+
+namespace kkw {
+
+inline thread_local std::string g_last_error;
+
+inline int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+template <typename F>
+inline int guarded(F &&f)
+{
+    try {
+        return f();
+    } catch (const Error &e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(KKW_E_NOMEM, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(KKW_E_INVALID, e.what());
+    }
+}
+
+inline double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+// Device allocation owned by a batch / request.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n)
+    {
+        bytes = n;
+        hipError_t e = hipMalloc(&p, n ? n : 1);
+        if (e != hipSuccess)
+            throw Error(KKW_E_NOMEM, "hipMalloc of " + std::to_string(n) + " bytes: " + hipGetErrorString(e));
+    }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace kkw
+
+struct ServeState;
+
+struct kkw_engine {
+    int device = 0;
+    int verbose = 0;
+    uint64_t hbm_budget = 0;
+    int cu_count = 256;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    kkw::GraphStore gs;
+    kkw_timing timing{};
+    bool adj_index_auto = true;
+    // bumped whenever device graph storage is reallocated; batches compare it
+    // before replaying a captured launch
+    uint64_t graph_version = 1;
+    std::unique_ptr<ServeState> serve;
+};

@@ -345,7 +345,8 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
                                               const float *__restrict__ w,
                                               const uint8_t *__restrict__ etype, uint32_t V,
                                               Bucket *__restrict__ out, float *__restrict__ vsum,
-                                              uint2 *__restrict__ qs, uint2 *__restrict__ ql, uint32_t big_threshold)
+                                              uint2 *__restrict__ qs, uint2 *__restrict__ ql, uint32_t big_threshold,
+                                              bool to_ids)
 {
     uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
@@ -384,8 +385,8 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
             uint2 le = L[lh++];
             float pl = __uint_as_float(le.y);
             bk.p = ps / ave;
-            bk.primary = cv[se.x];
-            bk.alias = cv[le.x];
+            bk.primary = to_ids ? cv[se.x] : se.x;
+            bk.alias = to_ids ? cv[le.x] : le.x;
             bk.meta = tv ? ((uint32_t)tv[se.x] | ((uint32_t)tv[le.x] << 8)) : 0u;
             float diff = ave - ps;
             pl -= diff;
@@ -394,7 +395,7 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
             else L[lt++] = ne;
         } else {
             bk.p = 1.0f;
-            bk.primary = cv[se.x];
+            bk.primary = to_ids ? cv[se.x] : se.x;
             bk.alias = bk.primary;
             bk.meta = tv ? (uint32_t)tv[se.x] * 0x101u : 0u;
         }
@@ -406,7 +407,7 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
         bk.prow = 0;
         bk.arow = 0;
         bk.p = 1.0f;
-        bk.primary = cv[le.x];
+        bk.primary = to_ids ? cv[le.x] : le.x;
         bk.alias = bk.primary;
         bk.meta = tv ? (uint32_t)tv[le.x] * 0x101u : 0u;
         *o++ = bk;
@@ -445,7 +446,8 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
                                                             const float *__restrict__ w,
                                                             const uint8_t *__restrict__ etype,
                                                             Bucket *__restrict__ out, float *__restrict__ vsum,
-                                                            uint2 *__restrict__ qs, uint2 *__restrict__ ql)
+                                                            uint2 *__restrict__ qs, uint2 *__restrict__ ql,
+                                                            bool to_ids)
 {
     __shared__ uint2 win_s[AW_WAVES][AW_WIN], win_l[AW_WAVES][AW_WIN];
     __shared__ float stage[AW_WAVES][AW_SUM_CHUNK];
@@ -587,6 +589,7 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
         o[ob + i] = bk;
     }
     wave_sync();
+    if (!to_ids) return;   // tables of local edge indices (walk programs)
     // 5. local edge indices -> neighbour ids and edge types
     for (uint32_t i = lane; i < d; i += 64) {
         const uint32_t pe = o[i].primary, ae = o[i].alias;
@@ -705,6 +708,7 @@ void GraphStore::release()
     dfree(vmask);
     dfree(bucket);
     dfree(ubucket);
+    dfree(lbucket);
     dfree(vsum);
     dfree(adjset);
     V = 0;
@@ -903,6 +907,7 @@ void set_edge_types(GraphStore &gs, const uint8_t *h_etype, hipStream_t s)
     rebuild_vmask(gs, s);
     dfree(gs.bucket);  // bucket meta carries types: rebuild on next prepare
     dfree(gs.ubucket);
+    dfree(gs.lbucket);
     dfree(gs.vsum);
 }
 
@@ -923,6 +928,7 @@ void type_by_dst(GraphStore &gs, int ntypes, uint64_t seed, hipStream_t s)
     rebuild_vmask(gs, s);
     dfree(gs.bucket);
     dfree(gs.ubucket);
+    dfree(gs.lbucket);
     dfree(gs.vsum);
 }
 
@@ -963,6 +969,7 @@ void sort_adjacency(GraphStore &gs, hipStream_t s)
     if (et2) { dfree(gs.etype); gs.etype = et2; }
     dfree(gs.bucket);
     dfree(gs.ubucket);
+    dfree(gs.lbucket);
     dfree(gs.vsum);
     gs.sorted = true;
     detect_symmetric(gs, s);
@@ -994,14 +1001,15 @@ void build_uniform_buckets(GraphStore &gs, hipStream_t s)
     KKW_HIP(hipStreamSynchronize(s));
 }
 
-void build_alias(GraphStore &gs, hipStream_t s)
+void build_alias(GraphStore &gs, hipStream_t s, bool local)
 {
     if (!gs.row_ptr) throw Error(-2, "no graph loaded");
     if (!gs.w) throw Error(-2, "graph has no weights: alias tables need a weighted graph");
-    dfree(gs.bucket);
-    dfree(gs.vsum);
-    gs.bucket = dalloc<Bucket>(gs.E);
-    gs.vsum = dalloc<float>(gs.V);
+    Bucket *&target = local ? gs.lbucket : gs.bucket;
+    dfree(target);
+    target = dalloc<Bucket>(gs.E);
+    // both tables give the same sums; keep the array (and its address) if present
+    if (!gs.vsum) gs.vsum = dalloc<float>(gs.V);
     uint2 *qs = dalloc<uint2>(gs.E), *ql = dalloc<uint2>(gs.E);
     // lists of AW_MIN or more entries: one wave each; the rest: one lane each
     uint32_t *big = dalloc<uint32_t>(gs.V);
@@ -1013,10 +1021,10 @@ void build_alias(GraphStore &gs, hipStream_t s)
     KKW_HIP(hipStreamSynchronize(s));
     if (n_big)
         hipLaunchKernelGGL(k_alias_wave, dim3(grid_for(n_big, AW_WAVES)), dim3(64 * AW_WAVES), 0, s, big, n_big,
-                           gs.row_ptr, gs.col, gs.w, gs.etype, gs.bucket, gs.vsum, qs, ql);
+                           gs.row_ptr, gs.col, gs.w, gs.etype, target, gs.vsum, qs, ql, !local);
     hipLaunchKernelGGL(k_alias, dim3(grid_for(gs.V, 64)), dim3(64), 0, s, gs.row_ptr, gs.col, gs.w,
-                       gs.etype, gs.V, gs.bucket, gs.vsum, qs, ql, AW_MIN);
-    if (gs.packable && gs.E)
+                       gs.etype, gs.V, target, gs.vsum, qs, ql, AW_MIN, !local);
+    if (!local && gs.packable && gs.E)
         hipLaunchKernelGGL(k_bucket_rows, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.bucket, gs.E, gs.row_ptr);
     KKW_HIP(hipGetLastError());
     KKW_HIP(hipStreamSynchronize(s));

@@ -33,6 +33,7 @@ struct GraphStore {
     uint8_t *vmask = nullptr;
     Bucket *bucket = nullptr;
     Bucket *ubucket = nullptr;
+    Bucket *lbucket = nullptr;   // alias tables holding local edge indices (walk programs)
     float *vsum = nullptr;
     uint32_t *adjset = nullptr;
     bool sorted = false;
@@ -53,7 +54,9 @@ void build_from_csr(GraphStore &gs, uint32_t V, const uint64_t *h_row_ptr, const
                     const float *h_w, hipStream_t s);
 void set_edge_types(GraphStore &gs, const uint8_t *h_etype, hipStream_t s);
 void type_by_dst(GraphStore &gs, int ntypes, uint64_t seed, hipStream_t s);
-void build_alias(GraphStore &gs, hipStream_t s);
+// local: buckets hold local edge indices (walk programs) instead of neighbour
+// ids and rows; either way the per-vertex weight sums go to vsum
+void build_alias(GraphStore &gs, hipStream_t s, bool local = false);
 void build_adj_index(GraphStore &gs, hipStream_t s);
 void build_uniform_buckets(GraphStore &gs, hipStream_t s);
 

@@ -1,0 +1,34 @@
+// program_args.h -- kernel arguments of a walk program's walk loop; shared by
+// the host code of libkkw (hipcc) and the run-time compiled kernel (hipRTC).
+#pragma once
+
+#include "kkw_program.h"
+
+// Layout of the library's 32-byte alias bucket; walk programs use tables whose
+// primary/alias fields hold local edge indices.
+struct kkw_prog_bucket {
+    float p;
+    uint32_t primary;
+    uint64_t prow;
+    uint32_t alias;
+    uint32_t meta;
+    uint64_t arow;
+};
+
+struct kkw_prog_args {
+    kkw_ctx c;
+    const kkw_prog_bucket *lb;   // alias tables of the weights (local indices), or null: uniform
+    const float *vsum;           // per-vertex sequential weight sums, null when lb is
+    uint64_t seed;
+    uint32_t walker_begin;
+    uint32_t walker_num;
+    uint32_t max_len;            // path capacity in steps
+    uint32_t stride;             // path row length (entries)
+    int32_t start_mode;          // 0: id % V, 1: starts[id % n_starts], 2: uniform draw
+    uint32_t n_starts;
+    const uint32_t *starts;
+    int32_t retest_ext;          // extension re-tested on every retry (second order)
+    uint32_t reserved;
+    uint32_t *paths;             // [walker_num * stride]
+    uint32_t *lens;              // [walker_num]: vertices in the path | 1 << 31 if cut
+};

@@ -1,0 +1,153 @@
+// program_walk.h -- the walk loop of a walk program, compiled at run time
+// (hipRTC) after the program's own functions (include/kkw_program.h).
+//
+// One walker per lane, the first-order epoch of include/walk.hpp:954-1064:
+// per step the extension test (once, or on every retry for a second-order
+// program, walk.hpp:1161-1173), then trials until one is accepted -- the
+// outlier appendix first when the program has one (981-1014), else a static
+// draw (uniform, or the alias bucket of the edge weights, 1016-1029) and,
+// with a dynamic component, the dart against lower bound and dynamic_comp
+// (1030-1036); then step++, update_state and the new vertex (1039-1056).
+// Draws use the same counters as the library's built-in walks: trial words
+// (index, coin, dart) at (walker, step, trial, TRIAL), the outlier word at
+// (walker, step, trial, OUTLIER), the extension coin at (walker, step, 0 or
+// trial, EXT), a uniform start at (walker, 0, 0, INIT).
+#pragma once
+
+#include "kkw_rng.h"
+#include "program_args.h"
+
+#ifndef KKW_DYNAMIC
+#define KKW_DYNAMIC 0
+#endif
+#ifndef KKW_LOWER
+#define KKW_LOWER 0
+#endif
+#ifndef KKW_OUTLIER
+#define KKW_OUTLIER 0
+#endif
+#if KKW_OUTLIER && !KKW_DYNAMIC
+#error "an outlier appendix needs a dynamic component (walk.hpp:251-261)"
+#endif
+#if KKW_LOWER && !KKW_DYNAMIC
+#error "a lower bound needs a dynamic component (walk.hpp:251-261)"
+#endif
+
+#define KKW_PROG_MAX_TRIALS (1u << 20)   // per step; guards degenerate programs
+
+__device__ __forceinline__ bool kkw_prog_terminates(const WalkerState &s, const kkw_walker &w, uint32_t cur,
+                                                    uint32_t deg, uint32_t trial, const kkw_prog_args &A)
+{
+    if (deg == 0) return true;
+    const float ext = extension(s, w, cur, A.c);
+    if (ext == 0.0f) return true;
+    if (ext < 1.0f) return kkw::u01(kkw::draw(A.seed, w.id, w.step, trial, kkw::TAG_EXT).x) > ext;
+    return false;
+}
+
+// One trial at cur; returns the accepted edge or -1.
+__device__ __forceinline__ int64_t kkw_prog_trial(const WalkerState &s, const kkw_walker &w, uint32_t cur,
+                                                  uint64_t beg, uint32_t deg, uint32_t trial,
+                                                  const kkw_prog_args &A)
+{
+    const kkw_ctx &c = A.c;
+#if KKW_DYNAMIC
+    const float up = upper_bound(cur, c);
+#endif
+#if KKW_OUTLIER
+    {
+        float ov = 0.0f;
+        uint32_t num = 0;
+        outlier_bound(s, w, cur, ov, num, c);
+        if (ov > 0.0f && num > 0) {
+            const float regular = (A.vsum ? A.vsum[cur] : (float)deg) * up;
+            const float rv = kkw::u01(kkw::draw(A.seed, w.id, w.step, trial, kkw::TAG_OUTLIER).x) *
+                                 (ov * (float)num + regular) -
+                             regular;
+            if (rv > 0.0f) {
+                uint32_t idx = (uint32_t)(rv / ov);
+                if (idx + 1 > num) idx = num - 1;
+                const int64_t e = outlier_search(s, w, cur, idx, c);
+                if (e >= 0) {
+                    const float sc = A.lb ? c.weight[e] : 1.0f;
+                    const float h = (rv - (float)idx * ov) / sc + up;
+                    if (h <= dynamic_comp(s, w, cur, (uint64_t)e, c)) return e;
+                }
+                return -1;
+            }
+        }
+    }
+#endif
+    const kkw::u32x4 r = kkw::draw(A.seed, w.id, w.step, trial, kkw::TAG_TRIAL);
+    const uint32_t k = kkw::bounded(r.x, r.y, deg);
+    uint32_t el = k;
+    if (A.lb) {
+        const kkw_prog_bucket &b = A.lb[beg + k];
+        el = kkw::u01(r.z) < b.p ? b.primary : b.alias;
+    }
+    const uint64_t e = beg + el;
+#if KKW_DYNAMIC
+    const float dart = kkw::u01(r.w) * up;
+    bool ok = false;
+#if KKW_LOWER
+    ok = dart <= lower_bound(cur, c);
+#endif
+    if (!ok) ok = dart <= dynamic_comp(s, w, cur, e, c);
+    return ok ? (int64_t)e : -1;
+#else
+    return (int64_t)e;
+#endif
+}
+
+extern "C" __global__ void __launch_bounds__(256) kkw_program_walk(kkw_prog_args A)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.walker_num) return;
+    const kkw_ctx &c = A.c;
+    kkw_walker w;
+    w.id = A.walker_begin + i;
+    w.step = 0;
+    uint32_t cur;
+    if (A.start_mode == 1) {
+        cur = A.starts[w.id % A.n_starts];
+    } else if (A.start_mode == 2) {
+        const kkw::u32x4 r = kkw::draw(A.seed, w.id, 0, 0, kkw::TAG_INIT);
+        cur = kkw::bounded(r.z, r.w, c.v_num);
+    } else {
+        cur = w.id % c.v_num;
+    }
+    WalkerState s;
+    init_state(s, w, cur, c);
+    uint32_t *path = A.paths + (uint64_t)i * A.stride;
+    path[0] = cur;
+    uint64_t beg = c.row_ptr[cur];
+    uint32_t deg = (uint32_t)(c.row_ptr[cur + 1] - beg);
+    bool capped = false;
+    uint32_t trial = 0;
+    for (;;) {
+        if (trial == 0 || A.retest_ext) {
+            if (kkw_prog_terminates(s, w, cur, deg, A.retest_ext ? trial : 0u, A)) break;
+            if (w.step >= A.max_len) {
+                capped = true;
+                break;
+            }
+        }
+        if (trial >= KKW_PROG_MAX_TRIALS) {
+            capped = true;
+            break;
+        }
+        const int64_t e = kkw_prog_trial(s, w, cur, beg, deg, trial, A);
+        if (e < 0) {
+            trial++;
+            continue;
+        }
+        w.step++;
+        update_state(s, w, cur, (uint64_t)e, c);
+        cur = c.col[e];
+        path[w.step] = cur;
+        beg = c.row_ptr[cur];
+        deg = (uint32_t)(c.row_ptr[cur + 1] - beg);
+        trial = 0;
+    }
+    A.lens[i] = (w.step + 1) | (capped ? 0x80000000u : 0u);
+}

@@ -243,10 +243,12 @@ void orc_csr_from_edges(uint32_t V, const uint32_t *src, const uint32_t *dst,
  * "No alias" is stored as alias = primary with p = 1, which selects the
  * primary exactly as `q_ptr == nullptr` does (p < 1 for every paired bucket,
  * and U < 1 always).  bmeta packs the edge types of primary | alias << 8. */
-void orc_alias_build(uint32_t V, const uint64_t *row_ptr, const uint32_t *col,
-                     const float *w, const uint8_t *etype,
-                     float *bp, uint32_t *bprim, uint32_t *balias, uint32_t *bmeta,
-                     float *vsum)
+/* local != 0: bprim/balias receive local edge indices (position in the list)
+ * instead of neighbour ids -- the reference's p_ptr/q_ptr point at edges. */
+static void alias_build(uint32_t V, const uint64_t *row_ptr, const uint32_t *col,
+                        const float *w, const uint8_t *etype,
+                        float *bp, uint32_t *bprim, uint32_t *balias, uint32_t *bmeta,
+                        float *vsum, int local)
 {
     uint64_t maxd = 0;
     for (uint32_t v = 0; v < V; v++) {
@@ -282,8 +284,8 @@ void orc_alias_build(uint32_t V, const uint64_t *row_ptr, const uint32_t *col,
                 if (qt != qh) {
                     uint32_t qi = qs[qh++];
                     bp[o] = prob[pi] / ave;
-                    bprim[o] = col[b + pi];
-                    balias[o] = col[b + qi];
+                    bprim[o] = local ? pi : col[b + pi];
+                    balias[o] = local ? qi : col[b + qi];
                     bmeta[o] = etype ? ((uint32_t)etype[b + pi] | ((uint32_t)etype[b + qi] << 8)) : 0u;
                     float diff = ave - prob[pi];
                     prob[qi] -= diff;
@@ -291,8 +293,8 @@ void orc_alias_build(uint32_t V, const uint64_t *row_ptr, const uint32_t *col,
                     else qs[qt++] = qi;
                 } else {
                     bp[o] = 1.0f;
-                    bprim[o] = col[b + pi];
-                    balias[o] = col[b + pi];
+                    bprim[o] = local ? pi : col[b + pi];
+                    balias[o] = bprim[o];
                     bmeta[o] = etype ? ((uint32_t)etype[b + pi] * 0x101u) : 0u;
                 }
                 o++;
@@ -300,8 +302,8 @@ void orc_alias_build(uint32_t V, const uint64_t *row_ptr, const uint32_t *col,
             while (qt != qh) {
                 uint32_t qi = qs[qh++];
                 bp[o] = 1.0f;
-                bprim[o] = col[b + qi];
-                balias[o] = col[b + qi];
+                bprim[o] = local ? qi : col[b + qi];
+                balias[o] = bprim[o];
                 bmeta[o] = etype ? ((uint32_t)etype[b + qi] * 0x101u) : 0u;
                 o++;
             }
@@ -310,6 +312,23 @@ void orc_alias_build(uint32_t V, const uint64_t *row_ptr, const uint32_t *col,
         free(ps);
         free(qs);
     }
+}
+
+void orc_alias_build(uint32_t V, const uint64_t *row_ptr, const uint32_t *col,
+                     const float *w, const uint8_t *etype,
+                     float *bp, uint32_t *bprim, uint32_t *balias, uint32_t *bmeta,
+                     float *vsum)
+{
+    alias_build(V, row_ptr, col, w, etype, bp, bprim, balias, bmeta, vsum, 0);
+}
+
+void orc_alias_build_local(uint32_t V, const uint64_t *row_ptr, const float *w,
+                           float *bp, uint32_t *bprim, uint32_t *balias, float *vsum)
+{
+    uint64_t E = row_ptr[V];
+    uint32_t *meta = (uint32_t *)malloc(sizeof(uint32_t) * (E ? E : 1));
+    alias_build(V, row_ptr, NULL, w, NULL, bp, bprim, balias, meta, vsum, 1);
+    free(meta);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -67,6 +67,9 @@ void orc_alias_build(uint32_t V, const uint64_t *row_ptr, const uint32_t *col,
                      const float *w, const uint8_t *etype,
                      float *bp, uint32_t *bprim, uint32_t *balias, uint32_t *bmeta,
                      float *vsum);
+/* The same tables with local edge indices in bprim/balias (walk programs). */
+void orc_alias_build_local(uint32_t V, const uint64_t *row_ptr, const float *w,
+                           float *bp, uint32_t *bprim, uint32_t *balias, float *vsum);
 
 /* ---- walks --------------------------------------------------------------- */
 typedef struct orc_graph {

@@ -68,6 +68,7 @@ def lib():
         L.orc_csr_from_edges.argtypes = [C.c_uint32, P, P, P, C.c_uint64, C.c_int, C.c_int, P, P, P]
         L.orc_vertex_types.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, P]
         L.orc_alias_build.argtypes = [C.c_uint32, P, P, P, P, P, P, P, P, P]
+        L.orc_alias_build_local.argtypes = [C.c_uint32, P, P, P, P, P, P]
         L.orc_node2vec_constants.argtypes = [C.c_float, C.c_float] + [C.POINTER(C.c_float)] * 5
         L.orc_walk.argtypes = [C.POINTER(OrcGraph), C.POINTER(OrcJob), P, P, C.c_int, C.POINTER(OrcStats)]
         _lib = L
@@ -260,3 +261,78 @@ def walk(g: Graph, kind, walker_num, walk_length=0, use_weights=True, walker_beg
     lib().orc_walk(C.byref(og), C.byref(j), _p(paths), _p(lens), threads, C.byref(st))
     raw = lens[:walker_num]
     return paths, raw & 0x7FFFFFFF, (raw >> 31).astype(bool), st
+
+
+# ---- walk programs -----------------------------------------------------------
+_ROOT = os.path.dirname(_HERE)
+PROGRAM_ORACLE_SRC = os.path.join(_HERE, "program_oracle.cpp")
+
+
+def alias_build_local(row_ptr, w):
+    """Alias tables whose primary/alias are local edge indices (walk programs)."""
+    V = len(row_ptr) - 1
+    E = int(row_ptr[-1])
+    bp = np.zeros(max(E, 1), np.float32)
+    prim = np.zeros(max(E, 1), np.uint32)
+    al = np.zeros(max(E, 1), np.uint32)
+    vs = np.zeros(V, np.float32)
+    lib().orc_alias_build_local(V, _p(np.ascontiguousarray(row_ptr, np.uint64)),
+                                _p(np.ascontiguousarray(w, np.float32)), _p(bp), _p(prim), _p(al), _p(vs))
+    return bp, prim, al, vs
+
+
+def program_oracle(source_path: str, build_dir: str):
+    """Compile the oracle's walk-program loop (program_oracle.cpp) with one
+    program's source and load it; returns the ctypes library."""
+    import hashlib
+    import subprocess
+    with open(source_path, "rb") as f:
+        tag = hashlib.sha1(f.read() + open(PROGRAM_ORACLE_SRC, "rb").read()).hexdigest()[:12]
+    out = os.path.join(build_dir, f"orc_program_{tag}.so")
+    if not os.path.exists(out):
+        os.makedirs(build_dir, exist_ok=True)
+        cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared",
+               "-I", os.path.join(_ROOT, "include"), "-I", _HERE,
+               f'-DKKW_PROGRAM_SOURCE="{os.path.abspath(source_path)}"',
+               PROGRAM_ORACLE_SRC, "-x", "c", os.path.join(_HERE, "kkw_oracle.c"), "-o", out + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"program oracle does not compile:\n{r.stderr}")
+        os.replace(out + ".tmp", out)
+    L = C.CDLL(out)
+    P, U32, U64, I = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    L.orc_program_walk.restype = I
+    L.orc_program_walk.argtypes = [U32, P, P, P, P, P, P, P, P, P, P, U64, U32, U32, U32, U32, I, U32, P, I, P, P]
+    return L
+
+
+def program_walk(plib, row_ptr, col, w=None, etype=None, arrays=(), params=(), use_weights=False, seed=1,
+                 walker_num=0, max_length=0, walker_begin=0, start_vertices=None, uniform_start=False,
+                 second_order=False):
+    """Run a walk program on the oracle; returns (paths [n, stride], lengths,
+    truncated) with stride = max_length + 1 rounded up to 4."""
+    V = len(row_ptr) - 1
+    row_ptr = np.ascontiguousarray(row_ptr, np.uint64)
+    col = np.ascontiguousarray(col, np.uint32)
+    w = None if w is None else np.ascontiguousarray(w, np.float32)
+    etype = None if etype is None else np.ascontiguousarray(etype, np.uint8)
+    keep = [np.ascontiguousarray(a) if a is not None else None for a in arrays]
+    arr = (C.c_void_p * 8)(*([None if a is None else a.ctypes.data for a in keep] + [None] * (8 - len(keep))))
+    par = (C.c_float * 8)(*([float(np.float32(x)) for x in params] + [0.0] * (8 - len(params))))
+    bp = prim = al = vs = None
+    if use_weights:
+        bp, prim, al, vs = alias_build_local(row_ptr, w)
+    stride = (max_length + 1 + 3) & ~3
+    paths = np.zeros((max(walker_num, 1), stride), np.uint32)
+    lens = np.zeros(max(walker_num, 1), np.uint32)
+    if start_vertices is not None:
+        starts = np.ascontiguousarray(start_vertices, np.uint32)
+        mode = START_LIST
+    else:
+        starts = None
+        mode = START_UNIFORM if uniform_start else START_EQUAL
+    plib.orc_program_walk(V, _p(row_ptr), _p(col), _p(w), _p(etype), arr, par, _p(bp), _p(prim), _p(al), _p(vs),
+                          seed, walker_begin, walker_num, max_length, stride, mode,
+                          0 if starts is None else len(starts), _p(starts), int(second_order), _p(paths), _p(lens))
+    raw = lens[:walker_num]
+    return paths[:walker_num], raw & 0x7FFFFFFF, (raw >> 31).astype(bool)

@@ -1,0 +1,9 @@
+// PPR as a walk program (src/apps/ppr.hpp:30-59): extension = 1 - t, one
+// coin per step.  param[0] = 1 - t (as the float the library computes).
+struct WalkerState {
+    uint32_t unused;
+};
+
+KKW_FN void init_state(WalkerState &s, const kkw_walker &, uint32_t, const kkw_ctx &) { s.unused = 0; }
+KKW_FN void update_state(WalkerState &, const kkw_walker &, uint32_t, uint64_t, const kkw_ctx &) {}
+KKW_FN float extension(const WalkerState &, const kkw_walker &, uint32_t, const kkw_ctx &c) { return c.param[0]; }

@@ -41,7 +41,7 @@ def test_struct_layouts_match_header(tmp_path):
     """ctypes mirrors of the POD structs have the sizes and offsets gcc gives the header's."""
     import subprocess
     fields = {"kkw_job": kk._Job, "kkw_paths": kk._Paths, "kkw_engine_opts": kk._Opts,
-              "kkw_timing": kk._Timing, "kkw_serve_opts": kk._ServeOpts}
+              "kkw_timing": kk._Timing, "kkw_serve_opts": kk._ServeOpts, "kkw_prog_job": kk._ProgJob}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "kkw.h"', "int main(void){"]
     for cname, py in fields.items():
         lines.append(f'printf("%zu\\n", sizeof({cname}));')

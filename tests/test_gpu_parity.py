@@ -59,6 +59,36 @@ def test_edges_csr_matches_oracle(engine, karate, undirected, sort_adj):
     assert np.array_equal(rp, rp_o) and np.array_equal(col, col_o) and np.array_equal(wd, w_o)
 
 
+@pytest.mark.parametrize("weighted,text", [(True, False), (False, False), (True, True), (False, True)])
+def test_file_formats_load_like_the_arrays(engine, karate, tmp_path, weighted, text):
+    """Binary Edge<real_t> (12-byte records) / Edge<EmptyData> (8-byte) files as
+    write_graph leaves them (include/storage.hpp:57-66), and "src dst[ w]" text
+    edge lists, give the CSR of the same edges passed as arrays."""
+    src, dst, w = karate
+    w = w if weighted else None
+    path = tmp_path / ("g.txt" if text else "g.bin")
+    if text:
+        with open(path, "w") as f:
+            for i in range(len(src)):
+                f.write(f"{src[i]} {dst[i]}" + (f" {float(w[i]):.9g}" if weighted else "") + "\n")
+    else:
+        if weighted:
+            rec = np.zeros(len(src), dtype=[("s", "<u4"), ("d", "<u4"), ("w", "<f4")])
+            rec["w"] = w
+        else:
+            rec = np.zeros(len(src), dtype=[("s", "<u4"), ("d", "<u4")])
+        rec["s"] = src
+        rec["d"] = dst
+        rec.tofile(path)
+    print(f"[gpu] karate from a {'text' if text else 'binary'} file, weighted={weighted}", flush=True)
+    engine.load_graph(34, str(path), text=text, weighted=weighted, make_undirected=False, sort_adj=True)
+    rp, col, wd, _ = engine.download(weights=weighted)
+    rp_o, col_o, w_o = O.csr_from_edges(34, src, dst, w, undirected=False, sort_adj=True)
+    assert np.array_equal(rp, rp_o) and np.array_equal(col, col_o)
+    if weighted:
+        assert np.array_equal(wd.view(np.uint32), w_o.view(np.uint32))
+
+
 def test_edges_csr_rejects_bad_vertex(engine):
     with pytest.raises(kk.KkwError):
         engine.graph_from_edges(4, np.array([0, 1], np.uint32), np.array([1, 9], np.uint32))

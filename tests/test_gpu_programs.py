@@ -1,0 +1,131 @@
+"""Walk programs on the GPU (kkw_program_*): compiled at run time with hipRTC,
+run through the C ABI, bit-exact against the oracle's program loop compiled
+with the same source; programs restating the built-in apps are also
+bit-exact against the built-in kernels."""
+import os
+
+import numpy as np
+import pytest
+
+import knightking_amd as kk
+from oracle import oracle as O
+from test_oracle_programs import PROGRAMS, TAGS, same_paths, tagged_graph
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def build_dir(tmp_path_factory):
+    return str(tmp_path_factory.mktemp("program_oracles"))
+
+
+def source(name):
+    with open(os.path.join(PROGRAMS, name + ".cpp")) as f:
+        return f.read()
+
+
+def both(engine, build_dir, name, rp, col, w=None, et=None, arrays=(), **run):
+    """Run program `name` on the GPU and on the oracle; return both results."""
+    print(f"[gpu] program {name} V={len(rp) - 1} E={len(col)} {run}", flush=True)
+    engine.graph_from_csr(rp, col, w)
+    if et is not None:
+        engine.set_edge_types(et)
+    prog = engine.program(source(name))
+    try:
+        for k, a in enumerate(arrays):
+            prog.set_array(k, a)
+        got = prog.run(**run)
+    finally:
+        prog.close()
+    want = O.program_walk(O.program_oracle(os.path.join(PROGRAMS, name + ".cpp"), build_dir), rp, col, w, et,
+                          arrays=arrays, params=run.get("params", ()), use_weights=run.get("use_weights", False),
+                          seed=run.get("seed", 1), walker_num=run["walker_num"], max_length=run["max_length"],
+                          walker_begin=run.get("walker_begin", 0), start_vertices=run.get("start_vertices"),
+                          uniform_start=run.get("uniform_start", False),
+                          second_order=run.get("second_order", False))
+    return got, want
+
+
+def check(got, want):
+    pw, lw, tw = want
+    same_paths(got.vertices, got.lengths, pw, lw)
+    assert np.array_equal(got.truncated, tw)
+    assert got.steps == int((lw.astype(np.int64) - 1).sum())
+
+
+@pytest.mark.parametrize("weighted", [True, False])
+def test_deepwalk_program(engine, build_dir, weighted):
+    rp, col, w = O.rmat_graph(12, 16, seed=3, weighted=True)
+    check(*both(engine, build_dir, "deepwalk", rp, col, w, walker_num=20000, max_length=40, params=[40],
+                use_weights=weighted, seed=9))
+
+
+def test_ppr_program_uniform_start(engine, build_dir):
+    rp, col, w = O.rmat_graph(12, 16, seed=4, weighted=True)
+    t = 0.15
+    ext = np.float32(1.0) - np.float32(t)
+    check(*both(engine, build_dir, "ppr", rp, col, w, walker_num=30000, max_length=O.ppr_default_cap(t),
+                params=[ext], use_weights=True, seed=2, uniform_start=True, walker_begin=77))
+
+
+@pytest.mark.parametrize("p,q,weighted", [(2.0, 0.5, True), (0.5, 2.0, True), (0.25, 1.0, False)])
+def test_node2vec_program_matches_oracle_and_builtin_kernel(engine, build_dir, p, q, weighted):
+    rp, col, w = O.rmat_graph(12, 16, seed=5, weighted=True)
+    L = 40
+    up, lo, ip, iq, ov = O.node2vec_constants(p, q)
+    got, want = both(engine, build_dir, "node2vec", rp, col, w, walker_num=8192, max_length=L,
+                     params=[L, up, lo, ip, iq, ov, 1.0 if weighted else 0.0], use_weights=weighted, seed=11,
+                     second_order=True)
+    check(got, want)
+    (builtin,) = engine.run([kk.node2vec_job(8192, L, p, q, weighted, seed=11)], mode=kk.MODE_PLAIN)
+    same_paths(got.vertices, got.lengths, builtin.vertices, builtin.lengths)
+
+
+@pytest.mark.parametrize("weighted", [True, False])
+def test_tag_walk_program(engine, build_dir, weighted):
+    rp, col, w, et, vt = tagged_graph(300, 2000, 21, weighted)
+    check(*both(engine, build_dir, "tag_walk", rp, col, w, et, arrays=[vt], walker_num=20000, max_length=20,
+                params=[20, 0.5], use_weights=weighted, seed=3))
+
+
+@pytest.mark.parametrize("weighted", [True, False])
+def test_tag_outlier_program(engine, build_dir, weighted):
+    V = 300
+    rp, col, w, et, vt = tagged_graph(V, 2000, 22, weighted, sort_by_tag=True)
+    begin = np.zeros(V * TAGS, np.uint64)
+    end = np.zeros(V * TAGS, np.uint64)
+    for v in range(V):
+        b, e = int(rp[v]), int(rp[v + 1])
+        for t in range(TAGS):
+            idx = np.nonzero(et[b:e] == t)[0]
+            begin[v * TAGS + t] = b + (idx[0] if len(idx) else 0)
+            end[v * TAGS + t] = b + (idx[-1] + 1 if len(idx) else 0)
+    check(*both(engine, build_dir, "tag_outlier", rp, col, w, et, arrays=[vt, begin, end], walker_num=20000,
+                max_length=20, params=[20, 3.0, 1.0 if weighted else 0.0], use_weights=weighted, seed=4))
+
+
+def test_return_bias_program_start_list_and_cap(engine, build_dir, karate):
+    src, dst, w = karate
+    rp, col, ww = O.csr_from_edges(34, src, dst, w, False, True)
+    check(*both(engine, build_dir, "return_bias", rp, col, ww, walker_num=5000, max_length=7,
+                params=[1000], use_weights=True, seed=6, start_vertices=np.array([0, 33, 5], np.uint32)))
+
+
+def test_program_errors(engine):
+    rp, col, w = O.rmat_graph(8, 8, seed=2, weighted=True)
+    engine.graph_from_csr(rp, col, w)
+    with pytest.raises(kk.KkwError) as e:
+        engine.program("struct WalkerState { int x; };\nthis does not compile\n")
+    assert e.value.code == kk.KKW_E_INVALID and "compile" in str(e.value)
+    prog = engine.program(source("deepwalk"))
+    try:
+        with pytest.raises(kk.KkwError) as e:
+            prog.set_array(8, np.zeros(4, np.int32))
+        assert e.value.code == kk.KKW_E_INVALID
+        with pytest.raises(kk.KkwError) as e:
+            prog.run(10, 5, params=[5], start_vertices=np.array([1 << 20], np.uint32))
+        assert e.value.code == kk.KKW_E_INVALID
+        got = prog.run(0, 5, params=[5])
+        assert got.lengths.shape == (0,)
+    finally:
+        prog.close()
