@@ -11,13 +11,16 @@
 // metapath() attach to the transition config.  A config built from arbitrary
 // std::function callbacks carries no descriptor and random_walk() throws
 // std::logic_error for it (closures cannot run on the device, and there is no
-// CPU fallback).  Walks are reproducible: each random_walk() call draws from
+// CPU fallback); such a transition is written instead as a WalkProgram --
+// the same functions as device code (kkw_program.h) -- and run by the
+// random_walk(WalkerConfig*, WalkProgram*, WalkConfig*) overload.  Walks are reproducible: each random_walk() call draws from
 // the Philox stream keyed by the engine seed (set_seed, default 1) plus the
 // call's index on this engine.
 #pragma once
 
 #include <assert.h>
 #include <stdio.h>
+#include <string.h>
 
 #include <functional>
 #include <stdexcept>
@@ -279,6 +282,35 @@ public:
     }
 };
 
+// A first-order transition with arbitrary logic, as device code: the
+// counterpart of TransitionConfig plus WalkerConfig's state closures
+// (include/walk.hpp:147-271).  `source` defines the functions kkw_program.h
+// lists; the data a closure would capture by reference goes in arrays
+// (kkw_ctx.array[k]) and param (kkw_ctx.param).
+struct WalkProgram {
+    std::string source;
+    std::string options;                         // extra compiler options, e.g. "-DTAGS=4"
+    std::vector<std::vector<unsigned char> > arrays;
+    float param[8];
+    bool use_weights;                            // static component: edge weights (else uniform)
+    bool second_order;                           // re-test the extension on every retry
+    step_t max_length;                           // path capacity in steps
+
+    WalkProgram() : use_weights(false), second_order(false), max_length(0)
+    {
+        for (int k = 0; k < 8; k++) param[k] = 0.0f;
+    }
+    explicit WalkProgram(const std::string &src) : WalkProgram() { source = src; }
+    template <typename T>
+    void set_array(int slot, const T *data, size_t n)
+    {
+        if (slot < 0 || slot >= 8) throw std::invalid_argument("WalkProgram: array slot must be 0-7");
+        if ((size_t)slot >= arrays.size()) arrays.resize(slot + 1);
+        const unsigned char *b = reinterpret_cast<const unsigned char *>(data);
+        arrays[slot].assign(b, b + n * sizeof(T));
+    }
+};
+
 template <typename edge_data_t, typename walker_data_t>
 class WalkEngine {
     kkw_engine *eng_ = nullptr;
@@ -388,6 +420,72 @@ public:
             job.walker_num = n;
             kkw_paths out{};
             knightking_detail::check(kkw_run(eng_, &job, 1, KKW_MODE_PERSISTENT, KKW_OUT_HOST, 0, &out));
+            if (wc->output_file_flag || wc->output_consumer_flag) {
+                PathSet *ps = to_pathset(out);
+                if (wc->output_file_flag) {
+                    std::string p = wc->output_path_prefix + ".0";
+                    ps->dump(p.c_str(), iter == 0 ? "w" : "a", wc->print_with_head_info);
+                }
+                if (wc->output_consumer_flag) wc->output_consumer_func(ps);
+                else delete ps;
+            }
+            kkw_paths_free(&out);
+            iter++;
+        }
+    }
+
+    // random_walk for a WalkProgram: compiled for the GPU on each call, run in
+    // the epochs WalkConfig's rate gives, output as for the apps.
+    void random_walk(WalkerConfig<edge_data_t, walker_data_t> *walker_config, WalkProgram *program,
+                     WalkConfig *walk_config_param = nullptr)
+    {
+        WalkConfig local;
+        WalkConfig *wc = walk_config_param ? walk_config_param : &local;
+        if (program->max_length == 0) throw std::invalid_argument("WalkProgram: max_length must be set");
+        kkw_program *prog = nullptr;
+        knightking_detail::check(kkw_program_create(eng_, program->source.c_str(),
+                                                    program->options.empty() ? nullptr : program->options.c_str(),
+                                                    &prog));
+        struct Holder {
+            kkw_program *p;
+            ~Holder() { kkw_program_destroy(p); }
+        } hold{prog};
+        for (size_t k = 0; k < program->arrays.size(); k++)
+            if (!program->arrays[k].empty())
+                knightking_detail::check(
+                    kkw_program_set_array(prog, (int)k, program->arrays[k].data(), program->arrays[k].size()));
+        kkw_prog_job job;
+        memset(&job, 0, sizeof(job));
+        job.max_length = program->max_length;
+        job.use_weights = program->use_weights ? 1 : 0;
+        job.second_order = program->second_order ? 1 : 0;
+        job.seed = seed_ + calls_++;
+        for (int k = 0; k < 8; k++) job.param[k] = program->param[k];
+        const walker_id_t walker_num = walker_config->walker_num;
+        std::vector<uint32_t> host_starts;
+        auto &dist = walker_config->walker_init_dist_func;
+        job.start_mode = KKW_START_EQUAL;
+        if (dist && dist.template target<knightking_detail::UniformDist>()) {
+            job.start_mode = KKW_START_UNIFORM;
+        } else if (dist && !dist.template target<knightking_detail::EqualDist>()) {
+            host_starts.resize(walker_num);
+            for (walker_id_t i = 0; i < walker_num; i++) host_starts[i] = dist(i);
+            job.start_mode = KKW_START_LIST;
+            job.starts = host_starts.data();
+            job.n_starts = (uint32_t)host_starts.size();
+        }
+        walker_id_t per_iter = (walker_id_t)(walker_num * wc->rate);
+        if (per_iter == 0) per_iter = 1;
+        if (per_iter > walker_num) per_iter = walker_num;
+        walker_id_t remained = walker_num;
+        int iter = 0;
+        while (remained != 0) {
+            walker_id_t n = remained < per_iter ? remained : per_iter;
+            job.walker_begin = walker_num - remained;
+            job.walker_num = n;
+            remained -= n;
+            kkw_paths out{};
+            knightking_detail::check(kkw_program_run(prog, &job, &out));
             if (wc->output_file_flag || wc->output_consumer_flag) {
                 PathSet *ps = to_pathset(out);
                 if (wc->output_file_flag) {

@@ -30,6 +30,15 @@
  *   orc_walk            closed-form transition matrices of the reference tests
  *                       (static walk, node2vec second order, metapath reachability),
  *                       geometric PPR length law, path validity
+ *   orc_alias_build_local  the same construction as orc_alias_build (one code
+ *                       path, local indices instead of ids): col[b + local]
+ *                       equals the id table entry for entry
+ *   orc_program_walk (program_oracle.cpp, compiled with a program's source)
+ *                       programs restating deepwalk/ppr/node2vec (outlier
+ *                       included) reproduce orc_walk bit for bit; the tag walks
+ *                       of the reference's bound and outlier tests and the
+ *                       return-biased example walk follow their closed forms;
+ *                       a planted mistake in a program is rejected
  */
 #ifndef KKW_ORACLE_H
 #define KKW_ORACLE_H

@@ -21,6 +21,12 @@
  *   - step++, update_state(walker, current, edge), move (1039-1056).
  * Random words come from the oracle's Philox stream at the counters the
  * library's built-in walks use (kkw_oracle.h).
+ *
+ * Pinned by (tests/test_oracle_programs.py): programs restating DeepWalk, PPR
+ * and node2vec -- its outlier pair included -- give orc_walk's paths bit for
+ * bit; the tag walks of src/tests/test_bound.cpp and test_outlier.cpp and the
+ * return-biased walk of src/examples/dynamic_walk.cpp follow their closed-form
+ * transition laws; a program with a term dropped is rejected by that check.
  */
 #include "kkw_program.h"
 

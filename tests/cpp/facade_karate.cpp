@@ -1,9 +1,12 @@
 // Drives the C++ facade the way KnightKing's apps and examples do: load a text
 // edge list, run node2vec / deepwalk (two epochs) / ppr (start list) / a
-// std::function-only config (must be refused), dump PathSets to files.
-// usage: facade_karate <karate.txt> <out_prefix>
+// std::function-only config (must be refused) / a WalkProgram (the dynamic
+// walk example as device code), dump PathSets to files.
+// usage: facade_karate <karate.txt> <out_prefix> <program.cpp>
 #include <stdio.h>
 
+#include <fstream>
+#include <sstream>
 #include <string>
 
 #include "knightking/deepwalk.hpp"
@@ -12,7 +15,7 @@
 
 int main(int argc, char **argv)
 {
-    if (argc != 3) return 2;
+    if (argc != 4) return 2;
     std::string out = argv[2];
     {
         WalkEngine<real_t, Node2vecState> graph;
@@ -51,6 +54,23 @@ int main(int argc, char **argv)
             return 4;
         } catch (const std::logic_error &) {
         }
+    }
+    {
+        // src/examples/dynamic_walk.cpp as a WalkProgram, two epochs, with head info
+        WalkEngine<real_t, EmptyData> graph;
+        graph.load_graph(34, argv[1], false, GF_Edgelist);
+        std::ifstream f(argv[3]);
+        std::stringstream ss;
+        ss << f.rdbuf();
+        WalkProgram prog(ss.str());
+        prog.param[0] = 10;     // walk length
+        prog.max_length = 10;
+        prog.use_weights = true;
+        WalkerConfig<real_t, EmptyData> wconf(68);
+        WalkConfig wc;
+        wc.set_output_file((out + "_prog").c_str(), true);
+        wc.set_walk_rate(0.5);
+        graph.random_walk(&wconf, &prog, &wc);   // seed 1
     }
     printf("facade ok\n");
     return 0;

@@ -1,6 +1,6 @@
 """The C++ facade (include/knightking/) compiles as C++11 against kkw.h, and on
 a GPU it reproduces the oracle's paths through node2vec()/deepwalk()/ppr()
-with WalkConfig output files, epochs and consumers."""
+and a WalkProgram, with WalkConfig output files, epochs and consumers."""
 import os
 import subprocess
 
@@ -37,7 +37,8 @@ def read_dump(path, head):
 def test_facade_matches_oracle(tmp_path, karate):
     exe = build(tmp_path)
     prefix = str(tmp_path / "out")
-    r = subprocess.run([exe, os.path.join(ROOT, "tests", "golden", "karate.txt"), prefix],
+    prog_src = os.path.join(ROOT, "tests", "programs", "return_bias.cpp")
+    r = subprocess.run([exe, os.path.join(ROOT, "tests", "golden", "karate.txt"), prefix, prog_src],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     src, dst, w = karate
@@ -60,3 +61,9 @@ def test_facade_matches_oracle(tmp_path, karate):
     got = read_dump(prefix + "_ppr.0", True)
     for i, (wid, n, verts) in enumerate(got):
         assert wid == i and verts == paths[i, :lens[i]].tolist()
+    pa, la, _ = O.program_walk(O.program_oracle(prog_src, str(tmp_path / "orc")), rp2, col2, w2, params=[10],
+                               use_weights=True, seed=1, walker_num=68, max_length=10)
+    got = read_dump(prefix + "_prog.0", True)
+    assert [x[0] for x in got] == list(range(68))
+    for i, (wid, n, verts) in enumerate(got):
+        assert n == la[i] and verts == pa[i, :la[i]].tolist()

@@ -232,3 +232,17 @@ def test_walk_cap_and_second_order_retest(build_dir):
     pb, lb, tb = O.program_walk(lib, rp, col, w, params=[1000], use_weights=True, seed=1, walker_num=500,
                                 max_length=12, second_order=True)
     same_paths(pa, la, pb, lb)
+
+
+def test_local_alias_tables_are_the_id_tables():
+    """orc_alias_build_local: the same buckets, with local edge indices in
+    place of neighbour ids (col[row begin + local] gives the id back)."""
+    rp, col, w = O.rmat_graph(11, 16, seed=8, weighted=True)
+    p, prim, al, _, vs = O.alias_build(rp, col, w)
+    lp, lprim, lal, lvs = O.alias_build_local(rp, w)
+    E = len(col)
+    begin = np.repeat(rp[:-1].astype(np.int64), np.diff(rp.astype(np.int64)))
+    assert np.array_equal(p.view(np.uint32), lp[:E].view(np.uint32))
+    assert np.array_equal(prim, col[begin + lprim[:E]])
+    assert np.array_equal(al, col[begin + lal[:E]])
+    assert np.array_equal(vs.view(np.uint32), lvs.view(np.uint32))
