@@ -246,7 +246,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default persistent launch shape")
+    ap.add_argument("--no-autotune", action="store_true", help="use --shape instead of the autotuner's pick")
+    ap.add_argument("--shape", default="64,16,64",
+                    help="block,blocks_per_cu,chunk of the persistent launch with --no-autotune")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched-mixed and DeepWalk lines")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--calib", action="store_true",
@@ -292,8 +294,10 @@ def main():
         f"kernel bytes/step={bytes_launch / max(t['steps'], 1):.1f} (64 B lines: {touched / max(t['steps'], 1):.1f})")
 
     pb = eng.batch([job], kk.MODE_PERSISTENT, kk.BATCH_GRAPH)
-    shape = (256, 8, 64)
-    if not args.no_autotune:
+    shape = tuple(int(x) for x in args.shape.split(","))
+    if args.no_autotune:
+        pb.set_shape(*shape)
+    else:
         shape = pb.autotune(None)
         log(f"autotuned shape block/blocks_per_cu/chunk = {shape}")
     wall_p, kms_p = time_batch(pb, args.steps, args.warmup)
