@@ -179,6 +179,97 @@ def deepwalk_line(kk, steps, warmup, scale=22):
         eng.close()
 
 
+def unweighted_line(kk, eng, V, steps, warmup):
+    """node2vec p=2 q=0.5 without edge weights (uniform static component) on the
+    resident R-MAT-24 graph, walkers = |V|, L=80: persistent vs plain."""
+    job = kk.node2vec_job(V, 80, 2.0, 0.5, False, seed=1)
+    out = {}
+    for name, mode, flags in (("persistent", kk.MODE_PERSISTENT, kk.BATCH_GRAPH), ("plain", kk.MODE_PLAIN, 0)):
+        b = eng.batch([job], mode, flags)
+        if mode == kk.MODE_PERSISTENT:
+            b.set_shape(64, 16, 64)
+        wall, _ = time_batch(b, steps, warmup)
+        s = b.steps(0)
+        out[name + "_steps_per_s"] = round(s * steps / wall, 1)
+        out["steps_per_launch"] = int(s)
+        b.close()
+    out["persistent_over_plain"] = round(out["persistent_steps_per_s"] / out["plain_steps_per_s"], 3)
+    log(f"node2vec unweighted R-MAT-24: {out}")
+    return out
+
+
+def program_line(kk, eng, V, builtin_paths):
+    """node2vec p=2 q=0.5 weighted written as a walk program
+    (knightking_amd/programs/node2vec.cpp, compiled at run time), on the
+    resident R-MAT-24 graph: kernel time of one run and equality with the
+    built-in kernel's paths."""
+    from oracle import oracle as O
+    with open(os.path.join(ROOT, "knightking_amd", "programs", "node2vec.cpp")) as f:
+        src = f.read()
+    t0 = time.perf_counter()
+    prog = eng.program(src)
+    compile_s = time.perf_counter() - t0
+    try:
+        up, lo, ip, iq, ov = O.node2vec_constants(2.0, 0.5)
+        params = [80, up, lo, ip, iq, ov, 1.0]
+        prog.run(1024, 80, use_weights=True, seed=1, second_order=True, params=params)   # tables, warm-up
+        got = prog.run(V, 80, use_weights=True, seed=1, second_order=True, params=params)
+        kms = eng.timing()["kernel_ms"]
+    finally:
+        prog.close()
+    n = builtin_paths.vertices.shape[0]
+    mask = np.arange(81)[None, :] < builtin_paths.lengths[:, None]
+    same = bool(np.array_equal(got.lengths, builtin_paths.lengths) and
+                np.all((got.vertices[:n, :81] == builtin_paths.vertices[:, :81]) | ~mask))
+    out = {"steps_per_s": round(got.steps / (kms / 1e3), 1), "kernel_ms": round(kms, 3),
+           "compile_s": round(compile_s, 2), "same_paths_as_builtin": same,
+           "form": "plain, one walker per lane; adjacency through kkw_adjacent (neighbour hash sets)"}
+    log(f"node2vec as a walk program: {out}")
+    return out
+
+
+def rmat14_line(kk, steps, warmup):
+    """The reference's CPU-runnable case: node2vec p=2 q=0.5 weighted on
+    R-MAT-14, 16,384 walkers (one per vertex), L=80, seed 1 -- GPU launches
+    beside the oracle on the host cores, whole job, with parity."""
+    from oracle import oracle as O
+    eng = kk.Engine()
+    try:
+        eng.graph_rmat(14, 16, 0.57, 0.19, 0.19, seed=1, weighted=True)
+        eng.prepare()
+        V, _ = eng.info()
+        job = kk.node2vec_job(V, 80, 2.0, 0.5, True, seed=1)
+        out = {}
+        for name, mode, flags in (("persistent", kk.MODE_PERSISTENT, kk.BATCH_GRAPH), ("plain", kk.MODE_PLAIN, 0)):
+            b = eng.batch([job], mode, flags)
+            wall, kms = time_batch(b, steps, warmup)
+            s = b.steps(0)
+            out[name + "_steps_per_s"] = round(s * steps / wall, 1)
+            out[name + "_kernel_ms"] = round(float(np.mean(kms)), 4)
+            out["steps_per_job"] = int(s)
+            if mode == kk.MODE_PERSISTENT:
+                gp = b.paths(0)
+            b.close()
+        rp, col, w, _ = eng.download(weights=True)
+        g = O.Graph(rp, col, w)
+        g.with_alias()
+        threads = max(1, min(os.cpu_count() or 1, 16))
+        reps, dt = 0, 0.0
+        while dt < 2.0:
+            t0 = time.perf_counter()
+            paths, lens, _, st = O.walk(g, O.NODE2VEC, V, 80, True, p=2.0, q=0.5, seed=1, threads=threads)
+            dt += time.perf_counter() - t0
+            reps += 1
+        out["cpu_oracle_steps_per_s"] = round(st.steps * reps / dt, 1)
+        out["cpu_threads"] = threads
+        mask = np.arange(81)[None, :] < lens[:, None]
+        out["bit_exact"] = bool(np.array_equal(gp.lengths, lens) and np.all((gp.vertices[:, :81] == paths) | ~mask))
+        log(f"node2vec R-MAT-14 (CPU-path case): {out}")
+        return out
+    finally:
+        eng.close()
+
+
 def serve_latency(eng, rp, rates, seconds, seed=7):
     """Poisson PPR queries (t=0.15, 1e3..1e5 walkers log-uniform, source =
     source of a uniformly drawn directed edge) at each rate; latency from
@@ -345,6 +436,8 @@ def main():
         cpu = {"value": round(rate, 1), "unit": "walker steps/s", "cores": threads, "kind": "port",
                "sample": sample}
         log(f"cpu baseline {rate / 1e6:.2f} M steps/s on {threads} threads; sample parity {parity}")
+    run_extra = not args.no_extra and weighted and args.scale == 24
+    builtin_paths = pb.paths(0) if run_extra else None
     pb.close()
 
     variants = None
@@ -362,11 +455,15 @@ def main():
             log(f"{name}: {variants[name] / 1e9:.3f} G steps/s")
         eng.adj_index(True)
 
-    extra_batched = extra_deepwalk = None
-    if not args.no_extra and weighted and args.scale == 24:
+    extra_batched = extra_deepwalk = extra_unweighted = extra_rmat14 = extra_program = None
+    if run_extra:
+        extra_program = program_line(kk, eng, V, builtin_paths)
+        builtin_paths = None
+        extra_unweighted = unweighted_line(kk, eng, V, args.steps, args.warmup)
         eng.close()
         extra_batched = batched_vs_plain(kk, args.steps, args.warmup)
         extra_deepwalk = deepwalk_line(kk, args.steps, args.warmup)
+        extra_rmat14 = rmat14_line(kk, args.steps, args.warmup)
         eng = kk.Engine()
         eng.graph_rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=1, weighted=True)
         eng.prepare()
@@ -435,6 +532,9 @@ def main():
         "sample_parity": parity,
         "batched_mixed_jobs": extra_batched,
         "deepwalk_rmat22": extra_deepwalk,
+        "node2vec_unweighted_rmat24": extra_unweighted,
+        "node2vec_rmat14_cpu_case": extra_rmat14,
+        "node2vec_program_rmat24": extra_program,
         "ppr_serving": serving,
     }
     print(json.dumps(line), flush=True)

@@ -17,14 +17,14 @@
  *   KKW_FN void init_state(WalkerState &s, const kkw_walker &w, uint32_t start,
  *                          const kkw_ctx &c);                // walker_init_state_func
  *   KKW_FN void update_state(WalkerState &s, const kkw_walker &w, uint32_t cur,
- *                            uint64_t edge, const kkw_ctx &c); // walker_update_state_func
+ *                            const kkw_edge &e, const kkw_ctx &c); // walker_update_state_func
  *   KKW_FN float extension(const WalkerState &s, const kkw_walker &w, uint32_t cur,
  *                          const kkw_ctx &c);                // extension_comp
  *
  * and, after `#define KKW_DYNAMIC 1`:
  *
  *   KKW_FN float dynamic_comp(const WalkerState &s, const kkw_walker &w, uint32_t cur,
- *                             uint64_t edge, const kkw_ctx &c);
+ *                             const kkw_edge &e, const kkw_ctx &c);
  *   KKW_FN float upper_bound(uint32_t v, const kkw_ctx &c);  // dcomp_upperbound_func
  *
  * optionally `#define KKW_LOWER 1` with
@@ -38,10 +38,12 @@
  *   KKW_FN int64_t outlier_search(const WalkerState &s, const kkw_walker &w, uint32_t cur,
  *                                 uint32_t idx, const kkw_ctx &c);  // edge or -1
  *
- * `edge` is a global CSR position: the candidate is c.col[edge], its weight
- * c.weight[edge] (when the graph has weights), its type c.etype[edge].  In
- * update_state, w.step is already the new step (walk.hpp:1039-1043); in the
- * other functions it is the step being taken.
+ * `e` is the candidate edge, KnightKing's AdjUnit: e.neighbour the vertex
+ * it leads to, e.index its global CSR position -- its weight is
+ * c.weight[e.index] (when the graph has weights), its type c.etype[e.index],
+ * and per-edge caller data is indexed the same way.  outlier_search returns
+ * an index.  In update_state, w.step is already the new step
+ * (walk.hpp:1039-1043); in the other functions it is the step being taken.
  */
 #ifndef KKW_PROGRAM_H
 #define KKW_PROGRAM_H
@@ -71,15 +73,78 @@ typedef __hip_internal::int64_t int64_t;
  * the device every pointer is a device address. */
 typedef struct kkw_ctx {
     uint32_t v_num;
-    uint32_t reserved;
+    uint32_t sorted;           /* 1: every list is sorted by neighbour id */
     uint64_t e_num;
     const uint64_t *row_ptr;   /* [v_num + 1] */
     const uint32_t *col;       /* [e_num] */
     const float *weight;       /* [e_num], or null for an unweighted graph */
     const uint8_t *etype;      /* [e_num], or null */
+    const uint32_t *adjset;    /* neighbour hash sets (below), or null */
     const void *array[KKW_PROGRAM_ARRAYS];   /* kkw_program_set_array slots (null if unset) */
     float param[KKW_PROGRAM_PARAMS];         /* kkw_prog_job.param */
 } kkw_ctx;
+
+/* 1 if u is in v's list, else 0 -- the adjacency test of a second-order
+ * walk (node2vec's respond_query, src/apps/node2vec.hpp:158-166).  Reads the
+ * engine's neighbour hash sets when present: v's table of 2 deg(v) + 4 slots
+ * starts at 2 row_ptr[v] + 4 v, slot of a key = (key * 0x9E3779B1 mod 2^32) *
+ * size / 2^32, linear probing, 0xFFFFFFFF empty.  Otherwise a binary search
+ * (sorted lists) or a scan.  The answer does not depend on the method. */
+KKW_FN int kkw_adjacent(uint32_t v, uint32_t u, const struct kkw_ctx &c)
+{
+    const uint64_t b = c.row_ptr[v], e = c.row_ptr[v + 1];
+    if (e == b) return 0;
+    if (c.adjset) {
+        const uint32_t size = (uint32_t)(2 * (e - b) + 4);
+        const uint64_t tb = 2 * b + 4 * (uint64_t)v;   /* table start in adjset */
+        uint32_t s = (uint32_t)(((uint64_t)(u * 0x9E3779B1u) * size) >> 32);
+#if defined(__HIPCC_RTC__) || defined(__HIPCC__)
+        /* one aligned 16-byte block (four slots) per read, slots in probe order */
+        for (uint32_t seen = 0; seen < size;) {
+            const uint64_t a = tb + s, blk = a & ~(uint64_t)3;
+            const uint4 q = *(const uint4 *)(c.adjset + blk);
+            const uint32_t e4[4] = {q.x, q.y, q.z, q.w};
+            for (uint32_t k = (uint32_t)(a - blk); k < 4 && blk + k < tb + size; k++, seen++) {
+                if (e4[k] == u) return 1;
+                if (e4[k] == 0xFFFFFFFFu) return 0;
+            }
+            s = blk + 4 >= tb + size ? 0u : (uint32_t)(blk + 4 - tb);
+        }
+#else
+        for (uint32_t n = 0; n < size; n++) {
+            const uint32_t x = c.adjset[tb + s];
+            if (x == u) return 1;
+            if (x == 0xFFFFFFFFu) return 0;
+            s = s + 1 == size ? 0 : s + 1;
+        }
+#endif
+        return 0;
+    }
+    if (c.sorted) {
+        uint64_t lo = b, len = e - b;
+        while (len > 0) {
+            const uint64_t half = len >> 1;
+            if (c.col[lo + half] < u) {
+                lo += half + 1;
+                len -= half + 1;
+            } else {
+                len = half;
+            }
+        }
+        return lo < e && c.col[lo] == u;
+    }
+    for (uint64_t k = b; k < e; k++)
+        if (c.col[k] == u) return 1;
+    return 0;
+}
+
+/* AdjUnit<T> (include/graph.hpp:49-63): the neighbour and where the edge's
+ * data lives. */
+typedef struct kkw_edge {
+    uint64_t index;       /* global CSR position */
+    uint32_t neighbour;   /* c.col[index] */
+    uint32_t reserved;
+} kkw_edge;
 
 /* Walker<T>::id and ::step (include/walk.hpp:31-50). */
 typedef struct kkw_walker {

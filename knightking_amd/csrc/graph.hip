@@ -388,6 +388,10 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
             bk.primary = to_ids ? cv[se.x] : se.x;
             bk.alias = to_ids ? cv[le.x] : le.x;
             bk.meta = tv ? ((uint32_t)tv[se.x] | ((uint32_t)tv[le.x] << 8)) : 0u;
+            if (!to_ids) {   // local tables: the candidates' ids in the row fields
+                bk.prow = cv[se.x];
+                bk.arow = cv[le.x];
+            }
             float diff = ave - ps;
             pl -= diff;
             uint2 ne = make_uint2(le.x, __float_as_uint(pl));
@@ -398,6 +402,7 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
             bk.primary = to_ids ? cv[se.x] : se.x;
             bk.alias = bk.primary;
             bk.meta = tv ? (uint32_t)tv[se.x] * 0x101u : 0u;
+            if (!to_ids) bk.prow = bk.arow = cv[se.x];
         }
         *o++ = bk;
     }
@@ -410,6 +415,7 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
         bk.primary = to_ids ? cv[le.x] : le.x;
         bk.alias = bk.primary;
         bk.meta = tv ? (uint32_t)tv[le.x] * 0x101u : 0u;
+        if (!to_ids) bk.prow = bk.arow = cv[le.x];
         *o++ = bk;
     }
 }
@@ -427,7 +433,8 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
 // sequential FIFO run k steps at a time.  Queue windows of AW_WIN entries live
 // in LDS; lists longer than that also keep their queues in global memory and
 // refill the windows from there.  Buckets first hold local edge indices; the
-// last pass turns them into neighbour ids and edge types.
+// last pass turns them into neighbour ids and edge types (or, for the tables
+// of walk programs, keeps the indices and adds the ids).
 constexpr uint32_t AW_MIN = 32;
 constexpr uint32_t AW_WIN = 512;
 constexpr int AW_WAVES = 4;
@@ -589,13 +596,18 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
         o[ob + i] = bk;
     }
     wave_sync();
-    if (!to_ids) return;   // tables of local edge indices (walk programs)
-    // 5. local edge indices -> neighbour ids and edge types
+    // 5. local edge indices -> neighbour ids and edge types; tables of local
+    // indices (walk programs) keep them and take the ids in the row fields
     for (uint32_t i = lane; i < d; i += 64) {
         const uint32_t pe = o[i].primary, ae = o[i].alias;
         o[i].meta = etype ? ((uint32_t)etype[b + pe] | ((uint32_t)etype[b + ae] << 8)) : 0u;
-        o[i].primary = col[b + pe];
-        o[i].alias = col[b + ae];
+        if (to_ids) {
+            o[i].primary = col[b + pe];
+            o[i].alias = col[b + ae];
+        } else {
+            o[i].prow = col[b + pe];
+            o[i].arow = col[b + ae];
+        }
     }
 }
 

@@ -30,6 +30,7 @@ struct kkw_program {
     hipModule_t mod = nullptr;
     hipFunction_t fn = nullptr;
     std::string log;
+    bool uses_adjacency = false;   // the source calls kkw_adjacent: give it the hash sets
     void *arrays[KKW_PROGRAM_ARRAYS] = {};
     uint64_t array_bytes[KKW_PROGRAM_ARRAYS] = {};
     ~kkw_program()
@@ -115,6 +116,7 @@ int kkw_program_create(kkw_engine *eng, const char *source, const char *options,
         if (colon != std::string::npos) arch.resize(colon);
         std::unique_ptr<kkw_program> p(new kkw_program());
         p->eng = eng;
+        p->uses_adjacency = strstr(source, "kkw_adjacent") != nullptr;
         std::vector<char> code = compile(arch, source, options, p->log);
         KKW_HIP(hipModuleLoadData(&p->mod, code.data()));
         KKW_HIP(hipModuleGetFunction(&p->fn, p->mod, "kkw_program_walk"));
@@ -179,6 +181,12 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
         if ((uint64_t)job->walker_begin + job->walker_num > 0xFFFFFFFFull)
             return fail(KKW_E_INVALID, "walker ids exceed 32 bits");
         if (job->max_length >= 0x7FFFFFF0u) return fail(KKW_E_INVALID, "max_length too large");
+        if (prog->uses_adjacency && eng->adj_index_auto && !gs.adjset) {
+            double t0 = now_ms();
+            eng->graph_version++;
+            build_adj_index(gs, eng->stream);
+            eng->timing.index_ms = now_ms() - t0;
+        }
         if (job->use_weights && !gs.lbucket) {
             // local-index alias tables; the sums array keeps its address, so
             // batches captured earlier stay valid
@@ -206,7 +214,9 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
         kkw_prog_args A;
         memset(&A, 0, sizeof(A));
         A.c.v_num = gs.V;
+        A.c.sorted = gs.sorted ? 1u : 0u;
         A.c.e_num = gs.E;
+        A.c.adjset = gs.adjset;
         A.c.row_ptr = gs.row_ptr;
         A.c.col = gs.col;
         A.c.weight = gs.w;

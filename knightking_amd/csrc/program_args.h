@@ -5,7 +5,8 @@
 #include "kkw_program.h"
 
 // Layout of the library's 32-byte alias bucket; walk programs use tables whose
-// primary/alias fields hold local edge indices.
+// primary/alias fields hold local edge indices and whose prow/arow fields hold
+// the two candidates' vertex ids.
 struct kkw_prog_bucket {
     float p;
     uint32_t primary;

@@ -35,6 +35,25 @@
 
 #define KKW_PROG_MAX_TRIALS (1u << 20)   // per step; guards degenerate programs
 
+typedef uint32_t kkw_prog_u4a __attribute__((ext_vector_type(4), aligned(4)));
+
+// row_ptr[v] and row_ptr[v + 1] in one 16-byte load
+__device__ __forceinline__ void kkw_prog_row(const uint64_t *rp, uint32_t v, uint64_t &beg, uint32_t &deg)
+{
+    const kkw_prog_u4a q = *reinterpret_cast<const kkw_prog_u4a *>(rp + v);
+    beg = (uint64_t)q.x | ((uint64_t)q.y << 32);
+    deg = (uint32_t)(((uint64_t)q.z | ((uint64_t)q.w << 32)) - beg);
+}
+
+// four path entries held in registers, stored as one aligned 16-byte group
+struct kkw_prog_group {
+    uint32_t v[4];
+    __device__ __forceinline__ void store(uint32_t *path, uint32_t step) const
+    {
+        *reinterpret_cast<uint4 *>(path + (step & ~3u)) = make_uint4(v[0], v[1], v[2], v[3]);
+    }
+};
+
 __device__ __forceinline__ bool kkw_prog_terminates(const WalkerState &s, const kkw_walker &w, uint32_t cur,
                                                     uint32_t deg, uint32_t trial, const kkw_prog_args &A)
 {
@@ -45,10 +64,10 @@ __device__ __forceinline__ bool kkw_prog_terminates(const WalkerState &s, const 
     return false;
 }
 
-// One trial at cur; returns the accepted edge or -1.
-__device__ __forceinline__ int64_t kkw_prog_trial(const WalkerState &s, const kkw_walker &w, uint32_t cur,
-                                                  uint64_t beg, uint32_t deg, uint32_t trial,
-                                                  const kkw_prog_args &A)
+// One trial at cur; returns true with the accepted edge in `acc`.
+__device__ __forceinline__ bool kkw_prog_trial(const WalkerState &s, const kkw_walker &w, uint32_t cur,
+                                               uint64_t beg, uint32_t deg, uint32_t trial, const kkw_prog_args &A,
+                                               kkw_edge &acc)
 {
     const kkw_ctx &c = A.c;
 #if KKW_DYNAMIC
@@ -67,25 +86,40 @@ __device__ __forceinline__ int64_t kkw_prog_trial(const WalkerState &s, const kk
             if (rv > 0.0f) {
                 uint32_t idx = (uint32_t)(rv / ov);
                 if (idx + 1 > num) idx = num - 1;
-                const int64_t e = outlier_search(s, w, cur, idx, c);
-                if (e >= 0) {
-                    const float sc = A.lb ? c.weight[e] : 1.0f;
-                    const float h = (rv - (float)idx * ov) / sc + up;
-                    if (h <= dynamic_comp(s, w, cur, (uint64_t)e, c)) return e;
-                }
-                return -1;
+                const int64_t ei = outlier_search(s, w, cur, idx, c);
+                if (ei < 0) return false;
+                kkw_edge e;
+                e.index = (uint64_t)ei;
+                e.neighbour = c.col[ei];
+                e.reserved = 0;
+                const float sc = A.lb ? c.weight[ei] : 1.0f;
+                const float h = (rv - (float)idx * ov) / sc + up;
+                if (!(h <= dynamic_comp(s, w, cur, e, c))) return false;
+                acc = e;
+                return true;
             }
         }
     }
 #endif
     const kkw::u32x4 r = kkw::draw(A.seed, w.id, w.step, trial, kkw::TAG_TRIAL);
     const uint32_t k = kkw::bounded(r.x, r.y, deg);
-    uint32_t el = k;
+    kkw_edge e;
+    e.reserved = 0;
     if (A.lb) {
+        // first half {p, local primary, primary's id}; the alias half only when
+        // the coin picks it
         const kkw_prog_bucket &b = A.lb[beg + k];
-        el = kkw::u01(r.z) < b.p ? b.primary : b.alias;
+        if (kkw::u01(r.z) < b.p) {
+            e.index = beg + b.primary;
+            e.neighbour = (uint32_t)b.prow;
+        } else {
+            e.index = beg + b.alias;
+            e.neighbour = (uint32_t)b.arow;
+        }
+    } else {
+        e.index = beg + k;
+        e.neighbour = c.col[e.index];
     }
-    const uint64_t e = beg + el;
 #if KKW_DYNAMIC
     const float dart = kkw::u01(r.w) * up;
     bool ok = false;
@@ -93,10 +127,10 @@ __device__ __forceinline__ int64_t kkw_prog_trial(const WalkerState &s, const kk
     ok = dart <= lower_bound(cur, c);
 #endif
     if (!ok) ok = dart <= dynamic_comp(s, w, cur, e, c);
-    return ok ? (int64_t)e : -1;
-#else
-    return (int64_t)e;
+    if (!ok) return false;
 #endif
+    acc = e;
+    return true;
 }
 
 extern "C" __global__ void __launch_bounds__(256) kkw_program_walk(kkw_prog_args A)
@@ -118,10 +152,12 @@ extern "C" __global__ void __launch_bounds__(256) kkw_program_walk(kkw_prog_args
     }
     WalkerState s;
     init_state(s, w, cur, c);
-    uint32_t *path = A.paths + (uint64_t)i * A.stride;
-    path[0] = cur;
-    uint64_t beg = c.row_ptr[cur];
-    uint32_t deg = (uint32_t)(c.row_ptr[cur + 1] - beg);
+    uint32_t *path = A.paths + (uint64_t)i * A.stride;   // stride is a multiple of 4
+    kkw_prog_group grp;
+    grp.v[0] = grp.v[1] = grp.v[2] = grp.v[3] = cur;
+    uint64_t beg;
+    uint32_t deg;
+    kkw_prog_row(c.row_ptr, cur, beg, deg);
     bool capped = false;
     uint32_t trial = 0;
     for (;;) {
@@ -136,18 +172,19 @@ extern "C" __global__ void __launch_bounds__(256) kkw_program_walk(kkw_prog_args
             capped = true;
             break;
         }
-        const int64_t e = kkw_prog_trial(s, w, cur, beg, deg, trial, A);
-        if (e < 0) {
+        kkw_edge e;
+        if (!kkw_prog_trial(s, w, cur, beg, deg, trial, A, e)) {
             trial++;
             continue;
         }
         w.step++;
-        update_state(s, w, cur, (uint64_t)e, c);
-        cur = c.col[e];
-        path[w.step] = cur;
-        beg = c.row_ptr[cur];
-        deg = (uint32_t)(c.row_ptr[cur + 1] - beg);
+        update_state(s, w, cur, e, c);
+        cur = e.neighbour;
+        grp.v[w.step & 3u] = cur;
+        if ((w.step & 3u) == 3u) grp.store(path, w.step);
+        kkw_prog_row(c.row_ptr, cur, beg, deg);
         trial = 0;
     }
+    if ((w.step & 3u) != 3u) grp.store(path, w.step);   // the last, partial group
     A.lens[i] = (w.step + 1) | (capped ? 0x80000000u : 0u);
 }

@@ -1,6 +1,7 @@
 // node2vec as a walk program (src/apps/node2vec.hpp:108-197), outlier pair
 // included (node2vec.hpp:47-106; skipped at step 0, where the reference reads
 // an unset previous vertex).  Lists must be sorted (node2vec.hpp:122-125).
+// The adjacency test uses the engine's neighbour hash sets (kkw_adjacent).
 // param: 0 walk length, 1 upper, 2 lower, 3 1/p, 4 1/q, 5 overflow
 // (1/p - upper), 6 weighted (1) or not (0).
 #define KKW_DYNAMIC 1
@@ -12,7 +13,7 @@ struct WalkerState {
 };
 
 KKW_FN void init_state(WalkerState &s, const kkw_walker &, uint32_t, const kkw_ctx &) { s.prev = 0; }
-KKW_FN void update_state(WalkerState &s, const kkw_walker &, uint32_t cur, uint64_t, const kkw_ctx &) { s.prev = cur; }
+KKW_FN void update_state(WalkerState &s, const kkw_walker &, uint32_t cur, const kkw_edge &, const kkw_ctx &) { s.prev = cur; }
 KKW_FN float extension(const WalkerState &, const kkw_walker &w, uint32_t, const kkw_ctx &c)
 {
     return w.step >= (uint32_t)c.param[0] ? 0.0f : 1.0f;
@@ -34,14 +35,12 @@ KKW_FN uint64_t lower_pos(uint32_t v, uint32_t x, const kkw_ctx &c)
     return lo;
 }
 
-KKW_FN float dynamic_comp(const WalkerState &s, const kkw_walker &w, uint32_t, uint64_t edge, const kkw_ctx &c)
+KKW_FN float dynamic_comp(const WalkerState &s, const kkw_walker &w, uint32_t, const kkw_edge &e, const kkw_ctx &c)
 {
     if (w.step == 0) return c.param[1];
-    const uint32_t cand = c.col[edge];
+    const uint32_t cand = e.neighbour;
     if (cand == s.prev) return c.param[3];
-    const uint64_t pos = lower_pos(s.prev, cand, c);
-    const bool adjacent = pos < c.row_ptr[s.prev + 1] && c.col[pos] == cand;
-    return adjacent ? 1.0f : c.param[4];
+    return kkw_adjacent(s.prev, cand, c) ? 1.0f : c.param[4];   // respond_query (node2vec.hpp:158-166)
 }
 KKW_FN float upper_bound(uint32_t, const kkw_ctx &c) { return c.param[1]; }
 KKW_FN float lower_bound(uint32_t, const kkw_ctx &c) { return c.param[2]; }

@@ -5,5 +5,5 @@ struct WalkerState {
 };
 
 KKW_FN void init_state(WalkerState &s, const kkw_walker &, uint32_t, const kkw_ctx &) { s.unused = 0; }
-KKW_FN void update_state(WalkerState &, const kkw_walker &, uint32_t, uint64_t, const kkw_ctx &) {}
+KKW_FN void update_state(WalkerState &, const kkw_walker &, uint32_t, const kkw_edge &, const kkw_ctx &) {}
 KKW_FN float extension(const WalkerState &, const kkw_walker &, uint32_t, const kkw_ctx &c) { return c.param[0]; }

@@ -8,14 +8,14 @@ struct WalkerState {
 };
 
 KKW_FN void init_state(WalkerState &s, const kkw_walker &, uint32_t, const kkw_ctx &) { s.last = 0xFFFFFFFFu; }
-KKW_FN void update_state(WalkerState &s, const kkw_walker &, uint32_t cur, uint64_t, const kkw_ctx &) { s.last = cur; }
+KKW_FN void update_state(WalkerState &s, const kkw_walker &, uint32_t cur, const kkw_edge &, const kkw_ctx &) { s.last = cur; }
 KKW_FN float extension(const WalkerState &, const kkw_walker &w, uint32_t, const kkw_ctx &c)
 {
     return w.step >= (uint32_t)c.param[0] ? 0.0f : 1.0f;
 }
-KKW_FN float dynamic_comp(const WalkerState &s, const kkw_walker &w, uint32_t, uint64_t edge, const kkw_ctx &c)
+KKW_FN float dynamic_comp(const WalkerState &s, const kkw_walker &w, uint32_t, const kkw_edge &e, const kkw_ctx &)
 {
     if (w.step == 0) return 1.0f;
-    return c.col[edge] == s.last ? 2.0f : 1.0f;
+    return e.neighbour == s.last ? 2.0f : 1.0f;
 }
 KKW_FN float upper_bound(uint32_t, const kkw_ctx &) { return 2.0f; }

@@ -302,7 +302,7 @@ def program_oracle(source_path: str, build_dir: str):
     L = C.CDLL(out)
     P, U32, U64, I = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
     L.orc_program_walk.restype = I
-    L.orc_program_walk.argtypes = [U32, P, P, P, P, P, P, P, P, P, P, U64, U32, U32, U32, U32, I, U32, P, I, P, P]
+    L.orc_program_walk.argtypes = [U32, P, P, P, P, I, P, P, P, P, P, P, U64, U32, U32, U32, U32, I, U32, P, I, P, P]
     return L
 
 
@@ -331,7 +331,13 @@ def program_walk(plib, row_ptr, col, w=None, etype=None, arrays=(), params=(), u
     else:
         starts = None
         mode = START_UNIFORM if uniform_start else START_EQUAL
-    plib.orc_program_walk(V, _p(row_ptr), _p(col), _p(w), _p(etype), arr, par, _p(bp), _p(prim), _p(al), _p(vs),
+    d = np.diff(col.astype(np.int64))
+    starts_of = row_ptr[1:-1].astype(np.int64) - 1
+    breaks = np.zeros(len(d), bool)
+    breaks[starts_of[(starts_of >= 0) & (starts_of < len(d))]] = True
+    sorted_lists = bool(np.all((d >= 0) | breaks))
+    plib.orc_program_walk(V, _p(row_ptr), _p(col), _p(w), _p(etype), int(sorted_lists), arr, par,
+                          _p(bp), _p(prim), _p(al), _p(vs),
                           seed, walker_begin, walker_num, max_length, stride, mode,
                           0 if starts is None else len(starts), _p(starts), int(second_order), _p(paths), _p(lens))
     raw = lens[:walker_num]

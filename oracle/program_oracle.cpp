@@ -86,8 +86,8 @@ bool terminates(const WalkerState &s, const kkw_walker &w, uint32_t cur, uint32_
     return false;
 }
 
-int64_t trial_at(const WalkerState &s, const kkw_walker &w, uint32_t cur, uint64_t beg, uint32_t deg,
-                 uint32_t trial, const Job &J)
+bool trial_at(const WalkerState &s, const kkw_walker &w, uint32_t cur, uint64_t beg, uint32_t deg,
+              uint32_t trial, const Job &J, kkw_edge &acc)
 {
     const kkw_ctx &c = J.c;
     (void)s;
@@ -105,11 +105,17 @@ int64_t trial_at(const WalkerState &s, const kkw_walker &w, uint32_t cur, uint64
         if (rv > 0.0f) {
             uint32_t idx = (uint32_t)(rv / ov);
             if (idx + 1 > num) idx = num - 1;
-            const int64_t e = outlier_search(s, w, cur, idx, c);
-            if (e < 0) return -1;
-            const float sc = J.bp ? c.weight[e] : 1.0f;
+            const int64_t ei = outlier_search(s, w, cur, idx, c);
+            if (ei < 0) return false;
+            kkw_edge e;
+            e.index = (uint64_t)ei;
+            e.neighbour = c.col[ei];
+            e.reserved = 0;
+            const float sc = J.bp ? c.weight[ei] : 1.0f;
             const float h = (rv - (float)idx * ov) / sc + up;
-            return h <= dynamic_comp(s, w, cur, (uint64_t)e, c) ? e : -1;
+            if (!(h <= dynamic_comp(s, w, cur, e, c))) return false;
+            acc = e;
+            return true;
         }
     }
 #endif
@@ -117,16 +123,21 @@ int64_t trial_at(const WalkerState &s, const kkw_walker &w, uint32_t cur, uint64
     const uint32_t k = orc_bounded(r.w[0], r.w[1], deg);
     uint32_t local = k;
     if (J.bp) local = orc_u01(r.w[2]) < J.bp[beg + k] ? J.bprim[beg + k] : J.balias[beg + k];
-    const uint64_t e = beg + local;
+    kkw_edge e;
+    e.index = beg + local;
+    e.neighbour = c.col[e.index];
+    e.reserved = 0;
 #if KKW_DYNAMIC
     const float dart = orc_u01(r.w[3]) * up;
+    bool ok = false;
 #if KKW_LOWER
-    if (dart <= lower_bound(cur, c)) return (int64_t)e;
+    ok = dart <= lower_bound(cur, c);
 #endif
-    return dart <= dynamic_comp(s, w, cur, e, c) ? (int64_t)e : -1;
-#else
-    return (int64_t)e;
+    if (!ok) ok = dart <= dynamic_comp(s, w, cur, e, c);
+    if (!ok) return false;
 #endif
+    acc = e;
+    return true;
 }
 
 }  // namespace
@@ -135,7 +146,7 @@ int64_t trial_at(const WalkerState &s, const kkw_walker &w, uint32_t cur, uint64
  * `stride` entries, lens = vertices | 1 << 31 when cut at max_len or by the
  * trial guard.  Returns 0. */
 extern "C" int orc_program_walk(uint32_t V, const uint64_t *row_ptr, const uint32_t *col, const float *weight,
-                                const uint8_t *etype, const void *const *arrays, const float *params,
+                                const uint8_t *etype, int sorted, const void *const *arrays, const float *params,
                                 const float *bp, const uint32_t *bprim, const uint32_t *balias,
                                 const float *vsum, uint64_t seed, uint32_t walker_begin, uint32_t walker_num,
                                 uint32_t max_len, uint32_t stride, int start_mode, uint32_t n_starts,
@@ -143,7 +154,8 @@ extern "C" int orc_program_walk(uint32_t V, const uint64_t *row_ptr, const uint3
 {
     Job J;
     J.c.v_num = V;
-    J.c.reserved = 0;
+    J.c.sorted = sorted ? 1u : 0u;
+    J.c.adjset = 0;   // kkw_adjacent answers by binary search or scan here
     J.c.e_num = row_ptr[V];
     J.c.row_ptr = row_ptr;
     J.c.col = col;
@@ -193,14 +205,14 @@ extern "C" int orc_program_walk(uint32_t V, const uint64_t *row_ptr, const uint3
                 capped = 1;
                 break;
             }
-            const int64_t e = trial_at(s, w, cur, beg, deg, trial, J);
-            if (e < 0) {
+            kkw_edge e;
+            if (!trial_at(s, w, cur, beg, deg, trial, J, e)) {
                 trial++;
                 continue;
             }
             w.step++;
-            update_state(s, w, cur, (uint64_t)e, J.c);
-            cur = col[e];
+            update_state(s, w, cur, e, J.c);
+            cur = e.neighbour;
             path[w.step] = cur;
             trial = 0;
         }

@@ -20,7 +20,7 @@ KKW_FN void init_state(WalkerState &s, const kkw_walker &w, uint32_t, const kkw_
     s.tag = (int32_t)(w.id % TAGS);
     s.prev = 0;
 }
-KKW_FN void update_state(WalkerState &s, const kkw_walker &, uint32_t cur, uint64_t, const kkw_ctx &)
+KKW_FN void update_state(WalkerState &s, const kkw_walker &, uint32_t cur, const kkw_edge &, const kkw_ctx &)
 {
     s.tag = (s.tag + 1) % TAGS;
     s.prev = cur;
@@ -29,10 +29,10 @@ KKW_FN float extension(const WalkerState &, const kkw_walker &w, uint32_t, const
 {
     return w.step >= (uint32_t)c.param[0] ? 0.0f : 1.0f;
 }
-KKW_FN float dynamic_comp(const WalkerState &s, const kkw_walker &w, uint32_t cur, uint64_t edge, const kkw_ctx &c)
+KKW_FN float dynamic_comp(const WalkerState &s, const kkw_walker &w, uint32_t cur, const kkw_edge &e, const kkw_ctx &c)
 {
     if (w.step == 0) return (float)TAGS;
-    float temp = (float)1 + (float)((vtag(c)[s.prev] + s.tag + (int32_t)c.etype[edge]) % TAGS);
+    float temp = (float)1 + (float)((vtag(c)[s.prev] + s.tag + (int32_t)c.etype[e.index]) % TAGS);
     temp += (float)vtag(c)[cur] * c.param[1];
     return temp;
 }

@@ -37,7 +37,7 @@ def read_dump(path, head):
 def test_facade_matches_oracle(tmp_path, karate):
     exe = build(tmp_path)
     prefix = str(tmp_path / "out")
-    prog_src = os.path.join(ROOT, "tests", "programs", "return_bias.cpp")
+    prog_src = os.path.join(ROOT, "knightking_amd", "programs", "return_bias.cpp")
     r = subprocess.run([exe, os.path.join(ROOT, "tests", "golden", "karate.txt"), prefix, prog_src],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr

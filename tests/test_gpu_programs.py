@@ -9,7 +9,7 @@ import pytest
 
 import knightking_amd as kk
 from oracle import oracle as O
-from test_oracle_programs import PROGRAMS, TAGS, same_paths, tagged_graph
+from test_oracle_programs import TAGS, program_path, same_paths, tagged_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -20,7 +20,7 @@ def build_dir(tmp_path_factory):
 
 
 def source(name):
-    with open(os.path.join(PROGRAMS, name + ".cpp")) as f:
+    with open(program_path(name)) as f:
         return f.read()
 
 
@@ -37,7 +37,7 @@ def both(engine, build_dir, name, rp, col, w=None, et=None, arrays=(), **run):
         got = prog.run(**run)
     finally:
         prog.close()
-    want = O.program_walk(O.program_oracle(os.path.join(PROGRAMS, name + ".cpp"), build_dir), rp, col, w, et,
+    want = O.program_walk(O.program_oracle(program_path(name), build_dir), rp, col, w, et,
                           arrays=arrays, params=run.get("params", ()), use_weights=run.get("use_weights", False),
                           seed=run.get("seed", 1), walker_num=run["walker_num"], max_length=run["max_length"],
                           walker_begin=run.get("walker_begin", 0), start_vertices=run.get("start_vertices"),

@@ -1,7 +1,7 @@
 """Walk programs on the oracle (oracle/program_oracle.cpp, compiled with each
 program's source).  The generic loop is pinned two ways:
 
-- programs that restate the built-in apps (tests/programs/deepwalk.cpp,
+- programs that restate the built-in apps (knightking_amd/programs/deepwalk.cpp,
   ppr.cpp, node2vec.cpp -- node2vec with its outlier appendix) give the
   built-in oracle walks bit for bit: two independent codings of the same
   counters and the same float arithmetic;
@@ -19,7 +19,18 @@ import pytest
 from oracle import oracle as O
 from test_oracle_alias_walk import chi2_rows, undirected_random_graph
 
-PROGRAMS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "programs")
+_TESTS = os.path.dirname(os.path.abspath(__file__))
+PROGRAM_DIRS = [os.path.join(os.path.dirname(_TESTS), "knightking_amd", "programs"), os.path.join(_TESTS, "programs")]
+
+
+def program_path(name):
+    """Example programs ship in knightking_amd/programs; the reference-test
+    programs (tag walks) live beside the tests."""
+    for d in PROGRAM_DIRS:
+        p = os.path.join(d, name + ".cpp")
+        if os.path.exists(p):
+            return p
+    raise FileNotFoundError(name)
 TAGS = 4
 
 
@@ -29,7 +40,7 @@ def build_dir(tmp_path_factory):
 
 
 def plib(name, build_dir):
-    return O.program_oracle(os.path.join(PROGRAMS, name + ".cpp"), build_dir)
+    return O.program_oracle(program_path(name), build_dir)
 
 
 def same_paths(pa, la, pb, lb):
@@ -199,7 +210,7 @@ def test_return_bias_follows_its_closed_form(build_dir, karate):
 def test_closed_form_catches_a_planted_mistake(build_dir, tmp_path):
     """The tag walk with its vertex term dropped from the dynamic component
     (bounds unchanged) is rejected by the same check."""
-    with open(os.path.join(PROGRAMS, "tag_walk.cpp")) as f:
+    with open(program_path("tag_walk")) as f:
         src = f.read()
     bad = src.replace("temp += (float)vtag(c)[cur] * c.param[1];", "")
     assert bad != src
