@@ -36,6 +36,36 @@ void dfree(T *&p)
     p = nullptr;
 }
 
+// Device array owned by a scope: freed on every exit, exceptions included;
+// release() hands the allocation over (a table is published only once built).
+template <typename T>
+struct DArr {
+    T *p = nullptr;
+    DArr() = default;
+    explicit DArr(size_t n) : p(dalloc<T>(n)) {}
+    ~DArr() { dfree(p); }
+    DArr(const DArr &) = delete;
+    DArr &operator=(const DArr &) = delete;
+    T *release()
+    {
+        T *q = p;
+        p = nullptr;
+        return q;
+    }
+};
+
+// A graph builder that fails part-way leaves no half-built graph behind.
+template <typename F>
+void building(GraphStore &gs, F &&f)
+{
+    try {
+        f();
+    } catch (...) {
+        gs.release();
+        throw;
+    }
+}
+
 struct Temp {
     void *p = nullptr;
     size_t n = 0;
@@ -663,19 +693,17 @@ unsigned long long read_counter(unsigned long long *d, hipStream_t s)
 
 void check_graph(GraphStore &gs, hipStream_t s)
 {
-    unsigned long long *bad = dalloc<unsigned long long>(1);
-    unsigned int *sorted = dalloc<unsigned int>(1);
-    KKW_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned long long), s));
+    DArr<unsigned long long> bad(1);
+    DArr<unsigned int> sorted(1);
+    KKW_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned long long), s));
     unsigned int one = 1;
-    KKW_HIP(hipMemcpyAsync(sorted, &one, sizeof(one), hipMemcpyHostToDevice, s));
+    KKW_HIP(hipMemcpyAsync(sorted.p, &one, sizeof(one), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_check_csr, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.col,
-                       gs.w, gs.V, gs.E, bad, sorted);
+                       gs.w, gs.V, gs.E, bad.p, sorted.p);
     KKW_HIP(hipGetLastError());
-    unsigned long long nbad = read_counter(bad, s);
+    unsigned long long nbad = read_counter(bad.p, s);
     unsigned int hs = 0;
-    KKW_HIP(hipMemcpy(&hs, sorted, sizeof(hs), hipMemcpyDeviceToHost));
-    dfree(bad);
-    dfree(sorted);
+    KKW_HIP(hipMemcpy(&hs, sorted.p, sizeof(hs), hipMemcpyDeviceToHost));
     if (nbad) throw Error(-1, "graph has " + std::to_string(nbad) +
                                   " invalid entries (vertex id >= v_num, bad row offsets, or a "
                                   "weight that is negative or not finite)");
@@ -687,24 +715,23 @@ void compute_packable(GraphStore &gs, hipStream_t s)
 {
     gs.packable = false;
     if (gs.V == 0 || gs.E >= (1ull << ROW_DEG_SHIFT)) return;
-    unsigned long long *mx = dalloc<unsigned long long>(1);
-    KKW_HIP(hipMemsetAsync(mx, 0, sizeof(unsigned long long), s));
-    hipLaunchKernelGGL(k_max_degree, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V, mx);
+    DArr<unsigned long long> mx(1);
+    KKW_HIP(hipMemsetAsync(mx.p, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_max_degree, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V, mx.p);
     KKW_HIP(hipGetLastError());
-    gs.packable = read_counter(mx, s) < (1ull << (64 - ROW_DEG_SHIFT));
-    dfree(mx);
+    gs.packable = read_counter(mx.p, s) < (1ull << (64 - ROW_DEG_SHIFT));
 }
 
 void detect_symmetric(GraphStore &gs, hipStream_t s)
 {
     gs.symmetric = false;
     if (!gs.sorted || gs.V == 0) return;
-    unsigned long long *miss = dalloc<unsigned long long>(1);
-    KKW_HIP(hipMemsetAsync(miss, 0, sizeof(unsigned long long), s));
-    hipLaunchKernelGGL(k_check_symmetric, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V, miss);
+    DArr<unsigned long long> miss(1);
+    KKW_HIP(hipMemsetAsync(miss.p, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_check_symmetric, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V,
+                       miss.p);
     KKW_HIP(hipGetLastError());
-    gs.symmetric = read_counter(miss, s) == 0;
-    dfree(miss);
+    gs.symmetric = read_counter(miss.p, s) == 0;
 }
 
 }  // namespace
@@ -755,55 +782,60 @@ void build_rmat(GraphStore &gs, int scale, int edge_factor, uint32_t thr_a, uint
     if (scale < 1 || scale > 30 || edge_factor < 1)
         throw Error(-1, "rmat: scale must be in [1,30] and edge_factor >= 1");
     gs.release();
-    const uint32_t V = 1u << scale;
-    const uint64_t n = (uint64_t)edge_factor << scale;
-    const int kb = 2 * scale;
-    uint64_t *k0 = dalloc<uint64_t>(n), *k1 = dalloc<uint64_t>(n);
-    hipLaunchKernelGGL(k_rmat_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, n, scale, thr_a, thr_ab,
-                       thr_abc, seed, k0);
-    KKW_HIP(hipGetLastError());
-    Temp tmp;
-    size_t tb = 0;
-    KKW_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, k0, k1, n, 0, kb, s));
-    tmp.ensure(tb);
-    KKW_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, k0, k1, n, 0, kb, s));
-    uint8_t *flags = dalloc<uint8_t>(n);
-    hipLaunchKernelGGL(k_unique_flags, dim3(grid_for(n, 256)), dim3(256), 0, s, k1, n, scale, flags);
-    KKW_HIP(hipGetLastError());
-    unsigned long long *nsel = dalloc<unsigned long long>(1);
-    tb = 0;
-    KKW_HIP(hipcub::DeviceSelect::Flagged(nullptr, tb, k1, flags, k0, nsel, (int64_t)n, s));
-    tmp.ensure(tb);
-    KKW_HIP(hipcub::DeviceSelect::Flagged(tmp.p, tb, k1, flags, k0, nsel, (int64_t)n, s));
-    const uint64_t u = read_counter(nsel, s);
-    dfree(nsel);
-    dfree(flags);
-    dfree(k1);
-    const uint64_t m = 2 * u;
-    uint64_t *m0 = dalloc<uint64_t>(m), *m1 = dalloc<uint64_t>(m);
-    hipLaunchKernelGGL(k_mirror, dim3(grid_for(u, 256)), dim3(256), 0, s, k0, u, scale, m0);
-    KKW_HIP(hipGetLastError());
-    dfree(k0);
-    tb = 0;
-    KKW_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, m0, m1, m, 0, kb, s));
-    tmp.ensure(tb);
-    KKW_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, m0, m1, m, 0, kb, s));
-    dfree(m0);
-    gs.V = V;
-    gs.E = m;
-    gs.row_ptr = dalloc<uint64_t>((uint64_t)V + 1);
-    gs.col = dalloc<uint32_t>(m + 4);
-    gs.w = weighted ? dalloc<float>(m) : nullptr;
-    hipLaunchKernelGGL(k_rowptr_from_sorted, dim3(grid_for((uint64_t)V + 1, 256)), dim3(256), 0, s, m1, m,
-                       scale, V, gs.row_ptr);
-    hipLaunchKernelGGL(k_col_weight_from_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, m1, m, scale, seed,
-                       gs.col, gs.w);
-    KKW_HIP(hipGetLastError());
-    KKW_HIP(hipStreamSynchronize(s));
-    dfree(m1);
-    gs.sorted = true;
-    gs.symmetric = true;   // every edge was mirrored
-    compute_packable(gs, s);
+    building(gs, [&] {
+        const uint32_t V = 1u << scale;
+        const uint64_t n = (uint64_t)edge_factor << scale;
+        const int kb = 2 * scale;
+        DArr<uint64_t> k0(n);
+        Temp tmp;
+        uint64_t u = 0;
+        {
+            DArr<uint64_t> k1(n);
+            hipLaunchKernelGGL(k_rmat_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, n, scale, thr_a, thr_ab,
+                               thr_abc, seed, k0.p);
+            KKW_HIP(hipGetLastError());
+            size_t tb = 0;
+            KKW_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, k0.p, k1.p, n, 0, kb, s));
+            tmp.ensure(tb);
+            KKW_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, k0.p, k1.p, n, 0, kb, s));
+            DArr<uint8_t> flags(n);
+            hipLaunchKernelGGL(k_unique_flags, dim3(grid_for(n, 256)), dim3(256), 0, s, k1.p, n, scale, flags.p);
+            KKW_HIP(hipGetLastError());
+            DArr<unsigned long long> nsel(1);
+            tb = 0;
+            KKW_HIP(hipcub::DeviceSelect::Flagged(nullptr, tb, k1.p, flags.p, k0.p, nsel.p, (int64_t)n, s));
+            tmp.ensure(tb);
+            KKW_HIP(hipcub::DeviceSelect::Flagged(tmp.p, tb, k1.p, flags.p, k0.p, nsel.p, (int64_t)n, s));
+            u = read_counter(nsel.p, s);
+        }
+        const uint64_t m = 2 * u;
+        DArr<uint64_t> m1(m);
+        {
+            DArr<uint64_t> m0(m);
+            hipLaunchKernelGGL(k_mirror, dim3(grid_for(u, 256)), dim3(256), 0, s, k0.p, u, scale, m0.p);
+            KKW_HIP(hipGetLastError());
+            size_t tb = 0;
+            KKW_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, m0.p, m1.p, m, 0, kb, s));
+            tmp.ensure(tb);
+            KKW_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, m0.p, m1.p, m, 0, kb, s));
+            KKW_HIP(hipStreamSynchronize(s));
+        }
+        dfree(k0.p);
+        gs.V = V;
+        gs.E = m;
+        gs.row_ptr = dalloc<uint64_t>((uint64_t)V + 1);
+        gs.col = dalloc<uint32_t>(m + 4);
+        gs.w = weighted ? dalloc<float>(m) : nullptr;
+        hipLaunchKernelGGL(k_rowptr_from_sorted, dim3(grid_for((uint64_t)V + 1, 256)), dim3(256), 0, s, m1.p, m,
+                           scale, V, gs.row_ptr);
+        hipLaunchKernelGGL(k_col_weight_from_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, m1.p, m, scale, seed,
+                           gs.col, gs.w);
+        KKW_HIP(hipGetLastError());
+        KKW_HIP(hipStreamSynchronize(s));
+        gs.sorted = true;
+        gs.symmetric = true;   // every edge was mirrored
+        compute_packable(gs, s);
+    });
 }
 
 void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const uint32_t *h_dst,
@@ -812,67 +844,61 @@ void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const u
 {
     if (V == 0) throw Error(-1, "v_num must be positive");
     gs.release();
-    const uint64_t m = undirected ? 2 * n : n;
-    uint32_t *src = dalloc<uint32_t>(n), *dst = dalloc<uint32_t>(n);
-    float *w = h_w ? dalloc<float>(n) : nullptr;
-    uint8_t *et = h_et ? dalloc<uint8_t>(n) : nullptr;
-    KKW_HIP(hipMemcpyAsync(src, h_src, n * 4, hipMemcpyHostToDevice, s));
-    KKW_HIP(hipMemcpyAsync(dst, h_dst, n * 4, hipMemcpyHostToDevice, s));
-    if (w) KKW_HIP(hipMemcpyAsync(w, h_w, n * 4, hipMemcpyHostToDevice, s));
-    if (et) KKW_HIP(hipMemcpyAsync(et, h_et, n, hipMemcpyHostToDevice, s));
-    const int vb = bits_for(V);
-    uint64_t *k0 = dalloc<uint64_t>(m), *k1 = dalloc<uint64_t>(m);
-    uint64_t *v0 = dalloc<uint64_t>(m), *v1 = dalloc<uint64_t>(m);
-    unsigned long long *bad = dalloc<unsigned long long>(1);
-    KKW_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned long long), s));
-    if (m) {
-        hipLaunchKernelGGL(k_entry_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, src, dst, n,
-                           undirected ? 1 : 0, sort_adj ? 1 : 0, vb, V, k0, v0, bad);
+    building(gs, [&] {
+        const uint64_t m = undirected ? 2 * n : n;
+        DArr<uint32_t> src(n), dst(n);
+        DArr<float> w;
+        DArr<uint8_t> et;
+        if (h_w) w.p = dalloc<float>(n);
+        if (h_et) et.p = dalloc<uint8_t>(n);
+        KKW_HIP(hipMemcpyAsync(src.p, h_src, n * 4, hipMemcpyHostToDevice, s));
+        KKW_HIP(hipMemcpyAsync(dst.p, h_dst, n * 4, hipMemcpyHostToDevice, s));
+        if (w.p) KKW_HIP(hipMemcpyAsync(w.p, h_w, n * 4, hipMemcpyHostToDevice, s));
+        if (et.p) KKW_HIP(hipMemcpyAsync(et.p, h_et, n, hipMemcpyHostToDevice, s));
+        const int vb = bits_for(V);
+        DArr<uint64_t> v1(m);
+        {
+            DArr<uint64_t> k0(m), k1(m), v0(m);
+            DArr<unsigned long long> bad(1);
+            KKW_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned long long), s));
+            if (m) {
+                hipLaunchKernelGGL(k_entry_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, src.p, dst.p, n,
+                                   undirected ? 1 : 0, sort_adj ? 1 : 0, vb, V, k0.p, v0.p, bad.p);
+                KKW_HIP(hipGetLastError());
+            }
+            if (read_counter(bad.p, s)) throw Error(-1, "edge list holds a vertex id >= v_num");
+            Temp tmp;
+            if (m) {
+                size_t tb = 0;
+                const int eb = sort_adj ? 2 * vb : vb;
+                KKW_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0.p, k1.p, v0.p, v1.p, m, 0, eb, s));
+                tmp.ensure(tb);
+                KKW_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0.p, k1.p, v0.p, v1.p, m, 0, eb, s));
+            }
+            KKW_HIP(hipStreamSynchronize(s));
+        }
+        gs.V = V;
+        gs.E = m;
+        gs.row_ptr = dalloc<uint64_t>((uint64_t)V + 1);
+        gs.col = dalloc<uint32_t>(m + 4);
+        gs.w = w.p ? dalloc<float>(m) : nullptr;
+        gs.etype = et.p ? dalloc<uint8_t>(m) : nullptr;
+        DArr<uint32_t> srcs(m);
+        if (m) {
+            hipLaunchKernelGGL(k_gather_entries, dim3(grid_for(m, 256)), dim3(256), 0, s, v1.p, m, src.p, dst.p,
+                               w.p, et.p, undirected ? 1 : 0, gs.col, gs.w, gs.etype, srcs.p);
+            KKW_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_rowptr_from_src, dim3(grid_for((uint64_t)V + 1, 256)), dim3(256), 0, s, srcs.p, m,
+                           V, gs.row_ptr);
         KKW_HIP(hipGetLastError());
-    }
-    if (read_counter(bad, s)) {
-        dfree(bad); dfree(k0); dfree(k1); dfree(v0); dfree(v1); dfree(src); dfree(dst); dfree(w); dfree(et);
-        throw Error(-1, "edge list holds a vertex id >= v_num");
-    }
-    dfree(bad);
-    Temp tmp;
-    if (m) {
-        size_t tb = 0;
-        const int eb = sort_adj ? 2 * vb : vb;
-        KKW_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, m, 0, eb, s));
-        tmp.ensure(tb);
-        KKW_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0, k1, v0, v1, m, 0, eb, s));
-    }
-    dfree(k0);
-    dfree(k1);
-    dfree(v0);
-    gs.V = V;
-    gs.E = m;
-    gs.row_ptr = dalloc<uint64_t>((uint64_t)V + 1);
-    gs.col = dalloc<uint32_t>(m + 4);
-    gs.w = w ? dalloc<float>(m) : nullptr;
-    gs.etype = et ? dalloc<uint8_t>(m) : nullptr;
-    uint32_t *srcs = dalloc<uint32_t>(m);
-    if (m) {
-        hipLaunchKernelGGL(k_gather_entries, dim3(grid_for(m, 256)), dim3(256), 0, s, v1, m, src, dst, w, et,
-                           undirected ? 1 : 0, gs.col, gs.w, gs.etype, srcs);
-        KKW_HIP(hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_rowptr_from_src, dim3(grid_for((uint64_t)V + 1, 256)), dim3(256), 0, s, srcs, m,
-                       V, gs.row_ptr);
-    KKW_HIP(hipGetLastError());
-    KKW_HIP(hipStreamSynchronize(s));
-    dfree(srcs);
-    dfree(v1);
-    dfree(src);
-    dfree(dst);
-    dfree(w);
-    dfree(et);
-    check_graph(gs, s);
-    if (undirected) gs.symmetric = true;   // every edge was mirrored
-    else detect_symmetric(gs, s);
-    compute_packable(gs, s);
-    if (gs.etype) rebuild_vmask(gs, s);
+        KKW_HIP(hipStreamSynchronize(s));
+        check_graph(gs, s);
+        if (undirected) gs.symmetric = true;   // every edge was mirrored
+        else detect_symmetric(gs, s);
+        compute_packable(gs, s);
+        if (gs.etype) rebuild_vmask(gs, s);
+    });
 }
 
 void build_from_csr(GraphStore &gs, uint32_t V, const uint64_t *h_row_ptr, const uint32_t *h_col,
@@ -881,67 +907,82 @@ void build_from_csr(GraphStore &gs, uint32_t V, const uint64_t *h_row_ptr, const
     if (V == 0) throw Error(-1, "v_num must be positive");
     if (h_row_ptr[0] != 0) throw Error(-1, "row_ptr[0] must be 0");
     gs.release();
-    const uint64_t E = h_row_ptr[V];
-    gs.V = V;
-    gs.E = E;
-    gs.row_ptr = dalloc<uint64_t>((uint64_t)V + 1);
-    gs.col = dalloc<uint32_t>(E + 4);
-    gs.w = h_w ? dalloc<float>(E) : nullptr;
-    KKW_HIP(hipMemcpyAsync(gs.row_ptr, h_row_ptr, ((uint64_t)V + 1) * 8, hipMemcpyHostToDevice, s));
-    if (E) KKW_HIP(hipMemcpyAsync(gs.col, h_col, E * 4, hipMemcpyHostToDevice, s));
-    if (gs.w && E) KKW_HIP(hipMemcpyAsync(gs.w, h_w, E * 4, hipMemcpyHostToDevice, s));
-    KKW_HIP(hipStreamSynchronize(s));
-    check_graph(gs, s);
-    detect_symmetric(gs, s);
-    compute_packable(gs, s);
+    building(gs, [&] {
+        const uint64_t E = h_row_ptr[V];
+        gs.V = V;
+        gs.E = E;
+        gs.row_ptr = dalloc<uint64_t>((uint64_t)V + 1);
+        gs.col = dalloc<uint32_t>(E + 4);
+        gs.w = h_w ? dalloc<float>(E) : nullptr;
+        KKW_HIP(hipMemcpyAsync(gs.row_ptr, h_row_ptr, ((uint64_t)V + 1) * 8, hipMemcpyHostToDevice, s));
+        if (E) KKW_HIP(hipMemcpyAsync(gs.col, h_col, E * 4, hipMemcpyHostToDevice, s));
+        if (gs.w && E) KKW_HIP(hipMemcpyAsync(gs.w, h_w, E * 4, hipMemcpyHostToDevice, s));
+        KKW_HIP(hipStreamSynchronize(s));
+        check_graph(gs, s);
+        detect_symmetric(gs, s);
+        compute_packable(gs, s);
+    });
 }
 
 static void rebuild_vmask(GraphStore &gs, hipStream_t s)
 {
     dfree(gs.vmask);
-    gs.vmask = dalloc<uint8_t>(gs.V);
-    unsigned long long *bad = dalloc<unsigned long long>(1);
-    KKW_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned long long), s));
+    DArr<uint8_t> vm(gs.V);
+    DArr<unsigned long long> bad(1);
+    KKW_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned long long), s));
     hipLaunchKernelGGL(k_vmask, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.etype, gs.V,
-                       gs.vmask, bad);
+                       vm.p, bad.p);
     KKW_HIP(hipGetLastError());
-    unsigned long long nb = read_counter(bad, s);
-    dfree(bad);
-    if (nb) throw Error(-1, "edge types must be < 8 (one bit each in a u8 mask, as metapath.hpp:30)");
+    if (read_counter(bad.p, s))
+        throw Error(-1, "edge types must be < 8 (one bit each in a u8 mask, as metapath.hpp:30)");
+    gs.vmask = vm.release();
+}
+
+// tables whose entries carry edge types (bucket meta) or depend on the list
+// order: rebuilt on next use
+static void drop_derived(GraphStore &gs)
+{
+    dfree(gs.bucket);
+    dfree(gs.ubucket);
+    dfree(gs.lbucket);
+    dfree(gs.vsum);
 }
 
 void set_edge_types(GraphStore &gs, const uint8_t *h_etype, hipStream_t s)
 {
     if (!gs.row_ptr) throw Error(-2, "no graph loaded");
-    dfree(gs.etype);
-    gs.etype = dalloc<uint8_t>(gs.E);
-    if (gs.E) KKW_HIP(hipMemcpyAsync(gs.etype, h_etype, gs.E, hipMemcpyHostToDevice, s));
-    rebuild_vmask(gs, s);
-    dfree(gs.bucket);  // bucket meta carries types: rebuild on next prepare
-    dfree(gs.ubucket);
-    dfree(gs.lbucket);
-    dfree(gs.vsum);
+    DArr<uint8_t> et(gs.E);
+    if (gs.E) KKW_HIP(hipMemcpyAsync(et.p, h_etype, gs.E, hipMemcpyHostToDevice, s));
+    KKW_HIP(hipStreamSynchronize(s));
+    uint8_t *old = gs.etype;
+    gs.etype = et.p;
+    try {
+        rebuild_vmask(gs, s);
+    } catch (...) {
+        gs.etype = old;   // keep the previous types (and masks) on bad input
+        if (old) rebuild_vmask(gs, s);
+        throw;
+    }
+    et.release();
+    dfree(old);
+    drop_derived(gs);
 }
 
 void type_by_dst(GraphStore &gs, int ntypes, uint64_t seed, hipStream_t s)
 {
     if (!gs.row_ptr) throw Error(-2, "no graph loaded");
     if (ntypes < 1 || ntypes > 8) throw Error(-1, "ntypes must be in [1,8]");
-    uint8_t *vt = dalloc<uint8_t>(gs.V);
-    hipLaunchKernelGGL(k_vertex_types, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.V, ntypes, seed, vt);
-    dfree(gs.etype);
-    gs.etype = dalloc<uint8_t>(gs.E);
+    DArr<uint8_t> vt(gs.V), et(gs.E);
+    hipLaunchKernelGGL(k_vertex_types, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.V, ntypes, seed, vt.p);
     if (gs.E)
-        hipLaunchKernelGGL(k_etype_by_dst, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.col, gs.E, vt,
-                           gs.etype);
+        hipLaunchKernelGGL(k_etype_by_dst, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.col, gs.E, vt.p,
+                           et.p);
     KKW_HIP(hipGetLastError());
     KKW_HIP(hipStreamSynchronize(s));
-    dfree(vt);
-    rebuild_vmask(gs, s);
-    dfree(gs.bucket);
-    dfree(gs.ubucket);
-    dfree(gs.lbucket);
-    dfree(gs.vsum);
+    dfree(gs.etype);
+    gs.etype = et.release();
+    rebuild_vmask(gs, s);   // types < 8 by construction
+    drop_derived(gs);
 }
 
 // Per-list stable sort by neighbour id, in place (node2vec.hpp:122-125 sorts
@@ -956,33 +997,42 @@ void sort_adjacency(GraphStore &gs, hipStream_t s)
         return;
     }
     const int vb = bits_for(gs.V);
-    uint32_t *srcs = dalloc<uint32_t>(E);
-    uint64_t *k0 = dalloc<uint64_t>(E), *k1 = dalloc<uint64_t>(E);
-    uint64_t *v0 = dalloc<uint64_t>(E), *v1 = dalloc<uint64_t>(E);
-    hipLaunchKernelGGL(k_expand_src, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V, srcs);
-    hipLaunchKernelGGL(k_sort_keys_from_csr, dim3(grid_for(E, 256)), dim3(256), 0, s, srcs, gs.col, E, vb, k0, v0);
-    KKW_HIP(hipGetLastError());
-    Temp tmp;
-    size_t tb = 0;
-    KKW_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, E, 0, 2 * vb, s));
-    tmp.ensure(tb);
-    KKW_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0, k1, v0, v1, E, 0, 2 * vb, s));
-    uint32_t *col2 = dalloc<uint32_t>(E + 4);
-    float *w2 = gs.w ? dalloc<float>(E) : nullptr;
-    uint8_t *et2 = gs.etype ? dalloc<uint8_t>(E) : nullptr;
-    hipLaunchKernelGGL(k_permute_csr, dim3(grid_for(E, 256)), dim3(256), 0, s, v1, E, gs.col, gs.w, gs.etype,
-                       col2, w2, et2);
+    DArr<uint64_t> v1(E);
+    {
+        DArr<uint32_t> srcs(E);
+        DArr<uint64_t> k0(E), k1(E), v0(E);
+        hipLaunchKernelGGL(k_expand_src, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V, srcs.p);
+        hipLaunchKernelGGL(k_sort_keys_from_csr, dim3(grid_for(E, 256)), dim3(256), 0, s, srcs.p, gs.col, E, vb,
+                           k0.p, v0.p);
+        KKW_HIP(hipGetLastError());
+        Temp tmp;
+        size_t tb = 0;
+        KKW_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0.p, k1.p, v0.p, v1.p, E, 0, 2 * vb, s));
+        tmp.ensure(tb);
+        KKW_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0.p, k1.p, v0.p, v1.p, E, 0, 2 * vb, s));
+        KKW_HIP(hipStreamSynchronize(s));
+    }
+    DArr<uint32_t> col2(E + 4);
+    DArr<float> w2;
+    DArr<uint8_t> et2;
+    if (gs.w) w2.p = dalloc<float>(E);
+    if (gs.etype) et2.p = dalloc<uint8_t>(E);
+    hipLaunchKernelGGL(k_permute_csr, dim3(grid_for(E, 256)), dim3(256), 0, s, v1.p, E, gs.col, gs.w, gs.etype,
+                       col2.p, w2.p, et2.p);
     KKW_HIP(hipGetLastError());
     KKW_HIP(hipStreamSynchronize(s));
-    dfree(srcs); dfree(k0); dfree(k1); dfree(v0); dfree(v1);
+    // the permuted lists replace the old ones only once complete
     dfree(gs.col);
-    gs.col = col2;
-    if (w2) { dfree(gs.w); gs.w = w2; }
-    if (et2) { dfree(gs.etype); gs.etype = et2; }
-    dfree(gs.bucket);
-    dfree(gs.ubucket);
-    dfree(gs.lbucket);
-    dfree(gs.vsum);
+    gs.col = col2.release();
+    if (w2.p) {
+        dfree(gs.w);
+        gs.w = w2.release();
+    }
+    if (et2.p) {
+        dfree(gs.etype);
+        gs.etype = et2.release();
+    }
+    drop_derived(gs);   // the hash sets hold sets: sorting leaves them valid
     gs.sorted = true;
     detect_symmetric(gs, s);
 }
@@ -992,25 +1042,27 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
     if (!gs.row_ptr) throw Error(-2, "no graph loaded");
     if (gs.adjset) return;
     const uint64_t n = 2 * gs.E + 4 * (uint64_t)gs.V + 4;
-    gs.adjset = dalloc<uint32_t>(n);
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, gs.adjset, ADJ_EMPTY, n);
+    DArr<uint32_t> t(n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, t.p, ADJ_EMPTY, n);
     if (gs.E)
         hipLaunchKernelGGL(k_adjset_insert, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V,
-                           gs.E, gs.adjset);
+                           gs.E, t.p);
     KKW_HIP(hipGetLastError());
     KKW_HIP(hipStreamSynchronize(s));
+    gs.adjset = t.release();
 }
 
 void build_uniform_buckets(GraphStore &gs, hipStream_t s)
 {
     if (!gs.row_ptr) throw Error(-2, "no graph loaded");
     if (gs.ubucket) return;
-    gs.ubucket = dalloc<Bucket>(gs.E);
+    DArr<Bucket> t(gs.E);
     if (gs.E)
         hipLaunchKernelGGL(k_uniform_buckets, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.col, gs.etype, gs.row_ptr,
-                           gs.E, gs.packable ? 1 : 0, gs.ubucket);
+                           gs.E, gs.packable ? 1 : 0, t.p);
     KKW_HIP(hipGetLastError());
     KKW_HIP(hipStreamSynchronize(s));
+    gs.ubucket = t.release();
 }
 
 void build_alias(GraphStore &gs, hipStream_t s, bool local)
@@ -1019,31 +1071,34 @@ void build_alias(GraphStore &gs, hipStream_t s, bool local)
     if (!gs.w) throw Error(-2, "graph has no weights: alias tables need a weighted graph");
     Bucket *&target = local ? gs.lbucket : gs.bucket;
     dfree(target);
-    target = dalloc<Bucket>(gs.E);
+    DArr<Bucket> t(gs.E);
     // both tables give the same sums; keep the array (and its address) if present
-    if (!gs.vsum) gs.vsum = dalloc<float>(gs.V);
-    uint2 *qs = dalloc<uint2>(gs.E), *ql = dalloc<uint2>(gs.E);
-    // lists of AW_MIN or more entries: one wave each; the rest: one lane each
-    uint32_t *big = dalloc<uint32_t>(gs.V);
-    unsigned int *n_big_d = dalloc<unsigned int>(1);
-    KKW_HIP(hipMemsetAsync(n_big_d, 0, sizeof(unsigned int), s));
-    hipLaunchKernelGGL(k_list_big, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V, big, n_big_d);
-    unsigned int n_big = 0;
-    KKW_HIP(hipMemcpyAsync(&n_big, n_big_d, sizeof(n_big), hipMemcpyDeviceToHost, s));
-    KKW_HIP(hipStreamSynchronize(s));
-    if (n_big)
-        hipLaunchKernelGGL(k_alias_wave, dim3(grid_for(n_big, AW_WAVES)), dim3(64 * AW_WAVES), 0, s, big, n_big,
-                           gs.row_ptr, gs.col, gs.w, gs.etype, target, gs.vsum, qs, ql, !local);
-    hipLaunchKernelGGL(k_alias, dim3(grid_for(gs.V, 64)), dim3(64), 0, s, gs.row_ptr, gs.col, gs.w,
-                       gs.etype, gs.V, target, gs.vsum, qs, ql, AW_MIN, !local);
-    if (!local && gs.packable && gs.E)
-        hipLaunchKernelGGL(k_bucket_rows, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.bucket, gs.E, gs.row_ptr);
-    KKW_HIP(hipGetLastError());
-    KKW_HIP(hipStreamSynchronize(s));
-    dfree(qs);
-    dfree(ql);
-    dfree(big);
-    dfree(n_big_d);
+    DArr<float> vs;
+    float *vsum = gs.vsum;
+    if (!vsum) vsum = vs.p = dalloc<float>(gs.V);
+    {
+        DArr<uint2> qs(gs.E), ql(gs.E);
+        // lists of AW_MIN or more entries: one wave each; the rest: one lane each
+        DArr<uint32_t> big(gs.V);
+        DArr<unsigned int> n_big_d(1);
+        KKW_HIP(hipMemsetAsync(n_big_d.p, 0, sizeof(unsigned int), s));
+        hipLaunchKernelGGL(k_list_big, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V, big.p,
+                           n_big_d.p);
+        unsigned int n_big = 0;
+        KKW_HIP(hipMemcpyAsync(&n_big, n_big_d.p, sizeof(n_big), hipMemcpyDeviceToHost, s));
+        KKW_HIP(hipStreamSynchronize(s));
+        if (n_big)
+            hipLaunchKernelGGL(k_alias_wave, dim3(grid_for(n_big, AW_WAVES)), dim3(64 * AW_WAVES), 0, s, big.p,
+                               n_big, gs.row_ptr, gs.col, gs.w, gs.etype, t.p, vsum, qs.p, ql.p, !local);
+        hipLaunchKernelGGL(k_alias, dim3(grid_for(gs.V, 64)), dim3(64), 0, s, gs.row_ptr, gs.col, gs.w,
+                           gs.etype, gs.V, t.p, vsum, qs.p, ql.p, AW_MIN, !local);
+        if (!local && gs.packable && gs.E)
+            hipLaunchKernelGGL(k_bucket_rows, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, t.p, gs.E, gs.row_ptr);
+        KKW_HIP(hipGetLastError());
+        KKW_HIP(hipStreamSynchronize(s));
+    }
+    target = t.release();
+    if (vs.p) gs.vsum = vs.release();
 }
 
 }  // namespace kkw

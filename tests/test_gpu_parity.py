@@ -94,6 +94,35 @@ def test_edges_csr_rejects_bad_vertex(engine):
         engine.graph_from_edges(4, np.array([0, 1], np.uint32), np.array([1, 9], np.uint32))
 
 
+def test_failed_build_leaves_no_graph(engine):
+    """A build that fails part-way (a bad vertex id found on the device)
+    leaves no half-built graph: the engine reports that none is loaded."""
+    engine.graph_rmat(8, 8, seed=1, weighted=True)
+    with pytest.raises(kk.KkwError):
+        engine.graph_from_edges(4, np.array([0, 1], np.uint32), np.array([1, 9], np.uint32))
+    with pytest.raises(kk.KkwError) as e:
+        engine.info()
+    assert e.value.code == kk.KKW_E_NOGRAPH
+
+
+def test_bad_edge_types_keep_the_previous_ones(engine):
+    """Edge types of 8 or more are refused (one bit each in a u8 mask); the
+    previous types stay, and a metapath walk gives the same paths as before."""
+    load_rmat(engine, 10)
+    engine.type_by_dst(5, 3)
+    job = kk.metapath_job(2000, 12, [[0b00111, 0b11000], [0b11111]], True, seed=4)
+    (before,) = engine.run([job])
+    V, E = engine.info()
+    bad = np.full(E, 9, np.uint8)
+    with pytest.raises(kk.KkwError) as e:
+        engine.set_edge_types(bad)
+    assert e.value.code == kk.KKW_E_INVALID
+    (after,) = engine.run([job])
+    assert np.array_equal(before.lengths, after.lengths)
+    mask = np.arange(before.vertices.shape[1])[None, :] < before.lengths[:, None]
+    assert np.all((before.vertices == after.vertices) | ~mask)
+
+
 @pytest.mark.parametrize("scale", [10, 14])
 def test_alias_matches_oracle(engine, scale):
     load_rmat(engine, scale)
