@@ -291,16 +291,21 @@ def serve_latency(eng, rp, rates, seconds, seed=7):
             steps = [0]
             done = threading.Event()
 
+            failure = []
+
             def collect():
                 i = 0
-                while i < n_q:
-                    if i >= len(tickets):
-                        time.sleep(0.0002)
-                        continue
-                    paths, ms = eng.wait(tickets[i])
-                    lat.append(ms)
-                    steps[0] += paths.steps
-                    i += 1
+                try:
+                    while i < n_q:
+                        if i >= len(tickets):
+                            time.sleep(0.0002)
+                            continue
+                        paths, ms = eng.wait(tickets[i], 30.0, packed=True)
+                        lat.append(ms)
+                        steps[0] += paths.steps
+                        i += 1
+                except Exception as e:   # a stalled or failed query ends this rate's measurement
+                    failure.append(repr(e))
                 done.set()
 
             th = threading.Thread(target=collect)
@@ -317,6 +322,10 @@ def serve_latency(eng, rp, rates, seconds, seed=7):
             done.wait()
             th.join()
             span = time.perf_counter() - t_start
+            if failure:
+                out[str(rate)] = {"queries": n_q, "completed": len(lat), "error": failure[0]}
+                log(f"serving {rate} qps: {out[str(rate)]}")
+                break
             lat = np.array(lat)
             out[str(rate)] = {"queries": n_q, "p50_ms": round(float(np.percentile(lat, 50)), 3),
                               "p99_ms": round(float(np.percentile(lat, 99)), 3),

@@ -88,7 +88,10 @@ typedef struct kkw_job {
  * walker-id order.  Path i holds (lengths[i] & 0x7fffffff) vertices (steps + 1)
  * of walker walker_begin + i; bit 31 of lengths[i] flags a path stopped by the
  * PPR cap.  Rows start at vertices + i * stride, or, when offsets is not NULL
- * (packed form, used by the serving loop), at vertices + offsets[i]. */
+ * (packed form, used by the serving loop), at vertices + offsets[i]; packed
+ * rows are consecutive but offsets[0] need not be 0.  A served result points
+ * into the serving loop's pinned staging memory until kkw_paths_free returns
+ * it (results held back for long keep that memory from new queries). */
 typedef struct kkw_paths {
     uint32_t walker_begin;
     uint32_t walker_num;

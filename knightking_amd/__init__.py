@@ -254,17 +254,57 @@ def _take_paths(p: _Paths) -> Paths:
     raw = np.ctypeslib.as_array(p.lengths, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
     lens = raw & 0x7FFFFFFF
     if n and bool(p.offsets):
-        # packed form: expand to rows of `stride` entries (unused entries zero)
+        # packed form: row i at vertices + offsets[i] (offsets need not start
+        # at 0); expand to rows of `stride` entries (unused entries zero)
         offs = np.ctypeslib.as_array(p.offsets, shape=(n + 1,)).astype(np.int64)
-        flat = np.ctypeslib.as_array(p.vertices, shape=(max(int(offs[-1]), 1),))[: int(offs[-1])]
+        lo, hi = int(offs[0]), int(offs[-1])
+        flat = np.ctypeslib.as_array(p.vertices, shape=(max(hi, 1),))[lo:hi]
         verts = np.zeros((n, s), np.uint32)
-        cols = np.arange(int(offs[-1])) - np.repeat(offs[:-1], lens.astype(np.int64))
+        cols = np.arange(hi - lo) - np.repeat(offs[:-1] - lo, lens.astype(np.int64))
         verts[np.repeat(np.arange(n), lens.astype(np.int64)), cols] = flat
     elif n:
         verts = np.ctypeslib.as_array(p.vertices, shape=(n * s,)).reshape(n, s).copy()
     else:
         verts = np.zeros((0, s), np.uint32)
     out = Paths(p.walker_begin, verts, lens, (raw >> 31).astype(bool), int(p.steps))
+    _lib.kkw_paths_free(C.byref(p))
+    return out
+
+
+@dataclass
+class PackedPaths:
+    """A PathSet in packed form: walker i's path is
+    vertices[offsets[i]:offsets[i+1]] (lengths[i] entries)."""
+    walker_begin: int
+    vertices: np.ndarray      # [total] uint32
+    offsets: np.ndarray       # [walker_num + 1] int64, offsets[0] == 0
+    lengths: np.ndarray       # [walker_num] uint32
+    truncated: np.ndarray     # [walker_num] bool
+    steps: int
+
+    def path(self, i: int) -> np.ndarray:
+        return self.vertices[self.offsets[i]:self.offsets[i + 1]]
+
+
+def _take_packed(p: _Paths) -> "PackedPaths":
+    n = p.walker_num
+    raw = np.ctypeslib.as_array(p.lengths, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
+    lens = raw & 0x7FFFFFFF
+    if n and bool(p.offsets):
+        offs = np.ctypeslib.as_array(p.offsets, shape=(n + 1,)).astype(np.int64)
+        lo, hi = int(offs[0]), int(offs[-1])
+        verts = np.ctypeslib.as_array(p.vertices, shape=(max(hi, 1),))[lo:hi].copy()
+        offs = offs - lo
+    elif n:   # row-major result: gather each row's entries
+        s = p.stride
+        rows = np.ctypeslib.as_array(p.vertices, shape=(n * s,)).reshape(n, s)
+        offs = np.zeros(n + 1, np.int64)
+        offs[1:] = np.cumsum(lens.astype(np.int64))
+        verts = rows[np.arange(s)[None, :] < lens[:, None]].copy()
+    else:
+        offs = np.zeros(1, np.int64)
+        verts = np.zeros(0, np.uint32)
+    out = PackedPaths(p.walker_begin, verts, offs, lens, (raw >> 31).astype(bool), int(p.steps))
     _lib.kkw_paths_free(C.byref(p))
     return out
 
@@ -533,11 +573,14 @@ class Engine:
         _check(_lib.kkw_poll(self.h, ticket, C.byref(d)))
         return bool(d.value)
 
-    def wait(self, ticket: int, timeout_s: float = -1.0):
+    def wait(self, ticket: int, timeout_s: float = -1.0, packed: bool = False):
+        """Result of a served query and its latency (ms, submit to completion).
+        packed=True keeps the packed form (PackedPaths: one flat vertex array
+        and offsets) instead of expanding rows to `stride` entries."""
         p = _Paths()
         lat = C.c_double()
         _check(_lib.kkw_wait(self.h, ticket, timeout_s, C.byref(p), C.byref(lat)))
-        return _take_paths(p), lat.value
+        return (_take_packed(p) if packed else _take_paths(p)), lat.value
 
     def cancel(self, ticket: int):
         _check(_lib.kkw_cancel(self.h, ticket))

@@ -455,6 +455,55 @@ struct SubAlloc {
     }
 };
 
+// Pinned staging memory shared by the serving loop and the results it hands
+// out: a served result's paths stay in the flight's pinned block (no copy),
+// and the block returns to the arena when the last result using it is freed
+// -- after kkw_serve_stop and kkw_engine_destroy too.
+struct PinArena {
+    std::mutex mu;
+    SubAlloc a;
+    char *base = nullptr;       // host address
+    char *dbase = nullptr;      // the same memory as the device addresses it
+    ~PinArena()
+    {
+        if (base) (void)hipHostFree(base);
+    }
+    bool alloc(size_t n, size_t &off)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        return a.alloc(n, off);
+    }
+    void release(size_t off, size_t n)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        a.release(off, n);
+    }
+    size_t largest()
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        return a.largest();
+    }
+    bool in_use_by_results()
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        return a.largest() < a.cap;
+    }
+};
+
+struct PinLease {
+    std::shared_ptr<PinArena> arena;
+    size_t off = 0, bytes = 0;
+    PinLease(std::shared_ptr<PinArena> a, size_t o, size_t b) : arena(std::move(a)), off(o), bytes(b) {}
+    ~PinLease() { arena->release(off, bytes); }
+};
+
+// kkw_paths.priv of a served result
+struct ServedPaths {
+    static constexpr uint64_t kMagic = 0x6b6b7753657276ull;
+    uint64_t magic = kMagic;
+    std::shared_ptr<PinLease> lease;
+};
+
 struct Request {
     uint64_t ticket = 0;
     PreparedJob pj;
@@ -468,19 +517,17 @@ struct Request {
 // and inside the pinned staging arena.
 struct Flight {
     std::vector<std::shared_ptr<Request>> reqs;
+    double t_admit = 0, t_enqueued = 0, t_phase1 = 0, t_phase2 = 0, t_retired = 0;   // flight trace
     int sidx = 0;
-    int phase = 0;                 // 1: walk + pack + D2H(lens, offs); 2: D2H(payload)
-    hipEvent_t ev = nullptr;
+    hipEvent_t ev = nullptr;       // recorded after walk, pack (into pinned memory) and D2H(lens, offs)
     size_t dev_off = 0, dev_bytes = 0;
     size_t pin_off = 0, pin_bytes = 0;
-    size_t pay_off = 0, pay_bytes = 0;
-    bool pay_pinned = false;
     uint64_t walkers = 0;
     uint64_t total = 0;            // packed vertices
     std::vector<uint64_t> row0;    // first flight row of each request
     size_t o_lens = 0, o_offs = 0; // device offsets of lens_all and offs (adjacent)
-    size_t o_packed = 0;
-    std::vector<uint32_t> *h_lens = nullptr;
+    size_t p_down = 0;             // pinned offsets: lens + offs, packed payload
+    size_t p_payload = 0;
 };
 
 struct ServeState {
@@ -495,18 +542,31 @@ struct ServeState {
     std::thread worker;
     hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
     char *dev_base = nullptr;
-    char *pin_base = nullptr;
-    SubAlloc dev, pin;
+    SubAlloc dev;
+    std::shared_ptr<PinArena> pin;
+    // KKW_SERVE_TRACE=<file>: one CSV line per flight, written at kkw_serve_stop
+    // (a measurement aid: where a query's latency goes)
+    std::string trace_path;
+    std::vector<std::string> trace;
 };
 
 static size_t al256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-// Device bytes one request adds to a flight: rows, packed copy, lengths,
-// counts, offsets, starts and scheme tables.
+// Device bytes one request adds to a flight: rows, lengths, counts, offsets,
+// starts and scheme tables.
 static size_t request_dev_bytes(const PreparedJob &pj)
 {
     const size_t n = pj.dj.walker_num;
-    return al256(n * pj.stride * 4) * 2 + al256(n * 4) + al256(n * 8) * 2 + al256(pj.starts.size() * 4) +
+    return al256(n * pj.stride * 4) + al256(n * 4) + al256(n * 8) * 2 + al256(pj.starts.size() * 4) +
+           al256(pj.scheme_len.size() * 4) + al256(pj.scheme_mask.size()) + 1024;
+}
+
+// Pinned bytes one request adds to a flight: its share of the upload mirror,
+// lengths and offsets, and the packed payload at its worst (rows x stride).
+static size_t request_pin_bytes(const PreparedJob &pj)
+{
+    const size_t n = pj.dj.walker_num;
+    return al256(n * pj.stride * 4) + al256(n * 12 + 8) + al256(pj.starts.size() * 4) +
            al256(pj.scheme_len.size() * 4) + al256(pj.scheme_mask.size()) + 1024;
 }
 
@@ -519,9 +579,13 @@ static void finish_request(ServeState *S, const std::shared_ptr<Request> &r, int
 }
 
 // Build and launch one flight.  Returns false (and leaves reqs pending) when
-// the device arena has no room for them yet.
-static bool launch_flight(ServeState *S, Flight &f)
+// the device or pinned arena has no room for them yet; *pin_short tells which.  One stream-ordered
+// sequence: upload the job tables, walk, pack the rows straight into the
+// flight's pinned block, copy lengths and offsets beside them, record the
+// event; the host touches the flight again only when it has completed.
+static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
 {
+    *pin_short = false;
     kkw_engine *eng = S->eng;
     hipStream_t st = S->streams[f.sidx];
     const int nj = (int)f.reqs.size();
@@ -532,7 +596,7 @@ static bool launch_flight(ServeState *S, Flight &f)
     }
     f.walkers = N;
     const size_t scan_bytes = pack_scan_bytes(N ? N : 1);
-    // ---- layout: [upload: jobs | packjobs | per-job small arrays][queue][lens | offs][cnt][scan][rows...][packed]
+    // ---- device: [upload: jobs | packjobs | per-job small arrays][queue][lens | offs][cnt][scan][rows...]
     size_t o = 0;
     const size_t o_jobs = o;
     o += al256(sizeof(DevJob) * nj);
@@ -568,17 +632,20 @@ static bool launch_flight(ServeState *S, Flight &f)
         o += al256((size_t)pj.dj.walker_num * pj.stride * 4);
         packed_max += (size_t)pj.dj.walker_num * pj.stride;
     }
-    f.o_packed = o;
-    o += al256(packed_max * 4);
     f.dev_bytes = o;
-    f.pin_bytes = upload_bytes + al256(down_bytes);
+    // ---- pinned: [upload mirror][lens | offs][packed payload, at most rows x stride]
+    f.p_down = upload_bytes;
+    f.p_payload = upload_bytes + al256(down_bytes);
+    f.pin_bytes = f.p_payload + al256(packed_max * 4);
     if (!S->dev.alloc(f.dev_bytes, f.dev_off)) return false;
-    if (!S->pin.alloc(f.pin_bytes, f.pin_off)) {
+    if (!S->pin->alloc(f.pin_bytes, f.pin_off)) {
         S->dev.release(f.dev_off, f.dev_bytes);
+        *pin_short = true;
         return false;
     }
     char *D = S->dev_base + f.dev_off;
-    char *H = S->pin_base + f.pin_off;
+    char *H = S->pin->base + f.pin_off;
+    char *HD = S->pin->dbase + f.pin_off;   // the same block as the kernels address it
     // ---- fill the upload mirror in pinned memory
     std::vector<DevJob> djs(nj);
     std::vector<PackJob> pjs(nj);
@@ -606,7 +673,7 @@ static bool launch_flight(ServeState *S, Flight &f)
     }
     memcpy(H + o_jobs, djs.data(), sizeof(DevJob) * nj);
     memcpy(H + o_pjobs, pjs.data(), sizeof(PackJob) * nj);
-    // ---- enqueue: upload, walk, pack, download lengths + offsets
+    // ---- enqueue: upload, walk, pack into pinned memory, lengths + offsets
     DevGraph g = eng->gs.view();
     KKW_HIP(hipMemcpyAsync(D, H, upload_bytes, hipMemcpyHostToDevice, st));
     if (chunk_acc) {
@@ -616,55 +683,35 @@ static bool launch_flight(ServeState *S, Flight &f)
         launch_walk_persistent_mixed(g, (const DevJob *)(D + o_jobs), nj, chunk_acc, chunk,
                                      (unsigned long long *)(D + o_queue), false, nullptr, grid, 256, st);
         launch_pack((const PackJob *)(D + o_pjobs), nj, (const uint32_t *)(D + f.o_lens), (uint64_t *)(D + o_cnt),
-                    (uint64_t *)(D + f.o_offs), N, (uint32_t *)(D + f.o_packed), D + o_scan, scan_bytes, st);
+                    (uint64_t *)(D + f.o_offs), N, (uint32_t *)(HD + f.p_payload), D + o_scan, scan_bytes, st);
     } else {
         launch_fill_u64((unsigned long long *)(D + f.o_offs), 0ull, 1, st);
     }
     KKW_HIP(hipGetLastError());
-    KKW_HIP(hipMemcpyAsync(H + upload_bytes, D + f.o_lens, down_bytes, hipMemcpyDeviceToHost, st));
+    KKW_HIP(hipMemcpyAsync(H + f.p_down, D + f.o_lens, down_bytes, hipMemcpyDeviceToHost, st));
     KKW_HIP(hipEventCreateWithFlags(&f.ev, hipEventDisableTiming));
     KKW_HIP(hipEventRecord(f.ev, st));
-    f.phase = 1;
-    f.o_lens = upload_bytes;  // from here on: offsets into the pinned mirror
     return true;
 }
 
-// Phase 1 done: lengths and offsets are on the host; fetch the packed payload.
-static void flight_phase2(ServeState *S, Flight &f)
-{
-    const char *H = S->pin_base + f.pin_off;
-    const uint64_t N = f.walkers;
-    const size_t lens_bytes = ((N * 4 + 7) & ~(size_t)7);
-    const uint64_t *offs = (const uint64_t *)(H + f.o_lens + lens_bytes);
-    f.total = offs[N];
-    f.pay_bytes = f.total * 4;
-    hipStream_t st = S->streams[f.sidx];
-    char *D = S->dev_base + f.dev_off;
-    if (f.pay_bytes && S->pin.alloc(f.pay_bytes, f.pay_off)) {
-        f.pay_pinned = true;
-        KKW_HIP(hipMemcpyAsync(S->pin_base + f.pay_off, D + f.o_packed, f.pay_bytes, hipMemcpyDeviceToHost, st));
-    }
-    KKW_HIP(hipEventRecord(f.ev, st));
-    f.phase = 2;
-}
-
-// Phase 2 done: hand each request its paths.
+// The flight has completed: hand each request its paths, in place in the
+// flight's pinned block (shared by the requests through one lease).
 static void flight_retire(ServeState *S, Flight &f)
 {
-    const char *H = S->pin_base + f.pin_off;
+    const char *H = S->pin->base + f.pin_off;
     const uint64_t N = f.walkers;
     const size_t lens_bytes = ((N * 4 + 7) & ~(size_t)7);
-    const uint32_t *lens = (const uint32_t *)(H + f.o_lens);
-    const uint64_t *offs = (const uint64_t *)(H + f.o_lens + lens_bytes);
-    std::vector<uint32_t> fallback;
-    const uint32_t *payload = nullptr;
-    if (f.pay_pinned) {
-        payload = (const uint32_t *)(S->pin_base + f.pay_off);
-    } else if (f.total) {
-        fallback.resize(f.total);
-        KKW_HIP(hipMemcpy(fallback.data(), S->dev_base + f.dev_off + f.o_packed, f.total * 4, hipMemcpyDeviceToHost));
-        payload = fallback.data();
-    }
+    uint32_t *lens = (uint32_t *)(H + f.p_down);
+    uint64_t *offs = (uint64_t *)(H + f.p_down + lens_bytes);
+    uint32_t *payload = (uint32_t *)(H + f.p_payload);
+    f.total = offs[N];
+    // keep only lengths, offsets and the payload actually written; the upload
+    // mirror and the unused worst-case tail go back to the arena now
+    const size_t used = f.p_payload + al256(f.total * 4);
+    if (f.pin_bytes > used) S->pin->release(f.pin_off + used, f.pin_bytes - used);
+    if (f.p_down) S->pin->release(f.pin_off, f.p_down);
+    auto lease = std::make_shared<PinLease>(S->pin, f.pin_off + f.p_down, used - f.p_down);
+    f.pin_bytes = 0;   // owned by the lease from here on
     for (size_t j = 0; j < f.reqs.size(); j++) {
         auto &r = f.reqs[j];
         const DevJob &d = r->pj.dj;
@@ -674,38 +721,73 @@ static void flight_retire(ServeState *S, Flight &f)
         p.walker_begin = d.walker_begin;
         p.walker_num = (uint32_t)n;
         p.stride = r->pj.stride;
-        const uint64_t base = offs[r0], cnt = offs[r0 + n] - base;
-        p.lengths = (uint32_t *)malloc(n ? n * 4 : 4);
-        p.offsets = (uint64_t *)malloc((n + 1) * 8);
-        p.vertices = (uint32_t *)malloc(cnt ? cnt * 4 : 4);
-        int status = 0;
-        if (!p.lengths || !p.offsets || !p.vertices) {
-            free(p.lengths);
-            free(p.offsets);
-            free(p.vertices);
-            memset(&p, 0, sizeof(p));
-            status = KKW_E_NOMEM;
-        } else {
-            uint64_t steps = 0, trunc = 0;
-            for (uint64_t k = 0; k < n; k++) {
-                p.lengths[k] = lens[r0 + k];
-                p.offsets[k] = offs[r0 + k] - base;
-                steps += (lens[r0 + k] & 0x7fffffffu) - 1;
-                trunc += lens[r0 + k] >> 31;
-            }
-            p.offsets[n] = cnt;
-            if (cnt) memcpy(p.vertices, payload + base, cnt * 4);
-            p.steps = steps;
-            p.truncated = trunc;
-            p.priv = (void *)1;
-        }
-        finish_request(S, r, status);
+        p.vertices = payload;            // rows at payload + offsets[k]
+        p.lengths = lens + r0;
+        p.offsets = offs + r0;
+        uint64_t trunc = 0;
+        for (uint64_t k = 0; k < n; k++) trunc += lens[r0 + k] >> 31;
+        p.steps = offs[r0 + n] - offs[r0] - n;
+        p.truncated = trunc;
+        auto *sp = new ServedPaths();
+        sp->lease = lease;
+        p.priv = sp;
+        finish_request(S, r, 0);
     }
     S->dev.release(f.dev_off, f.dev_bytes);
-    S->pin.release(f.pin_off, f.pin_bytes);
-    if (f.pay_pinned) S->pin.release(f.pay_off, f.pay_bytes);
     (void)hipEventDestroy(f.ev);
     f.ev = nullptr;
+}
+
+// Copy finished results that still point into pinned memory to heap memory
+// and drop their leases (callers collect results in their own order, so
+// pinned memory held by uncollected results must not starve admission).
+// Returns how many results were moved.
+static int spill_uncollected(ServeState *S)
+{
+    int moved = 0;
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (auto &kv : S->all) {
+        Request &r = *kv.second;
+        kkw_paths &p = r.result;
+        if (r.status != 0 || !p.priv || p.priv == (void *)1) continue;
+        const uint64_t n = p.walker_num, base = p.offsets[0], cnt = p.offsets[n] - base;
+        uint32_t *lens = (uint32_t *)malloc(n ? n * 4 : 4);
+        uint64_t *offs = (uint64_t *)malloc((n + 1) * 8);
+        uint32_t *verts = (uint32_t *)malloc(cnt ? cnt * 4 : 4);
+        if (!lens || !offs || !verts) {
+            free(lens);
+            free(offs);
+            free(verts);
+            break;   // leave the rest where they are
+        }
+        memcpy(lens, p.lengths, n * 4);
+        for (uint64_t k = 0; k <= n; k++) offs[k] = p.offsets[k] - base;
+        if (cnt) memcpy(verts, p.vertices + base, cnt * 4);
+        delete static_cast<ServedPaths *>(p.priv);
+        p.lengths = lens;
+        p.offsets = offs;
+        p.vertices = verts;
+        p.priv = (void *)1;
+        moved++;
+    }
+    return moved;
+}
+
+// Give back what a failed flight still holds (arena blocks, event).
+static void release_flight(ServeState *S, Flight &f)
+{
+    if (f.dev_bytes && S->dev_base) {
+        S->dev.release(f.dev_off, f.dev_bytes);
+        f.dev_bytes = 0;
+    }
+    if (f.pin_bytes) {
+        S->pin->release(f.pin_off, f.pin_bytes);
+        f.pin_bytes = 0;
+    }
+    if (f.ev) {
+        (void)hipEventDestroy(f.ev);
+        f.ev = nullptr;
+    }
 }
 
 static void serve_loop(ServeState *S)
@@ -726,14 +808,23 @@ static void serve_loop(ServeState *S)
             }
             progressed = true;
             try {
-                if (f.phase == 1) {
-                    flight_phase2(S, f);
-                    ++it;
-                    continue;
-                }
+                f.t_phase1 = f.t_phase2 = now_ms();
                 flight_retire(S, f);
-            } catch (const Error &e) {
-                for (auto &r : f.reqs) finish_request(S, r, e.code);
+                f.t_retired = now_ms();
+                if (!S->trace_path.empty()) {
+                    double first_submit = f.t_admit;
+                    for (auto &r : f.reqs) first_submit = std::min(first_submit, r->t_submit);
+                    char line[256];
+                    snprintf(line, sizeof line, "%.4f,%.4f,%.4f,%.4f,%.4f,%.4f,%zu,%llu,%llu", first_submit,
+                             f.t_admit, f.t_enqueued, f.t_phase1, f.t_phase2, f.t_retired, f.reqs.size(),
+                             (unsigned long long)f.walkers, (unsigned long long)f.total);
+                    S->trace.push_back(line);
+                }
+            } catch (const std::exception &ex) {
+                const int code = dynamic_cast<const Error *>(&ex) ? static_cast<const Error &>(ex).code : KKW_E_NOMEM;
+                for (auto &r : f.reqs)
+                    if (r->status == 1) finish_request(S, r, code);
+                release_flight(S, f);
             }
             stream_busy[f.sidx] = false;
             walkers_in_flight -= f.walkers;
@@ -747,12 +838,15 @@ static void serve_loop(ServeState *S)
                 break;
             }
         std::vector<std::shared_ptr<Request>> admit;
+        bool pin_blocked = false;   // the head request waits for pinned memory
         {
             std::unique_lock<std::mutex> lk(S->mu);
             if (S->stop && S->pending.empty() && flights.empty()) break;
             if (sidx >= 0) {
                 size_t room = S->dev.largest();
                 size_t used = 4096;
+                const size_t pin_room = S->pin->largest();
+                size_t pin_used = 4096;
                 uint64_t walkers = walkers_in_flight;
                 while (!S->pending.empty()) {
                     auto r = S->pending.front();
@@ -764,28 +858,53 @@ static void serve_loop(ServeState *S)
                         continue;
                     }
                     size_t need = request_dev_bytes(r->pj) + 2 * pack_scan_bytes(r->pj.dj.walker_num + 1);
-                    if (need + 4096 > S->dev.cap || r->pj.dj.walker_num * (uint64_t)(r->pj.stride) * 4 > S->pin.cap) {
+                    if (need + 4096 > S->dev.cap || request_pin_bytes(r->pj) + 4096 > S->pin->a.cap) {
                         S->pending.pop_front();   // can never fit the arenas
                         r->status = KKW_E_NOMEM;
                         r->t_done = now_ms();
                         S->cv_done.notify_all();
                         continue;
                     }
+                    const size_t pneed = request_pin_bytes(r->pj);
                     bool fits_mem = used + need <= room;
+                    bool fits_pin = pin_used + pneed <= pin_room;   // worst-case payloads of the whole flight
                     bool fits_slots = walkers + r->pj.dj.walker_num <= S->opts.max_walkers_in_flight ||
                                       (walkers == 0 && admit.empty());
-                    if (!fits_mem || !fits_slots || admit.size() >= 1024) break;
+                    if (!fits_mem || !fits_pin || !fits_slots || admit.size() >= 1024) {
+                        pin_blocked = admit.empty() && !fits_pin;
+                        break;
+                    }
                     S->pending.pop_front();
                     admit.push_back(r);
                     used += need;
+                    pin_used += pneed;
                     walkers += r->pj.dj.walker_num;
                 }
             }
-            if (admit.empty() && flights.empty()) {
+            if (admit.empty() && flights.empty() && !pin_blocked) {
                 if (S->stop && S->pending.empty()) break;
                 S->cv_work.wait_for(lk, std::chrono::milliseconds(20));
                 continue;
             }
+        }
+        if (admit.empty() && pin_blocked) {
+            // pinned memory is held by finished results: move uncollected ones
+            // to heap memory; otherwise wait for flights or callers to free some
+            if (spill_uncollected(S) == 0) {
+                if (flights.empty() && S->pin->in_use_by_results()) {
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    if (S->stop) {   // stopping: callers still hold results, nothing more can start
+                        for (auto &r : S->pending) {
+                            r->status = KKW_E_CANCELLED;
+                            r->t_done = now_ms();
+                        }
+                        S->pending.clear();
+                        S->cv_done.notify_all();
+                    }
+                }
+                std::this_thread::sleep_for(std::chrono::microseconds(50));
+            }
+            continue;
         }
         if (!admit.empty()) {
             progressed = true;
@@ -793,21 +912,26 @@ static void serve_loop(ServeState *S)
             Flight &f = flights.back();
             f.sidx = sidx;
             f.reqs = admit;
-            bool ok = false;
+            f.t_admit = now_ms();
+            bool ok = false, pin_short = false;
             try {
-                ok = launch_flight(S, f);
+                ok = launch_flight(S, f, &pin_short);
+                f.t_enqueued = now_ms();
             } catch (const Error &e) {
+                (void)hipStreamSynchronize(S->streams[sidx]);   // nothing of it may still run
                 for (auto &r : f.reqs) finish_request(S, r, e.code);
-                if (f.dev_bytes && f.phase == 0) {
-                }
+                release_flight(S, f);
                 flights.pop_back();
                 continue;
             }
             if (!ok) {
                 // no room after all: put them back at the head of the queue
-                std::lock_guard<std::mutex> lk(S->mu);
-                for (auto it = admit.rbegin(); it != admit.rend(); ++it) S->pending.push_front(*it);
+                {
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    for (auto it = admit.rbegin(); it != admit.rend(); ++it) S->pending.push_front(*it);
+                }
                 flights.pop_back();
+                if (!(pin_short && spill_uncollected(S) > 0)) progressed = false;   // wait for room
             } else {
                 stream_busy[sidx] = true;
                 walkers_in_flight += f.walkers;
@@ -1237,9 +1361,17 @@ int kkw_paths_free(kkw_paths *p)
 {
     if (!p) return KKW_OK;
     if (p->priv && !p->on_device) {
-        free(p->vertices);
-        free(p->lengths);
-        free(p->offsets);
+        if (p->priv == (void *)1) {   // host buffers of kkw_run / kkw_batch_paths / kkw_program_run
+            free(p->vertices);
+            free(p->lengths);
+            free(p->offsets);
+        } else {                      // a served result: drop its lease on the pinned block
+            auto *sp = static_cast<ServedPaths *>(p->priv);
+            if (sp->magic == ServedPaths::kMagic) {
+                sp->magic = 0;
+                delete sp;
+            }
+        }
     }
     memset(p, 0, sizeof(*p));
     return KKW_OK;
@@ -1416,6 +1548,7 @@ int kkw_serve_start(kkw_engine *eng, const kkw_serve_opts *opts)
         if (eng->serve) return fail(KKW_E_INVALID, "serving already started");
         auto S = std::make_unique<ServeState>();
         S->eng = eng;
+        if (const char *tp = getenv("KKW_SERVE_TRACE")) S->trace_path = tp;
         if (opts) S->opts = *opts;
         if (S->opts.max_walkers_in_flight == 0) S->opts.max_walkers_in_flight = 1u << 22;
         if (S->opts.queue_capacity == 0) S->opts.queue_capacity = 1u << 16;
@@ -1428,9 +1561,16 @@ int kkw_serve_start(kkw_engine *eng, const kkw_serve_opts *opts)
         if (cap > free_b) cap = free_b / 2;
         KKW_HIP(hipMalloc((void **)&S->dev_base, cap ? cap : 256));
         S->dev.init(cap);
-        size_t pin_cap = std::min(cap, (size_t)1 << 30);
-        KKW_HIP(hipHostMalloc((void **)&S->pin_base, pin_cap ? pin_cap : 256, hipHostMallocDefault));
-        S->pin.init(pin_cap);
+        // pinned staging: flights pack their paths into it and results are
+        // handed out in place (released by kkw_paths_free), so it is sized for
+        // several flights' worst case (rows x stride)
+        size_t pin_cap = std::min(cap, (size_t)4 << 30);
+        S->pin = std::make_shared<PinArena>();
+        KKW_HIP(hipHostMalloc((void **)&S->pin->base, pin_cap ? pin_cap : 256, hipHostMallocDefault));
+        void *dp = nullptr;
+        KKW_HIP(hipHostGetDevicePointer(&dp, S->pin->base, 0));
+        S->pin->dbase = (char *)dp;
+        S->pin->a.init(pin_cap);
         ServeState *raw = S.get();
         S->worker = std::thread(serve_loop, raw);
         eng->serve = std::move(S);
@@ -1533,11 +1673,19 @@ int kkw_serve_stop(kkw_engine *eng)
     }
     S->cv_work.notify_all();
     if (S->worker.joinable()) S->worker.join();
+    if (!S->trace_path.empty()) {
+        if (FILE *f = fopen(S->trace_path.c_str(), "w")) {
+            fprintf(f, "first_submit_ms,admit_ms,enqueued_ms,phase1_seen_ms,phase2_seen_ms,retired_ms,requests,"
+                       "walkers,packed_vertices\n");
+            for (auto &l : S->trace) fprintf(f, "%s\n", l.c_str());
+            fclose(f);
+        }
+    }
     for (auto &kv : S->all) kkw_paths_free(&kv.second->result);
     for (auto &s : S->streams)
         if (s) (void)hipStreamDestroy(s);
     if (S->dev_base) (void)hipFree(S->dev_base);
-    if (S->pin_base) (void)hipHostFree(S->pin_base);
+    S->pin.reset();   // freed once the last served result using it is freed
     eng->serve.reset();
     return KKW_OK;
 }

@@ -531,22 +531,53 @@ __global__ void k_lens_to_counts(const uint32_t *__restrict__ lens, uint64_t n, 
 
 // Copy every walker's row into the packed array at its offset.  Rows of a
 // flight are numbered across its jobs; job j owns rows [row0, row0 + n).
-__global__ void k_pack_rows(const PackJob *__restrict__ jobs, int n_jobs, const uint32_t *__restrict__ lens,
-                            const uint64_t *__restrict__ offs, uint64_t n, uint32_t *__restrict__ out)
+// Copy every walker's row into the packed array at its offset.  Rows of a
+// flight are numbered across its jobs; job j owns rows [row0, row0 + n).  One
+// wave packs 64 consecutive rows: their packed rows are contiguous, so the
+// wave writes that whole range with consecutive lanes on consecutive
+// entries (the output may be pinned host memory, written over PCIe), finding
+// each entry's row by a binary search over the 64 offsets staged in LDS.
+__device__ __forceinline__ void pack_wave_sync()
 {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int lo = 0, hi = n_jobs - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].row0 <= i) lo = mid;
-        else hi = mid - 1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(256) k_pack_rows(const PackJob *__restrict__ jobs, int n_jobs,
+                                                   const uint64_t *__restrict__ offs, uint64_t n,
+                                                   uint32_t *__restrict__ out)
+{
+    __shared__ uint64_t s_off[4][65];
+    __shared__ const uint32_t *s_src[4][64];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t i0 = ((uint64_t)blockIdx.x * 4 + wv) * 64;
+    if (i0 >= n) return;   // whole wave; nothing below synchronises across waves
+    const uint32_t cnt = n - i0 < 64 ? (uint32_t)(n - i0) : 64u;
+    if (lane < cnt) {
+        const uint64_t i = i0 + lane;
+        int lo = 0, hi = n_jobs - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].row0 <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        s_src[wv][lane] = jobs[lo].paths + (i - jobs[lo].row0) * jobs[lo].stride;
     }
-    const PackJob &J = jobs[lo];
-    const uint32_t len = lens[i] & 0x7fffffffu;
-    const uint32_t *src = J.paths + (i - J.row0) * J.stride;
-    uint32_t *dst = out + offs[i];
-    for (uint32_t k = 0; k < len; k++) dst[k] = src[k];
+    s_off[wv][lane] = offs[i0 + (lane < cnt ? lane : cnt)];
+    if (lane == 0) s_off[wv][64] = offs[i0 + cnt];
+    pack_wave_sync();
+    const uint64_t base = s_off[wv][0];
+    const uint64_t total = s_off[wv][cnt] - base;
+    for (uint64_t o = lane; o < total; o += 64) {
+        uint32_t lo = 0, hi = cnt - 1;
+        while (lo < hi) {   // last row whose packed start is <= o
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (s_off[wv][mid] - base <= o) lo = mid;
+            else hi = mid - 1;
+        }
+        out[base + o] = s_src[wv][lo][o - (s_off[wv][lo] - base)];
+    }
 }
 
 void launch_pack(const PackJob *jobs, int n_jobs, const uint32_t *lens, uint64_t *cnt, uint64_t *offs, uint64_t n,
@@ -558,7 +589,7 @@ void launch_pack(const PackJob *jobs, int n_jobs, const uint32_t *lens, uint64_t
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long *)offs, 0ull, 1);
     size_t tb = scan_bytes;
     (void)hipcub::DeviceScan::InclusiveSum(scan_tmp, tb, cnt, offs + 1, n, s);
-    hipLaunchKernelGGL(k_pack_rows, dim3(g), dim3(256), 0, s, jobs, n_jobs, lens, offs, n, out);
+    hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, jobs, n_jobs, offs, n, out);
 }
 
 size_t pack_scan_bytes(uint64_t n)

@@ -16,8 +16,12 @@ pytestmark = pytest.mark.gpu
 
 
 def same(a, b):
+    """a: Paths or PackedPaths; b: Paths (rows of `stride` entries)."""
     if not np.array_equal(a.lengths, b.lengths):
         return False
+    if isinstance(a, kk.PackedPaths):
+        mask = np.arange(b.vertices.shape[1])[None, :] < b.lengths[:, None]
+        return bool(np.array_equal(a.vertices, b.vertices[mask]) and a.offsets[-1] == len(a.vertices))
     mask = np.arange(a.vertices.shape[1])[None, :] < a.lengths[:, None]
     return bool(np.all((a.vertices == b.vertices) | ~mask))
 
@@ -68,7 +72,7 @@ def test_results_same_alone_and_in_the_mix(served):
     for t in threads:
         t.join()
     for i, t in enumerate(tickets):
-        got, lat = eng.wait(t, 60.0)
+        got, lat = eng.wait(t, 60.0, packed=bool(i % 2))
         assert lat >= 0
         assert same(got, alone[i]), f"query {i} differs in the mix"
 
@@ -142,3 +146,28 @@ def test_malformed_descriptor_rejected_at_submit(served, bad):
     with pytest.raises(kk.KkwError) as e:
         eng.submit(job)
     assert e.value.code == kk.KKW_E_INVALID
+
+
+def test_results_collected_out_of_order_under_pinned_pressure(engine):
+    """Served results live in pinned memory until collected; with a pinned
+    arena that holds only a few of them and results collected newest first,
+    admission must still progress (finished results are moved to heap memory)
+    and every result must equal the same job run alone."""
+    eng = kk.Engine(hbm_budget=96 << 20)   # device and pinned arenas of 96 MiB
+    try:
+        eng.graph_rmat(12, 16, seed=3, weighted=True)
+        eng.prepare()
+        jobs = [kk.deepwalk_job(20000, 80, True, seed=500 + i) for i in range(40)]   # ~6.7 MB of paths each
+        print("[gpu] 40 served DeepWalk queries, collected newest first", flush=True)
+        eng.serve_start(max_walkers_in_flight=1 << 20)
+        tickets = [eng.submit(j) for j in jobs]
+        got = {}
+        for i in reversed(range(len(jobs))):
+            got[i], _ = eng.wait(tickets[i], 60.0)
+        eng.serve_stop()
+        for i in (0, 13, 39):
+            (alone,) = eng.run([jobs[i]])
+            assert same(got[i], alone), f"query {i} differs"
+    finally:
+        eng.serve_stop()
+        eng.close()
