@@ -63,7 +63,10 @@ static void node2vec_constants(float p, float q, DevJob &d)
     d.acc_hi = std::max(1.0f, d.inv_q);
 }
 
-static PreparedJob prepare_job(kkw_engine *eng, const kkw_job &j)
+// line_rows: the job runs alone in the persistent state-machine kernel, which
+// writes rows of whole 128-byte lines when the row stride is a multiple of 32
+// entries; rows are padded to that when the padding stays within a quarter.
+static PreparedJob prepare_job(kkw_engine *eng, const kkw_job &j, bool line_rows = false)
 {
     GraphStore &gs = eng->gs;
     if (!gs.row_ptr) throw Error(KKW_E_NOGRAPH, "no graph loaded");
@@ -90,6 +93,9 @@ static PreparedJob prepare_job(kkw_engine *eng, const kkw_job &j)
     if (L > (1u << 24)) throw Error(KKW_E_INVALID, "walk_length above 2^24");
     d.walk_length = L;
     pj.stride = (L + 1 + 3) & ~3u;   // rows of whole 16-byte groups (path write-out)
+    const uint64_t line_stride = ((uint64_t)L + 1 + 31) & ~(uint64_t)31;
+    if (line_rows && L + 1 >= 32 && (line_stride - (L + 1)) * 4 <= (uint64_t)L + 1)
+        pj.stride = (uint32_t)line_stride;
     d.stride = pj.stride;
     if (j.kind == KKW_NODE2VEC) {
         if (!(j.p > 0.0f && std::isfinite(j.p) && j.q > 0.0f && std::isfinite(j.q)))
@@ -184,7 +190,8 @@ static void batch_setup(kkw_batch *b, const kkw_job *jobs, int n_jobs)
 {
     kkw_engine *eng = b->eng;
     if (n_jobs < 0 || (n_jobs > 0 && !jobs)) throw Error(KKW_E_INVALID, "bad job array");
-    for (int i = 0; i < n_jobs; i++) b->jobs.push_back(prepare_job(eng, jobs[i]));
+    const bool line_rows = b->mode == KKW_MODE_PERSISTENT && n_jobs == 1;
+    for (int i = 0; i < n_jobs; i++) b->jobs.push_back(prepare_job(eng, jobs[i], line_rows));
     b->graph_V = eng->gs.V;
     b->graph_E = eng->gs.E;
     KKW_HIP(hipEventCreate(&b->e0));

@@ -15,6 +15,9 @@ struct kkw_prog_bucket {
     uint32_t meta;
     uint64_t arow;
 };
+// the walk loop reads each 16-byte half with one load
+static_assert(sizeof(kkw_prog_bucket) == 32 && __builtin_offsetof(kkw_prog_bucket, alias) == 16,
+              "bucket halves are {p, primary, prow} and {alias, meta, arow}");
 
 struct kkw_prog_args {
     kkw_ctx c;

@@ -106,15 +106,17 @@ __device__ __forceinline__ bool kkw_prog_trial(const WalkerState &s, const kkw_w
     kkw_edge e;
     e.reserved = 0;
     if (A.lb) {
-        // first half {p, local primary, primary's id}; the alias half only when
-        // the coin picks it
-        const kkw_prog_bucket &b = A.lb[beg + k];
-        if (kkw::u01(r.z) < b.p) {
-            e.index = beg + b.primary;
-            e.neighbour = (uint32_t)b.prow;
+        // first half {p, local primary, primary's id} as one 16-byte load; the
+        // alias half {local alias, meta, alias's id} only when the coin picks it
+        const uint32_t *bw = reinterpret_cast<const uint32_t *>(A.lb + beg + k);
+        const kkw_prog_u4a h0 = *reinterpret_cast<const kkw_prog_u4a *>(bw);
+        if (kkw::u01(r.z) < __uint_as_float(h0.x)) {
+            e.index = beg + h0.y;
+            e.neighbour = h0.z;
         } else {
-            e.index = beg + b.alias;
-            e.neighbour = (uint32_t)b.arow;
+            const kkw_prog_u4a h1 = *reinterpret_cast<const kkw_prog_u4a *>(bw + 4);
+            e.index = beg + h1.x;
+            e.neighbour = h1.z;
         }
     } else {
         e.index = beg + k;

@@ -47,6 +47,27 @@ struct PathGroup {
     }
 };
 
+// Path write-out in whole 128-byte lines (rows of a multiple of 32 entries):
+// the lane's current 32 entries wait in LDS and leave as eight 16-byte stores
+// to one line when the line fills or the walk ends, so every line reaches HBM
+// whole -- a line written 16 bytes at a time by a lane, minutes apart, is
+// written back piecemeal, and partial-line writes cost HBM far more than their
+// bytes.  Lanes sit 33 words apart in LDS, so lanes at the same step use
+// different banks.  Entries past a path's length are unspecified.
+constexpr uint32_t LINE_ENTRIES = 32, LINE_LDS_STRIDE = 33;
+
+struct LineBuf {
+    uint32_t *s;   // this lane's LINE_ENTRIES entries in LDS
+    __device__ __forceinline__ void set(uint32_t i, uint32_t v) { s[i & (LINE_ENTRIES - 1)] = v; }
+    __device__ __forceinline__ void store(uint32_t *row, uint32_t i) const
+    {
+        uint4 *dst = reinterpret_cast<uint4 *>(row + (i & ~(LINE_ENTRIES - 1)));
+#pragma unroll
+        for (uint32_t k = 0; k < LINE_ENTRIES / 4; k++)
+            dst[k] = make_uint4(s[4 * k], s[4 * k + 1], s[4 * k + 2], s[4 * k + 3]);
+    }
+};
+
 struct Walker {
     uint32_t job;      // job index (persistent batches)
     uint32_t idx;      // row within the job
@@ -499,7 +520,10 @@ void launch_stream_copy(const void *src, void *dst, uint64_t bytes, int cu_count
 // Independent random 16-byte reads, each from its own 128-byte line of a
 // buffer of `lines` lines: the rate the memory system serves such reads at is
 // the ceiling of a kernel whose accesses are all data-dependent 16-byte reads.
-__global__ void __launch_bounds__(256) k_random_gather(const uint4 *__restrict__ buf, uint64_t lines,
+// Each read is one global_load_dwordx4 (load16, all four words consumed): read
+// through a uint4 struct the compiler narrows it to the words used, and two
+// 4-byte reads of one line are served at a lower line rate.
+__global__ void __launch_bounds__(256) k_random_gather(const uint32_t *__restrict__ buf, uint64_t lines,
                                                        uint32_t reads, uint64_t seed, uint32_t *__restrict__ sink)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -508,8 +532,8 @@ __global__ void __launch_bounds__(256) k_random_gather(const uint4 *__restrict__
         const u32x4 r = philox(t, i, 0u, 0x200u, (uint32_t)seed, (uint32_t)(seed >> 32));
         const uint64_t h = (uint64_t)r.x | ((uint64_t)r.y << 32);
         const uint64_t line = __umul64hi(h, lines);   // < lines
-        const uint4 v = buf[line * 8 + (r.z & 7u)];
-        acc += v.x ^ v.w;
+        const u32x4 v = load16(buf + line * 32 + (r.z & 7u) * 4);
+        acc += v.x ^ v.y ^ v.z ^ v.w;
     }
     if (acc == 0x9E3779B9u) sink[t & 63] = acc;   // keeps the reads; practically never taken
 }
@@ -517,7 +541,7 @@ __global__ void __launch_bounds__(256) k_random_gather(const uint4 *__restrict__
 void launch_random_gather(const void *buf, uint64_t bytes, uint32_t threads, uint32_t reads, uint64_t seed,
                           uint32_t *sink, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_random_gather, dim3(threads / 256), dim3(256), 0, stream, (const uint4 *)buf, bytes / 128,
+    hipLaunchKernelGGL(k_random_gather, dim3(threads / 256), dim3(256), 0, stream, (const uint32_t *)buf, bytes / 128,
                        reads, seed, sink);
 }
 
@@ -611,10 +635,14 @@ size_t pack_scan_bytes(uint64_t n)
 enum : uint32_t { PH_ROW = 0, PH_TRIAL = 1, PH_SEARCH = 2, PH_CROW = 3, PH_HASH = 4, PH_ALIAS = 5 };
 
 
-template <int KIND, bool ALIAS, bool STATS>
+// LINES: rows of whole 128-byte lines, written through LineBuf (dynamic LDS of
+// LINE_LDS_STRIDE words per thread); otherwise 16-byte groups (PathGroup).
+template <int KIND, bool ALIAS, bool STATS, bool LINES>
 __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t n_chunks, uint32_t chunk,
                                                  unsigned long long *queue, DevStats *st)
 {
+    extern __shared__ uint32_t sm_lines[];
+    LineBuf lb = {sm_lines + threadIdx.x * LINE_LDS_STRIDE};
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     // candidates come from buckets: the alias tables, or the uniform buckets
@@ -675,7 +703,8 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
                     scheme = bounded(r.x, r.y, J.n_schemes);
                     state = 0;
                 }
-                pg.q0 = pg.q1 = pg.q2 = pg.q3 = cur;
+                if (LINES) lb.set(0, cur);
+                else pg.q0 = pg.q1 = pg.q2 = pg.q3 = cur;
                 active = true;
             }
             uint32_t want = (uint32_t)__popcll(need);
@@ -896,7 +925,11 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             trial++;
             phase = PH_TRIAL;
             if (trial >= MAX_TRIALS) {
-                if ((step & 3u) != 3u) pg.store(paths + (uint64_t)idx * stride, step);
+                if (LINES) {
+                    if ((step & (LINE_ENTRIES - 1)) != LINE_ENTRIES - 1) lb.store(paths + (uint64_t)idx * stride, step);
+                } else if ((step & 3u) != 3u) {
+                    pg.store(paths + (uint64_t)idx * stride, step);
+                }
                 J.lens[idx] = (step + 1) | 0x80000000u;
                 active = false;
             }
@@ -904,8 +937,13 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
         }
         if (moved) {
             step++;
-            pg.set(step, cand);
-            if ((step & 3u) == 3u) pg.store(paths + (uint64_t)idx * stride, step);
+            if (LINES) {
+                lb.set(step, cand);
+                if ((step & (LINE_ENTRIES - 1)) == LINE_ENTRIES - 1) lb.store(paths + (uint64_t)idx * stride, step);
+            } else {
+                pg.set(step, cand);
+                if ((step & 3u) == 3u) pg.store(paths + (uint64_t)idx * stride, step);
+            }
             if (KIND == KIND_METAPATH) state = (state + 1) % J.scheme_len[scheme];
             prev = cur;
             pbeg = beg;
@@ -935,7 +973,11 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             }
         }
         if (stop) {
-            if ((step & 3u) != 3u) pg.store(paths + (uint64_t)idx * stride, step);
+            if (LINES) {
+                if ((step & (LINE_ENTRIES - 1)) != LINE_ENTRIES - 1) lb.store(paths + (uint64_t)idx * stride, step);
+            } else if ((step & 3u) != 3u) {
+                pg.store(paths + (uint64_t)idx * stride, step);
+            }
             J.lens[idx] = (step + 1) | (capped ? 0x80000000u : 0u);
             active = false;
             continue;
@@ -1001,12 +1043,21 @@ static void launch_single_t(const DevGraph &g, const DevJob &J, uint64_t n_chunk
     }();
     const bool sm = !trial_form && !(KIND == KIND_NODE2VEC && J.overflow > 0.0f);
     if (sm) {
-        if (stats)
-            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, true>), dim3(grid), dim3(block), 0, stream, g, J, n_chunks,
-                               chunk, queue, st);
+        // rows of whole lines: 128-byte aligned rows (the path buffer is 256-byte aligned)
+        const bool lines = J.stride % LINE_ENTRIES == 0;
+        const size_t lds = lines ? (size_t)block * LINE_LDS_STRIDE * sizeof(uint32_t) : 0;
+        if (stats && lines)
+            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, true, true>), dim3(grid), dim3(block), lds, stream, g, J,
+                               n_chunks, chunk, queue, st);
+        else if (stats)
+            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, true, false>), dim3(grid), dim3(block), 0, stream, g, J,
+                               n_chunks, chunk, queue, st);
+        else if (lines)
+            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, false, true>), dim3(grid), dim3(block), lds, stream, g, J,
+                               n_chunks, chunk, queue, st);
         else
-            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, false>), dim3(grid), dim3(block), 0, stream, g, J, n_chunks,
-                               chunk, queue, st);
+            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, false, false>), dim3(grid), dim3(block), 0, stream, g, J,
+                               n_chunks, chunk, queue, st);
         return;
     }
     if (stats)
