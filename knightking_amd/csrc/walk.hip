@@ -49,22 +49,26 @@ struct PathGroup {
 
 // Path write-out in whole 128-byte lines (rows of a multiple of 32 entries):
 // the lane's current 32 entries wait in LDS and leave as eight 16-byte stores
-// to one line when the line fills or the walk ends, so every line reaches HBM
-// whole -- a line written 16 bytes at a time by a lane, minutes apart, is
-// written back piecemeal, and partial-line writes cost HBM far more than their
-// bytes.  Lanes sit 33 words apart in LDS, so lanes at the same step use
-// different banks.  Entries past a path's length are unspecified.
+// to one line when the line fills, so full lines reach HBM whole -- a line
+// written 16 bytes at a time by a lane, microseconds apart, is written back
+// piecemeal, and partial-line writes cost HBM far more than their bytes.  The
+// line that ends a walk is written up to its last 16-byte group.  Lanes sit 33
+// words apart in LDS, so lanes at the same step use different banks.  Entries
+// past a path's length are unspecified.
 constexpr uint32_t LINE_ENTRIES = 32, LINE_LDS_STRIDE = 33;
 
 struct LineBuf {
     uint32_t *s;   // this lane's LINE_ENTRIES entries in LDS
     __device__ __forceinline__ void set(uint32_t i, uint32_t v) { s[i & (LINE_ENTRIES - 1)] = v; }
+    // the groups of i's line up to the one holding entry i (the whole line when
+    // i is its last entry)
     __device__ __forceinline__ void store(uint32_t *row, uint32_t i) const
     {
         uint4 *dst = reinterpret_cast<uint4 *>(row + (i & ~(LINE_ENTRIES - 1)));
+        const uint32_t groups = ((i & (LINE_ENTRIES - 1)) >> 2) + 1;
 #pragma unroll
         for (uint32_t k = 0; k < LINE_ENTRIES / 4; k++)
-            dst[k] = make_uint4(s[4 * k], s[4 * k + 1], s[4 * k + 2], s[4 * k + 3]);
+            if (k < groups) dst[k] = make_uint4(s[4 * k], s[4 * k + 1], s[4 * k + 2], s[4 * k + 3]);
     }
 };
 
