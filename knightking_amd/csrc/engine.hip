@@ -63,8 +63,8 @@ static void node2vec_constants(float p, float q, DevJob &d)
     d.acc_hi = std::max(1.0f, d.inv_q);
 }
 
-// line_rows: the job runs alone in the persistent state-machine kernel, which
-// writes rows of whole 128-byte lines when the row stride is a multiple of 32
+// line_rows: the job runs in a persistent batch, whose kernels write rows of
+// whole 128-byte lines from LDS when the row stride is a multiple of 32
 // entries; rows are padded to that when the padding stays within a quarter.
 static PreparedJob prepare_job(kkw_engine *eng, const kkw_job &j, bool line_rows = false)
 {
@@ -156,6 +156,7 @@ struct kkw_batch {
     int mode = KKW_MODE_PLAIN;
     bool stats = false;
     bool use_graph = false;
+    bool line_rows = false;        // persistent: some job's rows are whole 128-byte lines
     std::vector<PreparedJob> jobs;
     std::vector<DevBuf> bufs;
     DevJob *d_jobs = nullptr;
@@ -190,8 +191,9 @@ static void batch_setup(kkw_batch *b, const kkw_job *jobs, int n_jobs)
 {
     kkw_engine *eng = b->eng;
     if (n_jobs < 0 || (n_jobs > 0 && !jobs)) throw Error(KKW_E_INVALID, "bad job array");
-    const bool line_rows = b->mode == KKW_MODE_PERSISTENT && n_jobs == 1;
-    for (int i = 0; i < n_jobs; i++) b->jobs.push_back(prepare_job(eng, jobs[i], line_rows));
+    for (int i = 0; i < n_jobs; i++)
+        b->jobs.push_back(prepare_job(eng, jobs[i], b->mode == KKW_MODE_PERSISTENT));
+    for (auto &pj : b->jobs) b->line_rows = b->line_rows || pj.stride % 32 == 0;
     b->graph_V = eng->gs.V;
     b->graph_E = eng->gs.E;
     KKW_HIP(hipEventCreate(&b->e0));
@@ -276,7 +278,7 @@ static void batch_enqueue(kkw_batch *b)
                                           b->d_stats, grid, b->block, s);
         else
             launch_walk_persistent_mixed(g, b->d_jobs, (int)b->jobs.size(), b->n_chunks, (uint32_t)b->chunk,
-                                         b->d_queue, b->stats, b->d_stats, grid, b->block, s);
+                                         b->d_queue, b->stats, b->d_stats, grid, b->block, b->line_rows, s);
     }
     KKW_HIP(hipGetLastError());
 }
@@ -687,8 +689,9 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
         launch_fill_u64((unsigned long long *)(D + o_queue), 0ull, 1, st);
         uint64_t blocks_needed = (chunk_acc * 64 + 255) / 256;
         int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)eng->cu_count * 8, blocks_needed));
+        // serving rows stay 16-byte groups (short PPR walks; full occupancy)
         launch_walk_persistent_mixed(g, (const DevJob *)(D + o_jobs), nj, chunk_acc, chunk,
-                                     (unsigned long long *)(D + o_queue), false, nullptr, grid, 256, st);
+                                     (unsigned long long *)(D + o_queue), false, nullptr, grid, 256, false, st);
         launch_pack((const PackJob *)(D + o_pjobs), nj, (const uint32_t *)(D + f.o_lens), (uint64_t *)(D + o_cnt),
                     (uint64_t *)(D + f.o_offs), N, (uint32_t *)(HD + f.p_payload), D + o_scan, scan_bytes, st);
     } else {

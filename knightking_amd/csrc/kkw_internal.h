@@ -74,9 +74,11 @@ void launch_walk_plain(const DevGraph &g, const DevJob &J, bool stats, DevStats 
 void launch_walk_persistent_single(const DevGraph &g, const DevJob &J, uint64_t n_chunks,
                                    uint32_t chunk, unsigned long long *queue, bool stats,
                                    DevStats *st, int grid, int block, hipStream_t stream);
+// lines: some job's rows are a multiple of 32 entries (written as whole lines
+// through the LDS line buffer the launch then allocates)
 void launch_walk_persistent_mixed(const DevGraph &g, const DevJob *jobs, int n_jobs, uint64_t n_chunks,
                                   uint32_t chunk, unsigned long long *queue, bool stats, DevStats *st,
-                                  int grid, int block, hipStream_t stream);
+                                  int grid, int block, bool lines, hipStream_t stream);
 void launch_fill_u64(unsigned long long *p, unsigned long long v, int n, hipStream_t stream);
 void launch_stream_copy(const void *src, void *dst, uint64_t bytes, int cu_count, hipStream_t stream);
 void launch_random_gather(const void *buf, uint64_t bytes, uint32_t threads, uint32_t reads, uint64_t seed,

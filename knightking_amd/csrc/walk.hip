@@ -55,22 +55,36 @@ struct PathGroup {
 // line that ends a walk is written up to its last 16-byte group.  Lanes sit 33
 // words apart in LDS, so lanes at the same step use different banks.  Entries
 // past a path's length are unspecified.
-constexpr uint32_t LINE_ENTRIES = 32, LINE_LDS_STRIDE = 33;
+constexpr uint32_t LINE_ENTRIES = 32, LINE_LDS_STRIDE = LINE_ENTRIES + 1;
+// The trial-loop kernels buffer 64-byte half lines: 17 words per lane keep
+// 256-thread blocks at eight per CU (their best occupancy), where whole lines
+// would allow four.
+constexpr uint32_t HALF_LINE_ENTRIES = 16;
 
-struct LineBuf {
-    uint32_t *s;   // this lane's LINE_ENTRIES entries in LDS
-    __device__ __forceinline__ void set(uint32_t i, uint32_t v) { s[i & (LINE_ENTRIES - 1)] = v; }
-    // the groups of i's line up to the one holding entry i (the whole line when
-    // i is its last entry)
+// N entries per buffered chunk (a power of two, a multiple of 4)
+template <uint32_t N>
+struct LineBufN {
+    uint32_t *s;   // this lane's N entries in LDS
+    __device__ __forceinline__ void set(uint32_t i, uint32_t v) const { s[i & (N - 1)] = v; }
+    // the groups of i's chunk up to the one holding entry i (the whole chunk
+    // when i is its last entry)
     __device__ __forceinline__ void store(uint32_t *row, uint32_t i) const
     {
-        uint4 *dst = reinterpret_cast<uint4 *>(row + (i & ~(LINE_ENTRIES - 1)));
-        const uint32_t groups = ((i & (LINE_ENTRIES - 1)) >> 2) + 1;
+        uint4 *dst = reinterpret_cast<uint4 *>(row + (i & ~(N - 1)));
+        const uint32_t groups = ((i & (N - 1)) >> 2) + 1;
 #pragma unroll
-        for (uint32_t k = 0; k < LINE_ENTRIES / 4; k++)
+        for (uint32_t k = 0; k < N / 4; k++)
             if (k < groups) dst[k] = make_uint4(s[4 * k], s[4 * k + 1], s[4 * k + 2], s[4 * k + 3]);
     }
 };
+using LineBuf = LineBufN<LINE_ENTRIES>;
+
+// Dynamic LDS of the line-buffered kernels: N + 1 words per thread for chunks
+// of N entries.
+extern __shared__ uint32_t lines_lds[];
+
+template <uint32_t N = LINE_ENTRIES>
+__device__ __forceinline__ LineBufN<N> lane_line() { return LineBufN<N>{lines_lds + threadIdx.x * (N + 1)}; }
 
 struct Walker {
     uint32_t job;      // job index (persistent batches)
@@ -81,6 +95,7 @@ struct Walker {
     uint64_t pbeg, pend; // list of prev
     uint32_t step, trial;
     uint32_t scheme, state;
+    bool lines;        // row written through the lane's LDS line buffer
     PathGroup pg;
 };
 
@@ -161,9 +176,11 @@ __device__ __forceinline__ uint32_t start_vertex(const DevGraph &g, const DevJob
     return wid % g.V;
 }
 
+// lines_ok: the kernel was launched with the LDS line buffer; rows of a
+// multiple of 32 entries then leave as whole lines.
 template <int KIND>
 __device__ __forceinline__ void init_walker(const DevGraph &g, const DevJob &J, uint32_t job,
-                                            uint32_t idx, Walker &s, LaneStats &ls)
+                                            uint32_t idx, Walker &s, LaneStats &ls, bool lines_ok = false)
 {
     s.job = job;
     s.idx = idx;
@@ -183,12 +200,19 @@ __device__ __forceinline__ void init_walker(const DevGraph &g, const DevJob &J, 
     s.beg = g.row_ptr[s.cur];
     s.end = g.row_ptr[s.cur + 1];
     ls.rows++;
-    s.pg.q0 = s.pg.q1 = s.pg.q2 = s.pg.q3 = s.cur;
+    s.lines = lines_ok && J.stride % LINE_ENTRIES == 0;
+    if (s.lines) lane_line<HALF_LINE_ENTRIES>().set(0, s.cur);
+    else s.pg.q0 = s.pg.q1 = s.pg.q2 = s.pg.q3 = s.cur;
 }
 
 __device__ __forceinline__ void finish_walker(const DevJob &J, const Walker &s, bool truncated)
 {
-    if ((s.step & 3u) != 3u) s.pg.store(J.paths + (uint64_t)s.idx * J.stride, s.step);
+    uint32_t *row = J.paths + (uint64_t)s.idx * J.stride;
+    if (s.lines) {
+        if ((s.step & (HALF_LINE_ENTRIES - 1)) != HALF_LINE_ENTRIES - 1) lane_line<HALF_LINE_ENTRIES>().store(row, s.step);
+    } else if ((s.step & 3u) != 3u) {
+        s.pg.store(row, s.step);
+    }
     J.lens[s.idx] = (s.step + 1) | (truncated ? 0x80000000u : 0u);
 }
 
@@ -197,8 +221,15 @@ __device__ __forceinline__ void move_to(const DevGraph &g, const DevJob &J, Walk
                                         uint32_t cand, LaneStats &ls, bool row_ok = false, uint64_t row = 0)
 {
     s.step++;
-    s.pg.set(s.step, cand);
-    if ((s.step & 3u) == 3u) s.pg.store(J.paths + (uint64_t)s.idx * J.stride, s.step);
+    if (s.lines) {
+        const auto lb = lane_line<HALF_LINE_ENTRIES>();
+        lb.set(s.step, cand);
+        if ((s.step & (HALF_LINE_ENTRIES - 1)) == HALF_LINE_ENTRIES - 1)
+            lb.store(J.paths + (uint64_t)s.idx * J.stride, s.step);
+    } else {
+        s.pg.set(s.step, cand);
+        if ((s.step & 3u) == 3u) s.pg.store(J.paths + (uint64_t)s.idx * J.stride, s.step);
+    }
     s.prev = s.cur;
     s.pbeg = s.beg;
     s.pend = s.end;
@@ -446,7 +477,8 @@ __device__ __forceinline__ uint32_t find_job(const DevJob *jobs, int n_jobs, uin
 }
 
 // SINGLE: one job (passed by value), otherwise a job table in device memory.
-template <int KIND, bool SINGLE, bool ALIAS_KNOWN, bool ALIAS, bool STATS>
+// LINES: launched with the LDS line buffer (jobs whose rows are whole lines use it).
+template <int KIND, bool SINGLE, bool ALIAS_KNOWN, bool ALIAS, bool STATS, bool LINES>
 __global__ void __launch_bounds__(256) k_walk_persistent(DevGraph g, DevJob J0, const DevJob *jobs,
                                                        int n_jobs, uint64_t n_chunks, uint32_t chunk,
                                                        unsigned long long *queue, DevStats *st)
@@ -481,7 +513,7 @@ __global__ void __launch_bounds__(256) k_walk_persistent(DevGraph g, DevJob J0, 
             uint32_t rank = (uint32_t)__popcll(need & lt_mask);
             if (!active && rank < avail) {
                 const DevJob &J = SINGLE ? J0 : jobs[pool_job];
-                init_walker<KIND>(g, J, pool_job, pool_next + rank, s, ls);
+                init_walker<KIND>(g, J, pool_job, pool_next + rank, s, ls, LINES);
                 active = true;
             }
             uint32_t want = (uint32_t)__popcll(need);
@@ -639,14 +671,15 @@ size_t pack_scan_bytes(uint64_t n)
 enum : uint32_t { PH_ROW = 0, PH_TRIAL = 1, PH_SEARCH = 2, PH_CROW = 3, PH_HASH = 4, PH_ALIAS = 5 };
 
 
-// LINES: rows of whole 128-byte lines, written through LineBuf (dynamic LDS of
-// LINE_LDS_STRIDE words per thread); otherwise 16-byte groups (PathGroup).
-template <int KIND, bool ALIAS, bool STATS, bool LINES>
+// LINE_N: rows written in chunks of LINE_N entries through LineBufN (dynamic
+// LDS of LINE_N + 1 words per thread); 0: 16-byte groups (PathGroup).
+template <int KIND, bool ALIAS, bool STATS, uint32_t LINE_N>
 __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t n_chunks, uint32_t chunk,
                                                  unsigned long long *queue, DevStats *st)
 {
-    extern __shared__ uint32_t sm_lines[];
-    LineBuf lb = {sm_lines + threadIdx.x * LINE_LDS_STRIDE};
+    constexpr bool LINES = LINE_N != 0;
+    constexpr uint32_t LN = LINES ? LINE_N : 4u;
+    const LineBufN<LN> lb = lane_line<LN>();
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     // candidates come from buckets: the alias tables, or the uniform buckets
@@ -930,7 +963,7 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             phase = PH_TRIAL;
             if (trial >= MAX_TRIALS) {
                 if (LINES) {
-                    if ((step & (LINE_ENTRIES - 1)) != LINE_ENTRIES - 1) lb.store(paths + (uint64_t)idx * stride, step);
+                    if ((step & (LN - 1)) != LN - 1) lb.store(paths + (uint64_t)idx * stride, step);
                 } else if ((step & 3u) != 3u) {
                     pg.store(paths + (uint64_t)idx * stride, step);
                 }
@@ -943,7 +976,7 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             step++;
             if (LINES) {
                 lb.set(step, cand);
-                if ((step & (LINE_ENTRIES - 1)) == LINE_ENTRIES - 1) lb.store(paths + (uint64_t)idx * stride, step);
+                if ((step & (LN - 1)) == LN - 1) lb.store(paths + (uint64_t)idx * stride, step);
             } else {
                 pg.set(step, cand);
                 if ((step & 3u) == 3u) pg.store(paths + (uint64_t)idx * stride, step);
@@ -978,7 +1011,7 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
         }
         if (stop) {
             if (LINES) {
-                if ((step & (LINE_ENTRIES - 1)) != LINE_ENTRIES - 1) lb.store(paths + (uint64_t)idx * stride, step);
+                if ((step & (LN - 1)) != LN - 1) lb.store(paths + (uint64_t)idx * stride, step);
             } else if ((step & 3u) != 3u) {
                 pg.store(paths + (uint64_t)idx * stride, step);
             }
@@ -1051,24 +1084,32 @@ static void launch_single_t(const DevGraph &g, const DevJob &J, uint64_t n_chunk
         const bool lines = J.stride % LINE_ENTRIES == 0;
         const size_t lds = lines ? (size_t)block * LINE_LDS_STRIDE * sizeof(uint32_t) : 0;
         if (stats && lines)
-            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, true, true>), dim3(grid), dim3(block), lds, stream, g, J,
-                               n_chunks, chunk, queue, st);
+            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, true, LINE_ENTRIES>), dim3(grid), dim3(block), lds, stream, g,
+                               J, n_chunks, chunk, queue, st);
         else if (stats)
-            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, true, false>), dim3(grid), dim3(block), 0, stream, g, J,
+            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, true, 0>), dim3(grid), dim3(block), 0, stream, g, J,
                                n_chunks, chunk, queue, st);
         else if (lines)
-            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, false, true>), dim3(grid), dim3(block), lds, stream, g, J,
-                               n_chunks, chunk, queue, st);
+            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, false, LINE_ENTRIES>), dim3(grid), dim3(block), lds, stream, g,
+                               J, n_chunks, chunk, queue, st);
         else
-            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, false, false>), dim3(grid), dim3(block), 0, stream, g, J,
+            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, false, 0>), dim3(grid), dim3(block), 0, stream, g, J,
                                n_chunks, chunk, queue, st);
         return;
     }
-    if (stats)
-        hipLaunchKernelGGL((k_walk_persistent<KIND, true, true, ALIAS, true>), dim3(grid), dim3(block), 0,
+    const bool lines = J.stride % LINE_ENTRIES == 0;
+    const size_t lds = lines ? (size_t)block * (HALF_LINE_ENTRIES + 1) * sizeof(uint32_t) : 0;
+    if (stats && lines)
+        hipLaunchKernelGGL((k_walk_persistent<KIND, true, true, ALIAS, true, true>), dim3(grid), dim3(block), lds,
+                           stream, g, J, (const DevJob *)nullptr, 1, n_chunks, chunk, queue, st);
+    else if (stats)
+        hipLaunchKernelGGL((k_walk_persistent<KIND, true, true, ALIAS, true, false>), dim3(grid), dim3(block), 0,
+                           stream, g, J, (const DevJob *)nullptr, 1, n_chunks, chunk, queue, st);
+    else if (lines)
+        hipLaunchKernelGGL((k_walk_persistent<KIND, true, true, ALIAS, false, true>), dim3(grid), dim3(block), lds,
                            stream, g, J, (const DevJob *)nullptr, 1, n_chunks, chunk, queue, st);
     else
-        hipLaunchKernelGGL((k_walk_persistent<KIND, true, true, ALIAS, false>), dim3(grid), dim3(block), 0,
+        hipLaunchKernelGGL((k_walk_persistent<KIND, true, true, ALIAS, false, false>), dim3(grid), dim3(block), 0,
                            stream, g, J, (const DevJob *)nullptr, 1, n_chunks, chunk, queue, st);
 }
 
@@ -1092,15 +1133,22 @@ void launch_walk_persistent_single(const DevGraph &g, const DevJob &J, uint64_t 
 // Mixed jobs from a device job table.
 void launch_walk_persistent_mixed(const DevGraph &g, const DevJob *jobs, int n_jobs, uint64_t n_chunks,
                                   uint32_t chunk, unsigned long long *queue, bool stats, DevStats *st,
-                                  int grid, int block, hipStream_t stream)
+                                  int grid, int block, bool lines, hipStream_t stream)
 {
     DevJob dummy{};
-    if (stats)
-        hipLaunchKernelGGL((k_walk_persistent<KIND_ANY, false, false, false, true>), dim3(grid), dim3(block),
-                           0, stream, g, dummy, jobs, n_jobs, n_chunks, chunk, queue, st);
+    const size_t lds = lines ? (size_t)block * (HALF_LINE_ENTRIES + 1) * sizeof(uint32_t) : 0;
+    if (stats && lines)
+        hipLaunchKernelGGL((k_walk_persistent<KIND_ANY, false, false, false, true, true>), dim3(grid), dim3(block),
+                           lds, stream, g, dummy, jobs, n_jobs, n_chunks, chunk, queue, st);
+    else if (stats)
+        hipLaunchKernelGGL((k_walk_persistent<KIND_ANY, false, false, false, true, false>), dim3(grid),
+                           dim3(block), 0, stream, g, dummy, jobs, n_jobs, n_chunks, chunk, queue, st);
+    else if (lines)
+        hipLaunchKernelGGL((k_walk_persistent<KIND_ANY, false, false, false, false, true>), dim3(grid),
+                           dim3(block), lds, stream, g, dummy, jobs, n_jobs, n_chunks, chunk, queue, st);
     else
-        hipLaunchKernelGGL((k_walk_persistent<KIND_ANY, false, false, false, false>), dim3(grid), dim3(block),
-                           0, stream, g, dummy, jobs, n_jobs, n_chunks, chunk, queue, st);
+        hipLaunchKernelGGL((k_walk_persistent<KIND_ANY, false, false, false, false, false>), dim3(grid),
+                           dim3(block), 0, stream, g, dummy, jobs, n_jobs, n_chunks, chunk, queue, st);
 }
 
 }  // namespace kkw
