@@ -237,7 +237,9 @@ typedef struct kkw_prog_job {
     uint32_t n_starts;
     const uint32_t *starts;  /* host array, KKW_START_LIST                                  */
     int32_t second_order;    /* 1: extension re-tested on every retry (walk.hpp:1161-1173)  */
-    uint32_t reserved;
+    int32_t mode;            /* KKW_MODE_PLAIN (0): one lane per walker for the launch;     */
+                             /* KKW_MODE_PERSISTENT: lanes take the next walker from a work */
+                             /* queue when theirs ends.  Paths are the same.                */
     float param[8];          /* kkw_ctx.param                                               */
 } kkw_prog_job;
 

@@ -294,9 +294,10 @@ struct WalkProgram {
     float param[8];
     bool use_weights;                            // static component: edge weights (else uniform)
     bool second_order;                           // re-test the extension on every retry
+    bool persistent;                             // lanes refilled from a work queue (same paths)
     step_t max_length;                           // path capacity in steps
 
-    WalkProgram() : use_weights(false), second_order(false), max_length(0)
+    WalkProgram() : use_weights(false), second_order(false), persistent(true), max_length(0)
     {
         for (int k = 0; k < 8; k++) param[k] = 0.0f;
     }
@@ -459,6 +460,7 @@ public:
         job.max_length = program->max_length;
         job.use_weights = program->use_weights ? 1 : 0;
         job.second_order = program->second_order ? 1 : 0;
+        job.mode = program->persistent ? KKW_MODE_PERSISTENT : KKW_MODE_PLAIN;
         job.seed = seed_ + calls_++;
         for (int k = 0; k < 8; k++) job.param[k] = program->param[k];
         const walker_id_t walker_num = walker_config->walker_num;

@@ -65,7 +65,7 @@ class _ProgJob(C.Structure):
     _fields_ = [
         ("walker_begin", C.c_uint32), ("walker_num", C.c_uint32), ("max_length", C.c_uint32),
         ("use_weights", C.c_int32), ("seed", C.c_uint64), ("start_mode", C.c_int32), ("n_starts", C.c_uint32),
-        ("starts", C.POINTER(C.c_uint32)), ("second_order", C.c_int32), ("reserved", C.c_uint32),
+        ("starts", C.POINTER(C.c_uint32)), ("second_order", C.c_int32), ("mode", C.c_int32),
         ("param", C.c_float * 8),
     ]
 
@@ -338,8 +338,11 @@ class Program:
 
     def run(self, walker_num: int, max_length: int, use_weights: bool = False, seed: int = 1,
             walker_begin: int = 0, start_vertices: Optional[np.ndarray] = None, uniform_start: bool = False,
-            second_order: bool = False, params: Sequence[float] = ()) -> Paths:
+            second_order: bool = False, params: Sequence[float] = (), mode: int = MODE_PLAIN) -> Paths:
+        """Walk with the program; mode MODE_PLAIN (one lane per walker) or
+        MODE_PERSISTENT (lanes refilled from a work queue): the same paths."""
         j = _ProgJob()
+        j.mode = mode
         j.walker_begin, j.walker_num, j.max_length = walker_begin, walker_num, max_length
         j.use_weights, j.seed = int(use_weights), seed
         starts = None

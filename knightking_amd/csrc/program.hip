@@ -28,7 +28,8 @@ static_assert(offsetof(kkw_prog_bucket, alias) == offsetof(Bucket, alias), "prog
 struct kkw_program {
     kkw_engine *eng = nullptr;
     hipModule_t mod = nullptr;
-    hipFunction_t fn = nullptr;
+    hipFunction_t fn = nullptr;     // plain form
+    hipFunction_t fn_p = nullptr;   // persistent form
     std::string log;
     bool uses_adjacency = false;   // the source calls kkw_adjacent: give it the hash sets
     void *arrays[KKW_PROGRAM_ARRAYS] = {};
@@ -120,6 +121,7 @@ int kkw_program_create(kkw_engine *eng, const char *source, const char *options,
         std::vector<char> code = compile(arch, source, options, p->log);
         KKW_HIP(hipModuleLoadData(&p->mod, code.data()));
         KKW_HIP(hipModuleGetFunction(&p->fn, p->mod, "kkw_program_walk"));
+        KKW_HIP(hipModuleGetFunction(&p->fn_p, p->mod, "kkw_program_walk_persistent"));
         *out = p.release();
         return KKW_OK;
     });
@@ -181,6 +183,8 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
         if ((uint64_t)job->walker_begin + job->walker_num > 0xFFFFFFFFull)
             return fail(KKW_E_INVALID, "walker ids exceed 32 bits");
         if (job->max_length >= 0x7FFFFFF0u) return fail(KKW_E_INVALID, "max_length too large");
+        if (job->mode != KKW_MODE_PLAIN && job->mode != KKW_MODE_PERSISTENT)
+            return fail(KKW_E_INVALID, "unknown mode");
         if (prog->uses_adjacency && eng->adj_index_auto && !gs.adjset) {
             double t0 = now_ms();
             eng->graph_version++;
@@ -206,7 +210,7 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
             out->priv = (void *)1;
             return KKW_OK;
         }
-        DevBuf paths((size_t)n * stride * 4), lens((size_t)n * 4);
+        DevBuf paths((size_t)n * stride * 4), lens((size_t)n * 4), queue(sizeof(unsigned long long));
         DevBuf starts(job->start_mode == KKW_START_LIST ? (size_t)job->n_starts * 4 : 0);
         if (job->start_mode == KKW_START_LIST)
             KKW_HIP(hipMemcpyAsync(starts.p, job->starts, (size_t)job->n_starts * 4, hipMemcpyHostToDevice,
@@ -236,13 +240,24 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
         A.retest_ext = job->second_order ? 1 : 0;
         A.paths = (uint32_t *)paths.p;
         A.lens = (uint32_t *)lens.p;
+        const uint32_t chunk = 64;
+        A.chunk = chunk;
+        A.n_chunks = ((uint64_t)n + chunk - 1) / chunk;
+        A.queue = (unsigned long long *)queue.p;
+        const bool persistent = job->mode == KKW_MODE_PERSISTENT;
+        // persistent: 8 blocks of 256 per CU, fewer when there are fewer work items
+        const uint64_t blocks_needed = (A.n_chunks * 64 + 255) / 256;
+        const uint32_t grid = persistent ? (uint32_t)std::min<uint64_t>((uint64_t)eng->cu_count * 8, blocks_needed)
+                                         : (n + 255) / 256;
+        if (persistent) KKW_HIP(hipMemsetAsync(queue.p, 0, sizeof(unsigned long long), eng->stream));
         void *args[] = {&A};
         hipEvent_t e0, e1;
         KKW_HIP(hipEventCreate(&e0));
         KKW_HIP(hipEventCreate(&e1));
         const double t0 = now_ms();
         KKW_HIP(hipEventRecord(e0, eng->stream));
-        KKW_HIP(hipModuleLaunchKernel(prog->fn, (n + 255) / 256, 1, 1, 256, 1, 1, 0, eng->stream, args, nullptr));
+        KKW_HIP(hipModuleLaunchKernel(persistent ? prog->fn_p : prog->fn, grid, 1, 1, 256, 1, 1, 0, eng->stream, args,
+                                      nullptr));
         KKW_HIP(hipEventRecord(e1, eng->stream));
         KKW_HIP(hipEventSynchronize(e1));
         float ms = 0.0f;

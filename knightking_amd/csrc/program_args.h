@@ -32,7 +32,9 @@ struct kkw_prog_args {
     uint32_t n_starts;
     const uint32_t *starts;
     int32_t retest_ext;          // extension re-tested on every retry (second order)
-    uint32_t reserved;
+    uint32_t chunk;              // persistent form: walkers per work item
     uint32_t *paths;             // [walker_num * stride]
     uint32_t *lens;              // [walker_num]: vertices in the path | 1 << 31 if cut
+    unsigned long long *queue;   // persistent form: next work item (zero at launch)
+    uint64_t n_chunks;           // persistent form: work items
 };

@@ -135,6 +135,18 @@ __device__ __forceinline__ bool kkw_prog_trial(const WalkerState &s, const kkw_w
     return true;
 }
 
+// Start vertex of walker id (get_equal_dist_func / a start list / a uniform draw).
+__device__ __forceinline__ uint32_t kkw_prog_start(const kkw_prog_args &A, uint32_t id)
+{
+    if (A.start_mode == 1) return A.starts[id % A.n_starts];
+    if (A.start_mode == 2) {
+        const kkw::u32x4 r = kkw::draw(A.seed, id, 0, 0, kkw::TAG_INIT);
+        return kkw::bounded(r.z, r.w, A.c.v_num);
+    }
+    return id % A.c.v_num;
+}
+
+// Plain form: one lane per walker for the whole launch.
 extern "C" __global__ void __launch_bounds__(256) kkw_program_walk(kkw_prog_args A)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -143,15 +155,7 @@ extern "C" __global__ void __launch_bounds__(256) kkw_program_walk(kkw_prog_args
     kkw_walker w;
     w.id = A.walker_begin + i;
     w.step = 0;
-    uint32_t cur;
-    if (A.start_mode == 1) {
-        cur = A.starts[w.id % A.n_starts];
-    } else if (A.start_mode == 2) {
-        const kkw::u32x4 r = kkw::draw(A.seed, w.id, 0, 0, kkw::TAG_INIT);
-        cur = kkw::bounded(r.z, r.w, c.v_num);
-    } else {
-        cur = w.id % c.v_num;
-    }
+    uint32_t cur = kkw_prog_start(A, w.id);
     WalkerState s;
     init_state(s, w, cur, c);
     uint32_t *path = A.paths + (uint64_t)i * A.stride;   // stride is a multiple of 4
@@ -189,4 +193,93 @@ extern "C" __global__ void __launch_bounds__(256) kkw_program_walk(kkw_prog_args
     }
     if ((w.step & 3u) != 3u) grp.store(path, w.step);   // the last, partial group
     A.lens[i] = (w.step + 1) | (capped ? 0x80000000u : 0u);
+}
+
+// Persistent form: a grid sized to the device; each wave takes work items of
+// A.chunk consecutive walkers from A.queue, and a lane whose walker has ended
+// takes the next walker of the wave's item (ballot + prefix count), so lanes
+// freed by short walks (isolated starts, early termination) are refilled at
+// once.  Each loop iteration runs one trial per active lane, with the same
+// draws as the plain form; a wave leaves when the queue is drained and its
+// lanes are idle, so every wave reaches the exit.
+extern "C" __global__ void __launch_bounds__(256) kkw_program_walk_persistent(kkw_prog_args A)
+{
+    const kkw_ctx &c = A.c;
+    const uint32_t lane = threadIdx.x & 63;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    bool active = false, drained = false;
+    uint32_t pool_next = 0, pool_end = 0;
+    uint32_t i = 0, cur = 0, deg = 0, trial = 0;
+    uint64_t beg = 0;
+    uint32_t *path = nullptr;
+    kkw_walker w;
+    w.id = 0;
+    w.step = 0;
+    WalkerState s;
+    kkw_prog_group grp;
+    for (;;) {
+        unsigned long long need = __ballot(!active);
+        while (need && !drained) {
+            if (pool_next >= pool_end) {
+                unsigned long long q = 0;
+                if (lane == 0) q = atomicAdd(A.queue, 1ull);
+                q = __shfl(q, 0, 64);
+                if (q >= A.n_chunks) {
+                    drained = true;
+                    break;
+                }
+                pool_next = (uint32_t)q * A.chunk;
+                const uint32_t e = pool_next + A.chunk;
+                pool_end = e < A.walker_num ? e : A.walker_num;
+            }
+            const uint32_t avail = pool_end - pool_next;
+            const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
+            if (!active && rank < avail) {
+                i = pool_next + rank;
+                w.id = A.walker_begin + i;
+                w.step = 0;
+                cur = kkw_prog_start(A, w.id);
+                init_state(s, w, cur, c);
+                path = A.paths + (uint64_t)i * A.stride;
+                grp.v[0] = grp.v[1] = grp.v[2] = grp.v[3] = cur;
+                kkw_prog_row(c.row_ptr, cur, beg, deg);
+                trial = 0;
+                active = true;
+            }
+            const uint32_t want = (uint32_t)__popcll(need);
+            pool_next += want < avail ? want : avail;
+            need = __ballot(!active);
+        }
+        if (__ballot(active) == 0ull) break;
+        if (active) {
+            bool done = false, capped = false;
+            if (trial == 0 || A.retest_ext) {
+                if (kkw_prog_terminates(s, w, cur, deg, A.retest_ext ? trial : 0u, A)) {
+                    done = true;
+                } else if (w.step >= A.max_len) {
+                    done = capped = true;
+                }
+            }
+            if (!done && trial >= KKW_PROG_MAX_TRIALS) done = capped = true;
+            if (!done) {
+                kkw_edge e;
+                if (kkw_prog_trial(s, w, cur, beg, deg, trial, A, e)) {
+                    w.step++;
+                    update_state(s, w, cur, e, c);
+                    cur = e.neighbour;
+                    grp.v[w.step & 3u] = cur;
+                    if ((w.step & 3u) == 3u) grp.store(path, w.step);
+                    kkw_prog_row(c.row_ptr, cur, beg, deg);
+                    trial = 0;
+                } else {
+                    trial++;
+                }
+            }
+            if (done) {
+                if ((w.step & 3u) != 3u) grp.store(path, w.step);   // the last, partial group
+                A.lens[i] = (w.step + 1) | (capped ? 0x80000000u : 0u);
+                active = false;
+            }
+        }
+    }
 }

@@ -13,6 +13,9 @@ from test_oracle_programs import TAGS, program_path, same_paths, tagged_graph
 
 pytestmark = pytest.mark.gpu
 
+# both launch forms of a program: one lane per walker, and lanes refilled from a work queue
+MODES = [pytest.param(kk.MODE_PLAIN, id="plain"), pytest.param(kk.MODE_PERSISTENT, id="persistent")]
+
 
 @pytest.fixture(scope="module")
 def build_dir(tmp_path_factory):
@@ -53,43 +56,48 @@ def check(got, want):
     assert got.steps == int((lw.astype(np.int64) - 1).sum())
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("weighted", [True, False])
-def test_deepwalk_program(engine, build_dir, weighted):
+def test_deepwalk_program(engine, build_dir, weighted, mode):
     rp, col, w = O.rmat_graph(12, 16, seed=3, weighted=True)
     check(*both(engine, build_dir, "deepwalk", rp, col, w, walker_num=20000, max_length=40, params=[40],
-                use_weights=weighted, seed=9))
+                use_weights=weighted, seed=9, mode=mode))
 
 
-def test_ppr_program_uniform_start(engine, build_dir):
+@pytest.mark.parametrize("mode", MODES)
+def test_ppr_program_uniform_start(engine, build_dir, mode):
     rp, col, w = O.rmat_graph(12, 16, seed=4, weighted=True)
     t = 0.15
     ext = np.float32(1.0) - np.float32(t)
     check(*both(engine, build_dir, "ppr", rp, col, w, walker_num=30000, max_length=O.ppr_default_cap(t),
-                params=[ext], use_weights=True, seed=2, uniform_start=True, walker_begin=77))
+                params=[ext], use_weights=True, seed=2, uniform_start=True, walker_begin=77, mode=mode))
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("p,q,weighted", [(2.0, 0.5, True), (0.5, 2.0, True), (0.25, 1.0, False)])
-def test_node2vec_program_matches_oracle_and_builtin_kernel(engine, build_dir, p, q, weighted):
+def test_node2vec_program_matches_oracle_and_builtin_kernel(engine, build_dir, p, q, weighted, mode):
     rp, col, w = O.rmat_graph(12, 16, seed=5, weighted=True)
     L = 40
     up, lo, ip, iq, ov = O.node2vec_constants(p, q)
     got, want = both(engine, build_dir, "node2vec", rp, col, w, walker_num=8192, max_length=L,
                      params=[L, up, lo, ip, iq, ov, 1.0 if weighted else 0.0], use_weights=weighted, seed=11,
-                     second_order=True)
+                     second_order=True, mode=mode)
     check(got, want)
     (builtin,) = engine.run([kk.node2vec_job(8192, L, p, q, weighted, seed=11)], mode=kk.MODE_PLAIN)
     same_paths(got.vertices, got.lengths, builtin.vertices, builtin.lengths)
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("weighted", [True, False])
-def test_tag_walk_program(engine, build_dir, weighted):
+def test_tag_walk_program(engine, build_dir, weighted, mode):
     rp, col, w, et, vt = tagged_graph(300, 2000, 21, weighted)
     check(*both(engine, build_dir, "tag_walk", rp, col, w, et, arrays=[vt], walker_num=20000, max_length=20,
-                params=[20, 0.5], use_weights=weighted, seed=3))
+                params=[20, 0.5], use_weights=weighted, seed=3, mode=mode))
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("weighted", [True, False])
-def test_tag_outlier_program(engine, build_dir, weighted):
+def test_tag_outlier_program(engine, build_dir, weighted, mode):
     V = 300
     rp, col, w, et, vt = tagged_graph(V, 2000, 22, weighted, sort_by_tag=True)
     begin = np.zeros(V * TAGS, np.uint64)
@@ -101,14 +109,15 @@ def test_tag_outlier_program(engine, build_dir, weighted):
             begin[v * TAGS + t] = b + (idx[0] if len(idx) else 0)
             end[v * TAGS + t] = b + (idx[-1] + 1 if len(idx) else 0)
     check(*both(engine, build_dir, "tag_outlier", rp, col, w, et, arrays=[vt, begin, end], walker_num=20000,
-                max_length=20, params=[20, 3.0, 1.0 if weighted else 0.0], use_weights=weighted, seed=4))
+                max_length=20, params=[20, 3.0, 1.0 if weighted else 0.0], use_weights=weighted, seed=4, mode=mode))
 
 
-def test_return_bias_program_start_list_and_cap(engine, build_dir, karate):
+@pytest.mark.parametrize("mode", MODES)
+def test_return_bias_program_start_list_and_cap(engine, build_dir, karate, mode):
     src, dst, w = karate
     rp, col, ww = O.csr_from_edges(34, src, dst, w, False, True)
     check(*both(engine, build_dir, "return_bias", rp, col, ww, walker_num=5000, max_length=7,
-                params=[1000], use_weights=True, seed=6, start_vertices=np.array([0, 33, 5], np.uint32)))
+                params=[1000], use_weights=True, seed=6, start_vertices=np.array([0, 33, 5], np.uint32), mode=mode))
 
 
 def test_program_errors(engine):
@@ -127,5 +136,10 @@ def test_program_errors(engine):
         assert e.value.code == kk.KKW_E_INVALID
         got = prog.run(0, 5, params=[5])
         assert got.lengths.shape == (0,)
+        got = prog.run(0, 5, params=[5], mode=kk.MODE_PERSISTENT)
+        assert got.lengths.shape == (0,)
+        with pytest.raises(kk.KkwError) as e:
+            prog.run(10, 5, params=[5], mode=7)
+        assert e.value.code == kk.KKW_E_INVALID
     finally:
         prog.close()
