@@ -199,7 +199,13 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
             eng->timing.alias_ms = now_ms() - t0;
         }
         const uint32_t n = job->walker_num;
-        const uint32_t stride = round_up4((uint64_t)job->max_length + 1);
+        const bool persistent = job->mode == KKW_MODE_PERSISTENT;
+        // persistent rows are padded to whole 128-byte lines when that adds at
+        // most a quarter (the kernel then writes 64-byte half lines from LDS)
+        const uint64_t len = (uint64_t)job->max_length + 1;
+        const uint64_t line_stride = (len + 31) & ~(uint64_t)31;
+        const bool line_rows = persistent && len >= 32 && (line_stride - len) * 4 <= len;
+        const uint32_t stride = line_rows ? (uint32_t)line_stride : round_up4(len);
         memset(out, 0, sizeof(*out));
         out->walker_begin = job->walker_begin;
         out->walker_num = n;
@@ -244,7 +250,7 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
         A.chunk = chunk;
         A.n_chunks = ((uint64_t)n + chunk - 1) / chunk;
         A.queue = (unsigned long long *)queue.p;
-        const bool persistent = job->mode == KKW_MODE_PERSISTENT;
+        A.line_rows = line_rows ? 1u : 0u;
         // persistent: 8 blocks of 256 per CU, fewer when there are fewer work items
         const uint64_t blocks_needed = (A.n_chunks * 64 + 255) / 256;
         const uint32_t grid = persistent ? (uint32_t)std::min<uint64_t>((uint64_t)eng->cu_count * 8, blocks_needed)
@@ -256,8 +262,9 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
         KKW_HIP(hipEventCreate(&e1));
         const double t0 = now_ms();
         KKW_HIP(hipEventRecord(e0, eng->stream));
-        KKW_HIP(hipModuleLaunchKernel(persistent ? prog->fn_p : prog->fn, grid, 1, 1, 256, 1, 1, 0, eng->stream, args,
-                                      nullptr));
+        const unsigned lds = line_rows ? 256u * 17u * 4u : 0u;   // 17 words per lane
+        KKW_HIP(hipModuleLaunchKernel(persistent ? prog->fn_p : prog->fn, grid, 1, 1, 256, 1, 1, lds, eng->stream,
+                                      args, nullptr));
         KKW_HIP(hipEventRecord(e1, eng->stream));
         KKW_HIP(hipEventSynchronize(e1));
         float ms = 0.0f;

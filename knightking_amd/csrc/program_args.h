@@ -37,4 +37,6 @@ struct kkw_prog_args {
     uint32_t *lens;              // [walker_num]: vertices in the path | 1 << 31 if cut
     unsigned long long *queue;   // persistent form: next work item (zero at launch)
     uint64_t n_chunks;           // persistent form: work items
+    uint32_t line_rows;          // persistent form: rows of whole lines, written from LDS
+    uint32_t pad;
 };

@@ -195,6 +195,23 @@ extern "C" __global__ void __launch_bounds__(256) kkw_program_walk(kkw_prog_args
     A.lens[i] = (w.step + 1) | (capped ? 0x80000000u : 0u);
 }
 
+// Path write-out of the persistent form when rows are whole 128-byte lines
+// (A.line_rows): the lane's current 16 entries (a 64-byte half line) wait in
+// LDS, 17 words per lane, and leave as four 16-byte stores when the half line
+// fills; the half line that ends a walk is written up to its last 16-byte
+// group.  Half lines reach HBM whole instead of 16 bytes at a time.
+extern __shared__ uint32_t kkw_prog_lds[];
+#define KKW_PROG_HALF_LINE 16u
+
+__device__ __forceinline__ void kkw_prog_flush(const uint32_t *buf, uint32_t *path, uint32_t step)
+{
+    uint4 *dst = reinterpret_cast<uint4 *>(path + (step & ~(KKW_PROG_HALF_LINE - 1)));
+    const uint32_t groups = ((step & (KKW_PROG_HALF_LINE - 1)) >> 2) + 1;
+#pragma unroll
+    for (uint32_t k = 0; k < KKW_PROG_HALF_LINE / 4; k++)
+        if (k < groups) dst[k] = make_uint4(buf[4 * k], buf[4 * k + 1], buf[4 * k + 2], buf[4 * k + 3]);
+}
+
 // Persistent form: a grid sized to the device; each wave takes work items of
 // A.chunk consecutive walkers from A.queue, and a lane whose walker has ended
 // takes the next walker of the wave's item (ballot + prefix count), so lanes
@@ -217,6 +234,8 @@ extern "C" __global__ void __launch_bounds__(256) kkw_program_walk_persistent(kk
     w.step = 0;
     WalkerState s;
     kkw_prog_group grp;
+    uint32_t *const lbuf = kkw_prog_lds + threadIdx.x * (KKW_PROG_HALF_LINE + 1);
+    const bool lines = A.line_rows != 0;
     for (;;) {
         unsigned long long need = __ballot(!active);
         while (need && !drained) {
@@ -241,7 +260,8 @@ extern "C" __global__ void __launch_bounds__(256) kkw_program_walk_persistent(kk
                 cur = kkw_prog_start(A, w.id);
                 init_state(s, w, cur, c);
                 path = A.paths + (uint64_t)i * A.stride;
-                grp.v[0] = grp.v[1] = grp.v[2] = grp.v[3] = cur;
+                if (lines) lbuf[0] = cur;
+                else grp.v[0] = grp.v[1] = grp.v[2] = grp.v[3] = cur;
                 kkw_prog_row(c.row_ptr, cur, beg, deg);
                 trial = 0;
                 active = true;
@@ -267,8 +287,14 @@ extern "C" __global__ void __launch_bounds__(256) kkw_program_walk_persistent(kk
                     w.step++;
                     update_state(s, w, cur, e, c);
                     cur = e.neighbour;
-                    grp.v[w.step & 3u] = cur;
-                    if ((w.step & 3u) == 3u) grp.store(path, w.step);
+                    if (lines) {
+                        lbuf[w.step & (KKW_PROG_HALF_LINE - 1)] = cur;
+                        if ((w.step & (KKW_PROG_HALF_LINE - 1)) == KKW_PROG_HALF_LINE - 1)
+                            kkw_prog_flush(lbuf, path, w.step);
+                    } else {
+                        grp.v[w.step & 3u] = cur;
+                        if ((w.step & 3u) == 3u) grp.store(path, w.step);
+                    }
                     kkw_prog_row(c.row_ptr, cur, beg, deg);
                     trial = 0;
                 } else {
@@ -276,7 +302,11 @@ extern "C" __global__ void __launch_bounds__(256) kkw_program_walk_persistent(kk
                 }
             }
             if (done) {
-                if ((w.step & 3u) != 3u) grp.store(path, w.step);   // the last, partial group
+                if (lines) {
+                    if ((w.step & (KKW_PROG_HALF_LINE - 1)) != KKW_PROG_HALF_LINE - 1) kkw_prog_flush(lbuf, path, w.step);
+                } else if ((w.step & 3u) != 3u) {
+                    grp.store(path, w.step);   // the last, partial group
+                }
                 A.lens[i] = (w.step + 1) | (capped ? 0x80000000u : 0u);
                 active = false;
             }

@@ -219,6 +219,9 @@ WALK_CASES = {
     "n2v_uniform_start": dict(kind=kk.NODE2VEC, walker_num=2000, walk_length=20, p=4.0, q=1.0, seed=20,
                               start_mode=kk.START_UNIFORM),
     "deepwalk_epoch": dict(kind=kk.DEEPWALK, walker_num=1500, walk_length=25, seed=21, walker_begin=1500),
+    # L = 80: persistent rows of three whole 128-byte lines (line-buffered write-out)
+    "deepwalk_w_l80": dict(kind=kk.DEEPWALK, walker_num=3000, walk_length=80, seed=24),
+    "n2v_u_l80": dict(kind=kk.NODE2VEC, walker_num=3000, walk_length=80, p=0.5, q=2.0, seed=25, use_weights=False),
 }
 
 
@@ -345,6 +348,7 @@ def test_mixed_batch_equals_single_jobs(engine):
         kk.node2vec_job(300, 15, 2.0, 0.5, False, seed=35),
         kk.deepwalk_job(1, 10, False, seed=36),
         kk.deepwalk_job(0, 10, False, seed=37),
+        kk.deepwalk_job(800, 63, True, seed=38),   # 64-entry rows: half-line write-out
     ]
     mixed = engine.run(jobs, mode=kk.MODE_PERSISTENT)
     for j, m in zip(jobs, mixed):

@@ -57,10 +57,11 @@ def check(got, want):
 
 
 @pytest.mark.parametrize("mode", MODES)
-@pytest.mark.parametrize("weighted", [True, False])
-def test_deepwalk_program(engine, build_dir, weighted, mode):
+@pytest.mark.parametrize("weighted,L", [(True, 40), (False, 40), (True, 80)])
+def test_deepwalk_program(engine, build_dir, weighted, L, mode):
+    # L = 80: persistent rows of whole lines (half-line write-out from LDS)
     rp, col, w = O.rmat_graph(12, 16, seed=3, weighted=True)
-    check(*both(engine, build_dir, "deepwalk", rp, col, w, walker_num=20000, max_length=40, params=[40],
+    check(*both(engine, build_dir, "deepwalk", rp, col, w, walker_num=20000, max_length=L, params=[L],
                 use_weights=weighted, seed=9, mode=mode))
 
 
