@@ -129,11 +129,13 @@ public:
     bool print_with_head_info;
     bool output_consumer_flag;
     std::function<void(PathSet *)> output_consumer_func;
+    bool device_consumer_flag;
+    std::function<void(const kkw_paths &)> device_consumer_func;
     double rate;
 
     WalkConfig()
         : output_file_flag(false), print_with_head_info(false), output_consumer_flag(false),
-          output_consumer_func(nullptr), rate(1)
+          output_consumer_func(nullptr), device_consumer_flag(false), device_consumer_func(nullptr), rate(1)
     {
     }
     void set_output_file(const char *prefix, bool with_head_info = false)
@@ -148,6 +150,16 @@ public:
         assert(f != nullptr);
         output_consumer_flag = true;
         output_consumer_func = f;
+    }
+    // No counterpart in the reference: each epoch's paths handed over where
+    // they were written, in HBM (on_device = 1, rows of `stride` entries,
+    // lengths with bit 31 marking a cap), for a consumer on the GPU such as a
+    // skip-gram trainer.  The pointers are valid during the call only.
+    void set_device_output_consumer(std::function<void(const kkw_paths &)> f)
+    {
+        assert(f != nullptr);
+        device_consumer_flag = true;
+        device_consumer_func = f;
     }
     void set_walk_rate(double r)
     {
@@ -419,10 +431,24 @@ public:
             remained -= n;
             job.walker_begin = begin;
             job.walker_num = n;
-            kkw_paths out{};
-            knightking_detail::check(kkw_run(eng_, &job, 1, KKW_MODE_PERSISTENT, KKW_OUT_HOST, 0, &out));
+            kkw_batch *b = nullptr;
+            knightking_detail::check(kkw_batch_create(eng_, &job, 1, KKW_MODE_PERSISTENT, 0, &b));
+            struct BatchHolder {
+                kkw_batch *b;
+                ~BatchHolder() { kkw_batch_destroy(b); }
+            } hold{b};
+            knightking_detail::check(kkw_batch_launch(b));
+            knightking_detail::check(kkw_batch_sync(b, nullptr));
+            if (wc->device_consumer_flag) {
+                kkw_paths dev{};
+                knightking_detail::check(kkw_batch_paths(b, 0, KKW_OUT_DEVICE, &dev));
+                wc->device_consumer_func(dev);
+            }
             if (wc->output_file_flag || wc->output_consumer_flag) {
+                kkw_paths out{};
+                knightking_detail::check(kkw_batch_paths(b, 0, KKW_OUT_HOST, &out));
                 PathSet *ps = to_pathset(out);
+                kkw_paths_free(&out);
                 if (wc->output_file_flag) {
                     std::string p = wc->output_path_prefix + ".0";
                     ps->dump(p.c_str(), iter == 0 ? "w" : "a", wc->print_with_head_info);
@@ -430,7 +456,6 @@ public:
                 if (wc->output_consumer_flag) wc->output_consumer_func(ps);
                 else delete ps;
             }
-            kkw_paths_free(&out);
             iter++;
         }
     }
@@ -443,6 +468,8 @@ public:
         WalkConfig local;
         WalkConfig *wc = walk_config_param ? walk_config_param : &local;
         if (program->max_length == 0) throw std::invalid_argument("WalkProgram: max_length must be set");
+        if (wc->device_consumer_flag)
+            throw std::invalid_argument("WalkProgram: paths come back to the host; no device consumer");
         kkw_program *prog = nullptr;
         knightking_detail::check(kkw_program_create(eng_, program->source.c_str(),
                                                     program->options.empty() ? nullptr : program->options.c_str(),

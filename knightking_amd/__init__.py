@@ -373,6 +373,16 @@ class Program:
             pass
 
 
+class _DeviceArray:
+    """int32 device array in the __cuda_array_interface__ form torch.as_tensor
+    reads; holds `owner` so the memory outlives the tensor."""
+
+    def __init__(self, ptr: int, shape, owner):
+        self._owner = owner
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": "<i4", "data": (ptr, False),
+                                         "version": 2, "strides": None}
+
+
 class Batch:
     """Resident batch (kkw_batch): job table, path buffers, optional hipGraph."""
 
@@ -406,6 +416,33 @@ class Batch:
         _check(_lib.kkw_batch_paths(self.h, i, OUT_DEVICE, C.byref(p)))
         return int(p.steps)
 
+    def paths_torch(self, i: int):
+        """Job i's paths where the kernel wrote them, as torch tensors on the
+        engine's GPU sharing the batch's memory (no copy): vertices int32
+        [walker_num, stride] (ids read as int32), lengths int32 [walker_num]
+        (bit 31 marks a capped path, so such lengths read negative).  For a
+        consumer on the GPU (e.g. skip-gram pairs).  The tensors keep the
+        batch object alive, hold the last launch's paths until the next
+        launch, and must not be used once the batch or its engine is closed.
+
+        torch bundles its own HIP runtime; one process can drive the GPU through
+        only one of the two, so import torch and initialise its GPU before the
+        first Engine: libkkw then binds to the runtime torch loaded."""
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("torch does not see the GPU in this process: import torch and call "
+                               "torch.cuda.init() before the first Engine, so that libkkw and torch "
+                               "share one HIP runtime")
+        p = _Paths()
+        _check(_lib.kkw_batch_paths(self.h, i, OUT_DEVICE, C.byref(p)))
+        n, s = int(p.walker_num), int(p.stride)
+        dev = torch.device("cuda", self.eng.device)
+        if n == 0:
+            return (torch.empty((0, s), dtype=torch.int32, device=dev), torch.empty(0, dtype=torch.int32, device=dev))
+        verts = _DeviceArray(C.cast(p.vertices, C.c_void_p).value, (n, s), self)
+        lens = _DeviceArray(C.cast(p.lengths, C.c_void_p).value, (n,), self)
+        return torch.as_tensor(verts, device=dev), torch.as_tensor(lens, device=dev)
+
     def autotune(self, cache_file: Optional[str] = None):
         b, bp, ch = C.c_int(), C.c_int(), C.c_int()
         _check(_lib.kkw_autotune(self.h, cache_file.encode() if cache_file else None,
@@ -433,6 +470,7 @@ class Engine:
         h = C.c_void_p()
         _check(_lib.kkw_engine_create(C.byref(o), C.byref(h)))
         self.h = h
+        self.device = device
         # batches and programs hold the engine's handle: closed before it is
         self._children = weakref.WeakSet()
 

@@ -1,9 +1,12 @@
 // Drives the C++ facade the way KnightKing's apps and examples do: load a text
 // edge list, run node2vec / deepwalk (two epochs) / ppr (start list) / a
 // std::function-only config (must be refused) / a WalkProgram (the dynamic
-// walk example as device code), dump PathSets to files.
+// walk example as device code), dump PathSets to files; node2vec's paths are
+// also taken in HBM by a device consumer and written out after a copy.
 // usage: facade_karate <karate.txt> <out_prefix> <program.cpp>
 #include <stdio.h>
+
+#include <hip/hip_runtime_api.h>
 
 #include <fstream>
 #include <sstream>
@@ -27,7 +30,29 @@ int main(int argc, char **argv)
         conf.walk_length = 10;
         WalkConfig wc;
         wc.set_output_file((out + "_n2v").c_str());
+        int dev_calls = 0;
+        uint32_t dev_stride = 0;
+        std::vector<uint32_t> rows, lens;
+        wc.set_device_output_consumer([&](const kkw_paths &p) {
+            dev_calls++;
+            if (!p.on_device) return;
+            dev_stride = p.stride;
+            rows.resize((size_t)p.walker_num * p.stride);
+            lens.resize(p.walker_num);
+            if (hipMemcpy(rows.data(), p.vertices, rows.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(lens.data(), p.lengths, lens.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+                rows.clear();
+        });
         node2vec(&graph, conf, &wc);   // engine seed 1, first call
+        if (dev_calls != 1 || lens.size() != 34 || rows.empty()) return 5;
+        FILE *f = fopen((out + "_n2v_dev.0").c_str(), "w");
+        if (!f) return 6;
+        for (size_t i = 0; i < lens.size(); i++) {
+            for (uint32_t k = 0; k < (lens[i] & 0x7fffffffu); k++)
+                fprintf(f, k ? " %u" : "%u", rows[i * dev_stride + k]);
+            fprintf(f, "\n");
+        }
+        fclose(f);
     }
     {
         WalkEngine<real_t, EmptyData> graph;

@@ -15,8 +15,11 @@ SRC = os.path.join(ROOT, "tests", "cpp", "facade_karate.cpp")
 
 def build(tmp_path):
     exe = str(tmp_path / "facade_karate")
-    subprocess.run(["g++", "-std=c++11", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), SRC,
+    # the HIP runtime API is only for the test's own copy of the device-consumer paths
+    subprocess.run(["g++", "-std=c++11", "-O1", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__",
+                    "-isystem", "/opt/rocm/include", "-I", os.path.join(ROOT, "include"), SRC,
                     "-o", exe, "-L", os.path.join(ROOT, "knightking_amd"), "-lkkw",
+                    "-L", "/opt/rocm/lib", "-lamdhip64",
                     "-Wl,-rpath," + os.path.join(ROOT, "knightking_amd")], check=True)
     return exe
 
@@ -45,10 +48,11 @@ def test_facade_matches_oracle(tmp_path, karate):
     rp, col, ww = O.csr_from_edges(34, src, dst, w, False, True)
     g = O.Graph(rp, col, ww)
     paths, lens, _, _ = O.walk(g, O.NODE2VEC, 34, 10, True, p=2.0, q=0.5, seed=1)
-    got = read_dump(prefix + "_n2v.0", False)
-    assert len(got) == 34
-    for i, (_, n, verts) in enumerate(got):
-        assert verts == paths[i, :lens[i]].tolist()
+    for name in ("_n2v.0", "_n2v_dev.0"):   # PathSet dump, and the device consumer's rows
+        got = read_dump(prefix + name, False)
+        assert len(got) == 34
+        for i, (_, n, verts) in enumerate(got):
+            assert verts == paths[i, :lens[i]].tolist(), name
     rp2, col2, w2 = O.csr_from_edges(34, src, dst, w, False, False)
     g2 = O.Graph(rp2, col2, w2)
     paths, lens, _, _ = O.walk(g2, O.DEEPWALK, 34, 12, True, seed=1)

@@ -4,6 +4,8 @@ Graph construction (R-MAT, edge lists), alias tables and every walk kind in
 both launch forms are compared with the CPU restatement on the same seeded
 inputs.  Larger sizes are covered by size-independent properties.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -465,3 +467,43 @@ def test_node2vec_with_and_without_hash_sets(engine, index):
             (gpu,) = engine.run([kk.node2vec_job(4096, 40, 2.0, 0.5, weighted, seed=61)], mode=mode)
             assert_paths_equal(gpu, op, ol, ot, f"index={index} weighted={weighted} mode={mode}")
     engine.adj_index(True)
+
+
+TORCH_HANDOFF = r"""
+import sys
+import numpy as np
+import torch
+torch.cuda.init()                     # torch's HIP runtime first: libkkw binds to it
+sys.path.insert(0, sys.argv[1])
+import knightking_amd as kk
+eng = kk.Engine()
+eng.graph_rmat(10, 16, seed=1, weighted=True)
+b = eng.batch([kk.node2vec_job(3000, 80, 2.0, 0.5, True, seed=61)], kk.MODE_PERSISTENT)
+b.launch()
+b.sync()
+host = b.paths(0)
+tv, tl = b.paths_torch(0)
+assert tv.is_cuda and tv.dtype == torch.int32 and tuple(tv.shape) == host.vertices.shape
+assert np.array_equal((tl & 0x7fffffff).cpu().numpy().view(np.uint32), host.lengths)
+hmask = np.arange(host.vertices.shape[1])[None, :] < host.lengths[:, None]
+assert np.array_equal(tv.cpu().numpy().view(np.uint32)[hmask], host.vertices[hmask])
+mask = torch.arange(tv.shape[1], device=tv.device)[None, :] < (tl & 0x7fffffff)[:, None]
+visits = torch.bincount(tv[mask].long(), minlength=1024).cpu().numpy()
+assert np.array_equal(visits, np.bincount(host.vertices[hmask].astype(np.int64), minlength=1024))
+del tv, tl
+b.close()
+eng.close()
+print("handoff ok")
+"""
+
+
+def test_device_paths_as_torch_tensors():
+    """Batch paths handed to torch where the kernel wrote them (no copy): the
+    same rows as the host copy, usable by torch ops on the GPU (visit counts).
+    Its own process, with torch's GPU initialised before the engine."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    print("[gpu] torch handoff in a child process", flush=True)
+    r = subprocess.run([sys.executable, "-c", TORCH_HANDOFF, root], capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0 and "handoff ok" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
