@@ -345,19 +345,33 @@ __global__ void k_etype_by_dst(const uint32_t *col, uint64_t E, const uint8_t *v
     etype[k] = vtype[col[k]];
 }
 
-// vertex_masks[v] = OR of 1 << meta(e) over out-edges (src/apps/metapath.hpp:77-88)
-__global__ void k_vmask(const uint64_t *row_ptr, const uint8_t *etype, uint32_t V, uint8_t *vmask,
-                        unsigned long long *bad)
+// vertex_masks[v] = OR of 1 << meta(e) over out-edges (src/apps/metapath.hpp:77-88).
+// One wave per vertex: lanes stride the list and the lane masks meet by a
+// butterfly of shuffles (one lane per vertex would leave a hub's list of 10^5
+// edges to a single lane).  Types of 8 and above are counted in *bad.
+__global__ void __launch_bounds__(256) k_vmask(const uint64_t *row_ptr, const uint8_t *etype, uint32_t V,
+                                               uint8_t *vmask, unsigned long long *bad)
 {
-    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    uint32_t m = 0;
-    for (uint64_t k = row_ptr[v]; k < row_ptr[v + 1]; k++) {
-        uint32_t t = etype[k];
-        if (t >= 8) atomicAdd(bad, 1ull);
-        m |= 1u << (t & 7u);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    // v is the same for all lanes of a wave: the shuffles see every lane
+    for (uint64_t v = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); v < V; v += waves) {
+        const uint64_t b = row_ptr[v], e = row_ptr[v + 1];
+        uint32_t m = 0, nbad = 0;
+        for (uint64_t k = b + lane; k < e; k += 64) {
+            const uint32_t t = etype[k];
+            nbad += t >= 8 ? 1u : 0u;
+            m |= 1u << (t & 7u);
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            m |= (uint32_t)__shfl_xor((int)m, off, 64);
+            nbad += (uint32_t)__shfl_xor((int)nbad, off, 64);
+        }
+        if (lane == 0) {
+            vmask[v] = (uint8_t)m;
+            if (nbad) atomicAdd(bad, (unsigned long long)nbad);
+        }
     }
-    vmask[v] = (uint8_t)m;
 }
 
 // -------------------------------------------------------------- alias ----
@@ -930,7 +944,8 @@ static void rebuild_vmask(GraphStore &gs, hipStream_t s)
     DArr<uint8_t> vm(gs.V);
     DArr<unsigned long long> bad(1);
     KKW_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned long long), s));
-    hipLaunchKernelGGL(k_vmask, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.etype, gs.V,
+    const uint32_t vm_blocks = (uint32_t)std::min<uint64_t>(((uint64_t)gs.V + 3) / 4, 1u << 16);   // 4 waves each
+    hipLaunchKernelGGL(k_vmask, dim3(std::max(vm_blocks, 1u)), dim3(256), 0, s, gs.row_ptr, gs.etype, gs.V,
                        vm.p, bad.p);
     KKW_HIP(hipGetLastError());
     if (read_counter(bad.p, s))
