@@ -252,9 +252,9 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
         A.queue = (unsigned long long *)queue.p;
         A.line_rows = line_rows ? 1u : 0u;
         // persistent: 8 blocks of 256 per CU, fewer when there are fewer work items
-        const uint64_t blocks_needed = (A.n_chunks * 64 + 255) / 256;
+        const uint64_t blocks_needed = (A.n_chunks * 64 + KKW_PROG_BLOCK - 1) / KKW_PROG_BLOCK;
         const uint32_t grid = persistent ? (uint32_t)std::min<uint64_t>((uint64_t)eng->cu_count * 8, blocks_needed)
-                                         : (n + 255) / 256;
+                                         : (n + KKW_PROG_BLOCK - 1) / KKW_PROG_BLOCK;
         if (persistent) KKW_HIP(hipMemsetAsync(queue.p, 0, sizeof(unsigned long long), eng->stream));
         void *args[] = {&A};
         hipEvent_t e0, e1;
@@ -262,9 +262,9 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
         KKW_HIP(hipEventCreate(&e1));
         const double t0 = now_ms();
         KKW_HIP(hipEventRecord(e0, eng->stream));
-        const unsigned lds = line_rows ? 256u * 17u * 4u : 0u;   // 17 words per lane
-        KKW_HIP(hipModuleLaunchKernel(persistent ? prog->fn_p : prog->fn, grid, 1, 1, 256, 1, 1, lds, eng->stream,
-                                      args, nullptr));
+        const unsigned lds = line_rows ? KKW_PROG_BLOCK * (KKW_PROG_HALF_LINE + 1) * 4u : 0u;
+        KKW_HIP(hipModuleLaunchKernel(persistent ? prog->fn_p : prog->fn, grid, 1, 1, KKW_PROG_BLOCK, 1, 1, lds,
+                                      eng->stream, args, nullptr));
         KKW_HIP(hipEventRecord(e1, eng->stream));
         KKW_HIP(hipEventSynchronize(e1));
         float ms = 0.0f;

@@ -19,6 +19,12 @@ struct kkw_prog_bucket {
 static_assert(sizeof(kkw_prog_bucket) == 32 && __builtin_offsetof(kkw_prog_bucket, alias) == 16,
               "bucket halves are {p, primary, prow} and {alias, meta, arow}");
 
+// Persistent form on rows of whole 128-byte lines: path entries wait in LDS in
+// 64-byte half lines, KKW_PROG_HALF_LINE + 1 words per lane of a block of
+// KKW_PROG_BLOCK lanes.
+#define KKW_PROG_BLOCK 256u
+#define KKW_PROG_HALF_LINE 16u
+
 struct kkw_prog_args {
     kkw_ctx c;
     const kkw_prog_bucket *lb;   // alias tables of the weights (local indices), or null: uniform

@@ -147,7 +147,7 @@ __device__ __forceinline__ uint32_t kkw_prog_start(const kkw_prog_args &A, uint3
 }
 
 // Plain form: one lane per walker for the whole launch.
-extern "C" __global__ void __launch_bounds__(256) kkw_program_walk(kkw_prog_args A)
+extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk(kkw_prog_args A)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.walker_num) return;
@@ -201,7 +201,6 @@ extern "C" __global__ void __launch_bounds__(256) kkw_program_walk(kkw_prog_args
 // fills; the half line that ends a walk is written up to its last 16-byte
 // group.  Half lines reach HBM whole instead of 16 bytes at a time.
 extern __shared__ uint32_t kkw_prog_lds[];
-#define KKW_PROG_HALF_LINE 16u
 
 __device__ __forceinline__ void kkw_prog_flush(const uint32_t *buf, uint32_t *path, uint32_t step)
 {
@@ -219,7 +218,7 @@ __device__ __forceinline__ void kkw_prog_flush(const uint32_t *buf, uint32_t *pa
 // once.  Each loop iteration runs one trial per active lane, with the same
 // draws as the plain form; a wave leaves when the queue is drained and its
 // lanes are idle, so every wave reaches the exit.
-extern "C" __global__ void __launch_bounds__(256) kkw_program_walk_persistent(kkw_prog_args A)
+extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk_persistent(kkw_prog_args A)
 {
     const kkw_ctx &c = A.c;
     const uint32_t lane = threadIdx.x & 63;
