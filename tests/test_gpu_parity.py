@@ -224,6 +224,8 @@ WALK_CASES = {
     # L = 80: persistent rows of three whole 128-byte lines (line-buffered write-out)
     "deepwalk_w_l80": dict(kind=kk.DEEPWALK, walker_num=3000, walk_length=80, seed=24),
     "n2v_u_l80": dict(kind=kk.NODE2VEC, walker_num=3000, walk_length=80, p=0.5, q=2.0, seed=25, use_weights=False),
+    # L = 63: full walks end on the last entry of a line (no partial line to flush)
+    "deepwalk_w_l63": dict(kind=kk.DEEPWALK, walker_num=2000, walk_length=63, seed=26),
 }
 
 
