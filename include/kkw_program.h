@@ -86,8 +86,8 @@ typedef struct kkw_ctx {
 
 /* 1 if u is in v's list, else 0 -- the adjacency test of a second-order
  * walk (node2vec's respond_query, src/apps/node2vec.hpp:158-166).  Reads the
- * engine's neighbour hash sets when present: v's table of 2 deg(v) + 4 slots
- * starts at 2 row_ptr[v] + 4 v, slot of a key = (key * 0x9E3779B1 mod 2^32) *
+ * engine's neighbour hash sets when present: v's table of 4 deg(v) + 4 slots
+ * starts at 4 (row_ptr[v] + v), slot of a key = (key * 0x9E3779B1 mod 2^32) *
  * size / 2^32, linear probing, 0xFFFFFFFF empty.  Otherwise a binary search
  * (sorted lists) or a scan.  The answer does not depend on the method. */
 KKW_FN int kkw_adjacent(uint32_t v, uint32_t u, const struct kkw_ctx &c)
@@ -95,8 +95,8 @@ KKW_FN int kkw_adjacent(uint32_t v, uint32_t u, const struct kkw_ctx &c)
     const uint64_t b = c.row_ptr[v], e = c.row_ptr[v + 1];
     if (e == b) return 0;
     if (c.adjset) {
-        const uint32_t size = (uint32_t)(2 * (e - b) + 4);
-        const uint64_t tb = 2 * b + 4 * (uint64_t)v;   /* table start in adjset */
+        const uint32_t size = (uint32_t)(4 * (e - b) + 4);
+        const uint64_t tb = 4 * (b + (uint64_t)v);   /* table start in adjset */
         uint32_t s = (uint32_t)(((uint64_t)(u * 0x9E3779B1u) * size) >> 32);
 #if defined(__HIPCC_RTC__) || defined(__HIPCC__)
         /* one aligned 16-byte block (four slots) per read, slots in probe order */

@@ -318,8 +318,8 @@ __global__ void k_adjset_insert(const uint64_t *row_ptr, const uint32_t *col, ui
     }
     const uint32_t v = lo;
     const uint64_t b = row_ptr[v];
-    const uint32_t size = (uint32_t)(2 * (row_ptr[v + 1] - b) + 4);
-    uint32_t *tab = t + 2 * b + 4 * (uint64_t)v;
+    const uint32_t size = adj_table_size(row_ptr[v + 1] - b);
+    uint32_t *tab = t + adj_table_start(b, v);
     const uint32_t u = col[e];
     uint32_t slot = adj_slot(u, size);
     for (uint32_t n = 0; n < size; n++) {
@@ -1056,7 +1056,7 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
 {
     if (!gs.row_ptr) throw Error(-2, "no graph loaded");
     if (gs.adjset) return;
-    const uint64_t n = 2 * gs.E + 4 * (uint64_t)gs.V + 4;
+    const uint64_t n = adj_table_start(gs.E, gs.V) + 4;   // every table ends by slot 4 (E + V)
     DArr<uint32_t> t(n);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, t.p, ADJ_EMPTY, n);
     if (gs.E)

@@ -147,8 +147,8 @@ __device__ __forceinline__ bool adj_member(const DevGraph &g, uint32_t prev, uin
                                            uint32_t x, uint32_t &probes, uint32_t &lines)
 {
     if (g.adjset) {
-        const uint32_t *tab = g.adjset + 2 * pbeg + 4 * (uint64_t)prev;
-        const uint32_t size = (uint32_t)(2 * (pend - pbeg) + 4);
+        const uint32_t *tab = g.adjset + adj_table_start(pbeg, prev);
+        const uint32_t size = adj_table_size(pend - pbeg);
         uint32_t s = adj_slot(x, size);
         uintptr_t last = ~(uintptr_t)0;
         for (uint32_t n = 0; n < size; n++) {   // load <= 1/2: an empty slot ends it
@@ -855,8 +855,8 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
                 } else if (dart > J.acc_hi) {
                     rejected = true;
                 } else if (g.adjset) {
-                    toff = 2 * pbeg + 4 * (uint64_t)prev;
-                    tsize = (uint32_t)(2 * (pend - pbeg) + 4);
+                    toff = adj_table_start(pbeg, prev);
+                    tsize = adj_table_size(pend - pbeg);
                     hslot = adj_slot(cand, tsize);
                     phase = PH_HASH;
                     ls.searches++;
