@@ -106,12 +106,7 @@ static PreparedJob prepare_job(kkw_engine *eng, const kkw_job &j, bool line_rows
         node2vec_constants(j.p, j.q, d);
         // neighbour hash sets for the adjacency test, unless the caller turned
         // them off (kkw_graph_adj_index(eng, 0): binary search of sorted lists)
-        if (eng->adj_index_auto && !gs.adjset) {
-            eng->graph_version++;
-            double t0 = now_ms();
-            build_adj_index(gs, eng->stream);
-            eng->timing.index_ms = now_ms() - t0;
-        }
+        auto_adj_index(eng);
     }
     if (!j.use_weights && !gs.ubucket) {   // uniform draws read uniform buckets
         eng->graph_version++;
@@ -1206,11 +1201,7 @@ int kkw_prepare(kkw_engine *eng)
             build_alias(eng->gs, eng->stream);
             eng->timing.alias_ms = now_ms() - t0;
         }
-        if (eng->adj_index_auto && !eng->gs.adjset) {
-            double t0 = now_ms();
-            build_adj_index(eng->gs, eng->stream);
-            eng->timing.index_ms = now_ms() - t0;
-        }
+        auto_adj_index(eng);
         return KKW_OK;
     });
 }

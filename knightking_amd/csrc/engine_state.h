@@ -51,8 +51,10 @@ struct DevBuf {
     {
         bytes = n;
         hipError_t e = hipMalloc(&p, n ? n : 1);
-        if (e != hipSuccess)
+        if (e != hipSuccess) {
+            (void)hipGetLastError();   // a later launch check must not see this failure
             throw Error(KKW_E_NOMEM, "hipMalloc of " + std::to_string(n) + " bytes: " + hipGetErrorString(e));
+        }
     }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -82,3 +84,23 @@ struct kkw_engine {
     uint64_t graph_version = 1;
     std::unique_ptr<ServeState> serve;
 };
+
+// Neighbour hash sets when the engine builds them on its own (first node2vec
+// job, kkw_prepare, a program that tests adjacency).  They only speed up the
+// adjacency test: when they do not fit in device memory the walks use binary
+// search over the sorted lists, with the same answers, and automatic builds
+// stay off until kkw_graph_adj_index(eng, 1).
+inline void auto_adj_index(kkw_engine *eng)
+{
+    if (!eng->adj_index_auto || eng->gs.adjset) return;
+    eng->graph_version++;
+    const double t0 = kkw::now_ms();
+    try {
+        kkw::build_adj_index(eng->gs, eng->stream);
+    } catch (const kkw::Error &e) {
+        if (e.code != KKW_E_NOMEM) throw;
+        eng->adj_index_auto = false;
+        return;
+    }
+    eng->timing.index_ms = kkw::now_ms() - t0;
+}

@@ -23,9 +23,11 @@ T *dalloc(size_t n)
     T *p = nullptr;
     if (n == 0) n = 1;
     hipError_t e = hipMalloc((void **)&p, n * sizeof(T));
-    if (e != hipSuccess)
+    if (e != hipSuccess) {
+        (void)hipGetLastError();   // a later launch check must not see this failure
         throw Error(-3, std::string("hipMalloc of ") + std::to_string(n * sizeof(T)) + " bytes: " +
                             hipGetErrorString(e));
+    }
     return p;
 }
 

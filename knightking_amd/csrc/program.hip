@@ -185,12 +185,7 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
         if (job->max_length >= 0x7FFFFFF0u) return fail(KKW_E_INVALID, "max_length too large");
         if (job->mode != KKW_MODE_PLAIN && job->mode != KKW_MODE_PERSISTENT)
             return fail(KKW_E_INVALID, "unknown mode");
-        if (prog->uses_adjacency && eng->adj_index_auto && !gs.adjset) {
-            double t0 = now_ms();
-            eng->graph_version++;
-            build_adj_index(gs, eng->stream);
-            eng->timing.index_ms = now_ms() - t0;
-        }
+        if (prog->uses_adjacency) auto_adj_index(eng);
         if (job->use_weights && !gs.lbucket) {
             // local-index alias tables; the sums array keeps its address, so
             // batches captured earlier stay valid
