@@ -417,6 +417,75 @@ def test_large_node2vec_properties(engine):
     assert np.array_equal(pl.vertices[sample, :81], pp.vertices[sample, :81])
 
 
+def _edge_keys(rp, col):
+    """Sorted (u << 32 | v) keys of the CSR's edges (lists sorted by neighbour)."""
+    deg = np.diff(rp.astype(np.int64))
+    return (np.repeat(np.arange(len(rp) - 1, dtype=np.int64), deg) << 32) | col.astype(np.int64)
+
+
+def _edges_exist(key, u, v):
+    """(u, v) is an edge for every pair: binary search in the sorted keys."""
+    probe = (np.asarray(u, np.int64) << 32) | np.asarray(v, np.int64)
+    pos = np.searchsorted(key, probe)
+    return bool(np.all(key[np.minimum(pos, len(key) - 1)] == probe))
+
+
+def test_deepwalk_rmat22_properties(engine):
+    """The DeepWalk workload at full size (R-MAT scale 22, 4,194,304 walkers,
+    L = 80, weighted): non-isolated walkers take exactly 80 steps, isolated
+    ones none; every sampled transition is an edge; the persistent and plain
+    launches agree on every row."""
+    load_rmat(engine, 22)
+    rp, col, _, _ = engine.download(weights=False)
+    V = len(rp) - 1
+    deg = np.diff(rp.astype(np.int64))
+    job = kk.deepwalk_job(V, 80, True, seed=1)
+    (pp,) = engine.run([job], mode=kk.MODE_PERSISTENT)
+    assert np.array_equal(pp.lengths, np.where(deg > 0, 81, 1))
+    sample = np.random.default_rng(1).choice(np.nonzero(deg > 0)[0], 20000, replace=False)
+    assert _edges_exist(_edge_keys(rp, col), pp.vertices[sample, :80].reshape(-1),
+                        pp.vertices[sample, 1:81].reshape(-1))
+    (pl,) = engine.run([job], mode=kk.MODE_PLAIN)
+    assert_same_rows(pl, pp, "deepwalk R-MAT-22")
+
+
+def test_mixed_jobs_rmat23_properties(engine):
+    """The batched workload at full size (R-MAT scale 23, 5 vertex types, the
+    64 mixed PPR / DeepWalk / metapath jobs of the benchmark): one persistent
+    launch gives every job the rows of its own plain launch, and sampled
+    metapath paths follow CSR edges whose types their schemes allow."""
+    import bench
+    engine.graph_rmat(23, 16, 0.57, 0.19, 0.19, seed=2, weighted=True)
+    engine.type_by_dst(5, 3)
+    rp, col, _, et = engine.download(weights=False, etype=True)
+    vtype = O.vertex_types(len(rp) - 1, 5, 3)
+    assert np.array_equal(et, vtype[col])
+    jobs, kinds = bench.mixed_jobs(kk)
+    print(f"[gpu] 64 mixed jobs on R-MAT-23, {sum(j.walker_num for j in jobs)} walkers", flush=True)
+    mixed = engine.run(jobs, mode=kk.MODE_PERSISTENT)
+    rng = np.random.default_rng(2)
+    us, vs = [], []
+    for j, (job, kind, m) in enumerate(zip(jobs, kinds, mixed)):
+        (alone,) = engine.run([job], mode=kk.MODE_PLAIN)
+        assert_same_rows(alone, m, f"job {j} ({kind})")
+        if kind != "metapath":
+            continue
+        long_rows = np.nonzero(m.lengths > 1)[0]
+        rows = rng.choice(long_rows, min(200, len(long_rows)), replace=False)
+        for r in rows:
+            path = m.vertices[r, :m.lengths[r]].astype(np.int64)
+            us.append(path[:-1])
+            vs.append(path[1:])
+            # the walker's scheme (its INIT draw) allows the type of every step's edge
+            wid = job.walker_begin + int(r)
+            d = O.philox([wid, 0, 0, 2], [job.seed & 0xFFFFFFFF, job.seed >> 32])
+            scheme = job.schemes[O.bounded(int(d[0]), int(d[1]), len(job.schemes))]
+            for s in range(len(path) - 1):
+                t_e = int(vtype[path[s + 1]])   # edge type = type of its destination
+                assert (scheme[s % len(scheme)] >> t_e) & 1, (j, r, s)
+    assert _edges_exist(_edge_keys(rp, col), np.concatenate(us), np.concatenate(vs))
+
+
 @pytest.mark.parametrize("p,q", [(2.0, 0.5), (0.5, 2.0), (1.0, 0.25)])
 @pytest.mark.parametrize("mode", [kk.MODE_PLAIN, kk.MODE_PERSISTENT], ids=["plain", "persistent"])
 def test_node2vec_directed_graph(engine, p, q, mode):
