@@ -616,9 +616,9 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
         o_smask[j] = o;
         o += al256(pj.scheme_mask.size());
     }
-    const size_t upload_bytes = o;
-    const size_t o_queue = o;
+    const size_t o_queue = o;   // the walk's work counter, uploaded as zero
     o += 256;
+    const size_t upload_bytes = o;
     f.o_lens = o;
     const size_t lens_bytes = ((N * 4 + 7) & ~(size_t)7);
     f.o_offs = o + lens_bytes;
@@ -677,11 +677,11 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
     }
     memcpy(H + o_jobs, djs.data(), sizeof(DevJob) * nj);
     memcpy(H + o_pjobs, pjs.data(), sizeof(PackJob) * nj);
+    memset(H + o_queue, 0, sizeof(unsigned long long));
     // ---- enqueue: upload, walk, pack into pinned memory, lengths + offsets
     DevGraph g = eng->gs.view();
     KKW_HIP(hipMemcpyAsync(D, H, upload_bytes, hipMemcpyHostToDevice, st));
     if (chunk_acc) {
-        launch_fill_u64((unsigned long long *)(D + o_queue), 0ull, 1, st);
         uint64_t blocks_needed = (chunk_acc * 64 + 255) / 256;
         int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)eng->cu_count * 8, blocks_needed));
         // serving rows stay 16-byte groups (short PPR walks; full occupancy)

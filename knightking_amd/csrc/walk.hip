@@ -590,8 +590,6 @@ __global__ void k_lens_to_counts(const uint32_t *__restrict__ lens, uint64_t n, 
 }
 
 // Copy every walker's row into the packed array at its offset.  Rows of a
-// flight are numbered across its jobs; job j owns rows [row0, row0 + n).
-// Copy every walker's row into the packed array at its offset.  Rows of a
 // flight are numbered across its jobs; job j owns rows [row0, row0 + n).  One
 // wave packs 64 consecutive rows: their packed rows are contiguous, so the
 // wave writes that whole range with consecutive lanes on consecutive
