@@ -659,7 +659,7 @@ size_t pack_scan_bytes(uint64_t n)
 
 
 // ------------------------------------------------- state-machine kernel ----
-// Persistent single-job kernel in which every loop iteration issues exactly
+// Persistent single-job kernel (node2vec and DeepWalk) in which every loop iteration issues exactly
 // one 16-byte load per active lane: the row_ptr pair of the current vertex
 // (ROW), the first half of a trial's alias bucket (TRIAL) and, when its coin
 // picks the alias, the second half (ALIAS), or four consecutive entries of a
@@ -683,11 +683,12 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
     // candidates come from buckets: the alias tables, or the uniform buckets
     // (both 32 B, both carrying the candidates' rows when the graph packs them)
     const Bucket *const tab = ALIAS ? g.bucket : g.ubucket;
-    const bool rows_in_buckets = g.packed_rows != 0 && KIND != KIND_METAPATH;
+    static_assert(KIND == KIND_NODE2VEC || KIND == KIND_DEEPWALK,
+                  "PPR and metapath jobs run in the trial-loop form (k_walk_persistent)");
+    const bool rows_in_buckets = g.packed_rows != 0;
     LaneStats ls = {0, 0, 0, 0, 0, 0, 0, 0};
     bool active = false;
     uint32_t idx = 0, wid = 0, cur = 0, prev = 0, step = 0, trial = 0, phase = PH_ROW;
-    uint32_t scheme = 0, state = 0, vm = 0;
     uint64_t beg = 0, end = 0, pbeg = 0, pend = 0, cbeg = 0, cend = 0;
     // A search and a hash probe are never live together, so they share state:
     // sorted-list search: key in [sbase, sbase + slen_all), current range [lo, hi);
@@ -733,11 +734,6 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
                 step = 0;
                 trial = 0;
                 phase = PH_ROW;
-                if (KIND == KIND_METAPATH) {
-                    u32x4 r = draw(J.seed, wid, 0, 0, TAG_INIT);
-                    scheme = bounded(r.x, r.y, J.n_schemes);
-                    state = 0;
-                }
                 if (LINES) lb.set(0, cur);
                 else pg.q0 = pg.q1 = pg.q2 = pg.q3 = cur;
                 active = true;
@@ -755,7 +751,6 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
         uint64_t blk = 0;
         if (phase == PH_ROW) {
             addr = reinterpret_cast<const uint32_t *>(g.row_ptr + cur);
-            if (KIND == KIND_METAPATH) vm = g.vmask[cur];
         } else if (phase == PH_TRIAL) {
             u32x4 r = draw(J.seed, wid, step, trial, TAG_TRIAL);
             uint32_t k = bounded(r.x, r.y, (uint32_t)(end - beg));
@@ -813,7 +808,6 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
         } else if (phase == PH_TRIAL || phase == PH_ALIAS) {
             // bucket halves: v = {p, primary, prow} (TRIAL) or {alias, meta, arow}
             // (ALIAS); without rows in the buckets v2 is the second half
-            uint32_t et = 0;
             if (phase == PH_TRIAL) {
                 ls.trials++;
                 const bool prim = u01(rz) < __uint_as_float(v.x);
@@ -825,7 +819,6 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
                     cand = v.y;
                 } else {
                     cand = prim ? v.y : v2.x;
-                    et = prim ? (v2.y & 0xffu) : ((v2.y >> 8) & 0xffu);
                 }
             } else {
                 cand = v.x;
@@ -891,13 +884,8 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
                     last_line = ~(uintptr_t)0;
                     continue;
                 }
-            } else if (KIND == KIND_METAPATH) {
-                float d = u01(rw) * 1.0f;
-                float pd = ((J.scheme_mask[scheme * 8 + state] >> et) & 1u) ? 1.0f : 0.0f;
-                moved = d <= pd;
-                rejected = !moved;
             } else {
-                moved = true;
+                moved = true;   // DeepWalk: a static draw is accepted
             }
         } else if (phase == PH_HASH) {
             // slots from hslot to the end of this aligned block, in probe order
@@ -979,7 +967,6 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
                 pg.set(step, cand);
                 if ((step & 3u) == 3u) pg.store(paths + (uint64_t)idx * stride, step);
             }
-            if (KIND == KIND_METAPATH) state = (state + 1) % J.scheme_len[scheme];
             prev = cur;
             pbeg = beg;
             pend = end;
@@ -993,27 +980,16 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             beg = cbeg;
             end = cend;
         }
-        // row of cur known: termination test of the new step (walk.hpp:956-973)
-        bool stop = end == beg;
-        bool capped = false;
-        if (!stop) {
-            if (KIND == KIND_PPR) {
-                if (J.ppr_ext == 0.0f) stop = true;
-                else if (J.ppr_ext < 1.0f) stop = u01(draw(J.seed, wid, step, 0, TAG_EXT).x) > J.ppr_ext;
-                if (!stop && step >= J.walk_length) stop = capped = true;
-            } else if (KIND == KIND_METAPATH) {
-                stop = step >= J.walk_length || !(vm & J.scheme_mask[scheme * 8 + state]);
-            } else {
-                stop = step >= J.walk_length;
-            }
-        }
+        // row of cur known: termination test of the new step (walk.hpp:956-973;
+        // node2vec and DeepWalk end at an isolated vertex or at length L)
+        const bool stop = end == beg || step >= J.walk_length;
         if (stop) {
             if (LINES) {
                 if ((step & (LN - 1)) != LN - 1) lb.store(paths + (uint64_t)idx * stride, step);
             } else if ((step & 3u) != 3u) {
                 pg.store(paths + (uint64_t)idx * stride, step);
             }
-            J.lens[idx] = (step + 1) | (capped ? 0x80000000u : 0u);
+            J.lens[idx] = step + 1;
             active = false;
             continue;
         }
@@ -1070,30 +1046,36 @@ static void launch_single_t(const DevGraph &g, const DevJob &J, uint64_t n_chunk
                             unsigned long long *queue, bool stats, DevStats *st, int grid, int block,
                             hipStream_t stream)
 {
-    // The state-machine kernel, unless the walk needs node2vec's outlier appendix
-    // (handled by advance()) or KKW_WALK_KERNEL=trial selects the trial-loop form.
+    // The state-machine kernel for node2vec (unless the walk needs the outlier
+    // appendix, handled by advance()) and DeepWalk; PPR and metapath run the
+    // trial-loop form, measured faster for them (R-MAT-22, 4 M walkers: PPR
+    // 6.1 vs 6.9 ms, metapath 6.9 vs 8.5 ms; DeepWalk 5.5 ms either way).
+    // KKW_WALK_KERNEL=trial selects the trial-loop form for every kind.
     static const bool trial_form = [] {
         const char *e = getenv("KKW_WALK_KERNEL");
         return e && strcmp(e, "trial") == 0;
     }();
-    const bool sm = !trial_form && !(KIND == KIND_NODE2VEC && J.overflow > 0.0f);
-    if (sm) {
-        // rows of whole lines: 128-byte aligned rows (the path buffer is 256-byte aligned)
-        const bool lines = J.stride % LINE_ENTRIES == 0;
-        const size_t lds = lines ? (size_t)block * LINE_LDS_STRIDE * sizeof(uint32_t) : 0;
-        if (stats && lines)
-            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, true, LINE_ENTRIES>), dim3(grid), dim3(block), lds, stream, g,
-                               J, n_chunks, chunk, queue, st);
-        else if (stats)
-            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, true, 0>), dim3(grid), dim3(block), 0, stream, g, J,
-                               n_chunks, chunk, queue, st);
-        else if (lines)
-            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, false, LINE_ENTRIES>), dim3(grid), dim3(block), lds, stream, g,
-                               J, n_chunks, chunk, queue, st);
-        else
-            hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, false, 0>), dim3(grid), dim3(block), 0, stream, g, J,
-                               n_chunks, chunk, queue, st);
-        return;
+    constexpr bool SM_KIND = KIND == KIND_NODE2VEC || KIND == KIND_DEEPWALK;
+    const bool sm = SM_KIND && !trial_form && !(KIND == KIND_NODE2VEC && J.overflow > 0.0f);
+    if constexpr (SM_KIND) {
+        if (sm) {
+            // rows of whole lines: 128-byte aligned rows (the path buffer is 256-byte aligned)
+            const bool lines = J.stride % LINE_ENTRIES == 0;
+            const size_t lds = lines ? (size_t)block * LINE_LDS_STRIDE * sizeof(uint32_t) : 0;
+            if (stats && lines)
+                hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, true, LINE_ENTRIES>), dim3(grid), dim3(block), lds, stream, g,
+                                   J, n_chunks, chunk, queue, st);
+            else if (stats)
+                hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, true, 0>), dim3(grid), dim3(block), 0, stream, g, J,
+                                   n_chunks, chunk, queue, st);
+            else if (lines)
+                hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, false, LINE_ENTRIES>), dim3(grid), dim3(block), lds, stream, g,
+                                   J, n_chunks, chunk, queue, st);
+            else
+                hipLaunchKernelGGL((k_walk_sm<KIND, ALIAS, false, 0>), dim3(grid), dim3(block), 0, stream, g, J,
+                                   n_chunks, chunk, queue, st);
+            return;
+        }
     }
     const bool lines = J.stride % LINE_ENTRIES == 0;
     const size_t lds = lines ? (size_t)block * (HALF_LINE_ENTRIES + 1) * sizeof(uint32_t) : 0;
