@@ -145,6 +145,8 @@ def batched_vs_plain(kk, steps, warmup, scale=23):
                "kinds": {k: kinds.count(k) for k in set(kinds)}}
         for name, mode, flags in (("persistent", kk.MODE_PERSISTENT, kk.BATCH_GRAPH), ("plain", kk.MODE_PLAIN, 0)):
             b = eng.batch(jobs, mode, flags)
+            if mode == kk.MODE_PERSISTENT:
+                out["launch_shape"] = dict(zip(("block", "blocks_per_cu", "chunk"), b.autotune(None)))
             wall, kms = time_batch(b, steps, warmup)
             total = sum(b.steps(i) for i in range(len(jobs)))
             out[name + "_steps_per_s"] = round(total * steps / wall, 1)
