@@ -665,14 +665,33 @@ __global__ void k_list_big(const uint64_t *row_ptr, uint32_t V, uint32_t *big, u
 }
 
 // rows of each bucket's candidates (a separate pass keeps the per-vertex
-// construction above free of random reads)
+// construction above free of random reads).  Each bucket is read and written
+// back whole as its two 16-byte halves, so consecutive lanes rewrite whole
+// lines (rewriting only the row fields left every line partly written); a
+// candidate's row pair is one 16-byte read.
+typedef uint32_t rows_u32x4 __attribute__((ext_vector_type(4), aligned(4)));
+
+__device__ __forceinline__ uint64_t row_of(const uint64_t *row_ptr, uint32_t v)
+{
+    const rows_u32x4 q = *reinterpret_cast<const rows_u32x4 *>(row_ptr + v);
+    return pack_row((uint64_t)q.x | ((uint64_t)q.y << 32), (uint64_t)q.z | ((uint64_t)q.w << 32));
+}
+
 __global__ void k_bucket_rows(Bucket *b, uint64_t E, const uint64_t *row_ptr)
 {
-    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
-    const uint32_t p = b[e].primary, a = b[e].alias;
-    b[e].prow = pack_row(row_ptr[p], row_ptr[p + 1]);
-    b[e].arow = p == a ? b[e].prow : pack_row(row_ptr[a], row_ptr[a + 1]);
+    uint4 *q = reinterpret_cast<uint4 *>(b + e);
+    uint4 h0 = q[0], h1 = q[1];   // {p, primary, prow}, {alias, meta, arow}
+    const uint32_t p = h0.y, a = h1.x;
+    const uint64_t pr = row_of(row_ptr, p);
+    const uint64_t ar = p == a ? pr : row_of(row_ptr, a);
+    h0.z = (uint32_t)pr;
+    h0.w = (uint32_t)(pr >> 32);
+    h1.z = (uint32_t)ar;
+    h1.w = (uint32_t)(ar >> 32);
+    q[0] = h0;
+    q[1] = h1;
 }
 
 // uniform candidates as buckets: p = 1, alias = primary = col[e]
@@ -687,7 +706,7 @@ __global__ void k_uniform_buckets(const uint32_t *col, const uint8_t *etype, con
     b.primary = u;
     b.alias = u;
     b.meta = etype ? (uint32_t)etype[e] * 0x101u : 0u;
-    b.prow = rows ? pack_row(row_ptr[u], row_ptr[u + 1]) : 0;
+    b.prow = rows ? row_of(row_ptr, u) : 0;
     b.arow = b.prow;
     out[e] = b;
 }
