@@ -493,6 +493,24 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A bucket built with local edge indices gets its edge types (meta) and, for
+// the engine's tables, its candidates' neighbour ids in place of the indices;
+// tables of local indices (walk programs) keep them and take the ids in the
+// row fields.  Done as each bucket is built, so every bucket is written once.
+__device__ __forceinline__ void finish_bucket(Bucket &bk, uint64_t b, const uint32_t *__restrict__ col,
+                                              const uint8_t *__restrict__ etype, bool to_ids)
+{
+    const uint32_t pe = bk.primary, ae = bk.alias;
+    bk.meta = etype ? ((uint32_t)etype[b + pe] | ((uint32_t)etype[b + ae] << 8)) : 0u;
+    if (to_ids) {
+        bk.primary = col[b + pe];
+        bk.alias = col[b + ae];
+    } else {
+        bk.prow = col[b + pe];
+        bk.arow = col[b + ae];
+    }
+}
+
 __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__restrict__ list, uint32_t n_list,
                                                             const uint64_t *__restrict__ row_ptr,
                                                             const uint32_t *__restrict__ col,
@@ -608,6 +626,7 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
                 bk.p = 1.0f;
                 bk.alias = se.x;
             }
+            finish_bucket(bk, b, col, etype, to_ids);
             o[ob + lane] = bk;
         }
         const uint64_t ms = __ballot(to_s), ml = __ballot(to_l);
@@ -639,21 +658,8 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
         bk.meta = 0;
         bk.prow = 0;
         bk.arow = 0;
+        finish_bucket(bk, b, col, etype, to_ids);
         o[ob + i] = bk;
-    }
-    wave_sync();
-    // 5. local edge indices -> neighbour ids and edge types; tables of local
-    // indices (walk programs) keep them and take the ids in the row fields
-    for (uint32_t i = lane; i < d; i += 64) {
-        const uint32_t pe = o[i].primary, ae = o[i].alias;
-        o[i].meta = etype ? ((uint32_t)etype[b + pe] | ((uint32_t)etype[b + ae] << 8)) : 0u;
-        if (to_ids) {
-            o[i].primary = col[b + pe];
-            o[i].alias = col[b + ae];
-        } else {
-            o[i].prow = col[b + pe];
-            o[i].arow = col[b + ae];
-        }
     }
 }
 
