@@ -478,9 +478,9 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
 // entries go to the tails in pairing order (ballot again).  This is exactly the
 // sequential FIFO run k steps at a time.  Queue windows of AW_WIN entries live
 // in LDS; lists longer than that also keep their queues in global memory and
-// refill the windows from there.  Buckets first hold local edge indices; the
-// last pass turns them into neighbour ids and edge types (or, for the tables
-// of walk programs, keeps the indices and adds the ids).
+// refill the windows from there.  The pairing works on local edge indices;
+// each bucket takes its neighbour ids and edge types as it is written
+// (finish_bucket; the tables of walk programs keep the indices and add the ids).
 constexpr uint32_t AW_MIN = 32;
 constexpr uint32_t AW_WIN = 512;
 constexpr int AW_WAVES = 4;
