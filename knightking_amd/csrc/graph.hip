@@ -308,6 +308,22 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n)
     if (i < n) p[i] = v;
 }
 
+// Neighbour hash sets are built in two tiers by list length: lists of up to
+// ADJ_LDS_MAX_DEG entries one wave each in LDS (k_adjset_small: each table's
+// lines leave once, instead of every insert fetching and dirtying a random
+// line of HBM), longer lists one lane per edge with global atomics.
+constexpr uint32_t ADJ_LDS_MAX_DEG = 255;   // table of at most 4 * 255 + 4 = 1024 slots
+
+__device__ __forceinline__ void adj_insert(uint32_t *tab, uint32_t size, uint32_t u)
+{
+    uint32_t slot = adj_slot(u, size);
+    for (uint32_t n = 0; n < size; n++) {   // deg < size entries: a free slot exists
+        const uint32_t old = atomicCAS(tab + slot, ADJ_EMPTY, u);
+        if (old == ADJ_EMPTY || old == u) return;
+        slot = slot + 1 == size ? 0 : slot + 1;
+    }
+}
+
 __global__ void k_adjset_insert(const uint64_t *row_ptr, const uint32_t *col, uint32_t V, uint64_t E, uint32_t *t)
 {
     uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -319,16 +335,9 @@ __global__ void k_adjset_insert(const uint64_t *row_ptr, const uint32_t *col, ui
         else hi = mid - 1;
     }
     const uint32_t v = lo;
-    const uint64_t b = row_ptr[v];
-    const uint32_t size = adj_table_size(row_ptr[v + 1] - b);
-    uint32_t *tab = t + adj_table_start(b, v);
-    const uint32_t u = col[e];
-    uint32_t slot = adj_slot(u, size);
-    for (uint32_t n = 0; n < size; n++) {
-        uint32_t old = atomicCAS(tab + slot, ADJ_EMPTY, u);
-        if (old == ADJ_EMPTY || old == u) return;
-        slot = slot + 1 == size ? 0 : slot + 1;
-    }
+    const uint64_t b = row_ptr[v], deg = row_ptr[v + 1] - b;
+    if (deg <= ADJ_LDS_MAX_DEG) return;   // k_adjset_small's
+    adj_insert(t + adj_table_start(b, v), adj_table_size(deg), col[e]);
 }
 
 // -------------------------------------------------------------- types ----
@@ -660,6 +669,30 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
         bk.arow = 0;
         finish_bucket(bk, b, col, etype, to_ids);
         o[ob + i] = bk;
+    }
+}
+
+// Neighbour hash sets of lists with 1..ADJ_LDS_MAX_DEG entries: one wave per
+// vertex (vertices strided over the grid's waves) builds the table in LDS with
+// LDS atomics and writes it out whole.
+__global__ void __launch_bounds__(256) k_adjset_small(const uint64_t *row_ptr, const uint32_t *col, uint32_t V,
+                                                      uint32_t *t)
+{
+    __shared__ uint32_t tabs[4][4 * ADJ_LDS_MAX_DEG + 4];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t *lt = tabs[wave];
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    for (uint64_t v = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave; v < V; v += waves) {   // wave-uniform
+        const uint64_t b = row_ptr[v], e = row_ptr[v + 1];
+        if (e == b || e - b > ADJ_LDS_MAX_DEG) continue;
+        const uint32_t size = adj_table_size(e - b);
+        for (uint32_t i = lane; i < size; i += 64) lt[i] = ADJ_EMPTY;
+        wave_sync();
+        for (uint64_t k = b + lane; k < e; k += 64) adj_insert(lt, size, col[k]);
+        wave_sync();
+        uint32_t *gt = t + adj_table_start(b, v);
+        for (uint32_t i = lane; i < size; i += 64) gt[i] = lt[i];
+        wave_sync();   // the next vertex reuses the LDS table
     }
 }
 
@@ -1086,9 +1119,12 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
     const uint64_t n = adj_table_start(gs.E, gs.V) + 4;   // every table ends by slot 4 (E + V)
     DArr<uint32_t> t(n);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, t.p, ADJ_EMPTY, n);
-    if (gs.E)
+    if (gs.E) {
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)gs.V + 3) / 4, 1u << 16);   // 4 waves each
+        hipLaunchKernelGGL(k_adjset_small, dim3(blocks), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V, t.p);
         hipLaunchKernelGGL(k_adjset_insert, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V,
                            gs.E, t.p);
+    }
     KKW_HIP(hipGetLastError());
     KKW_HIP(hipStreamSynchronize(s));
     gs.adjset = t.release();
