@@ -401,7 +401,7 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
                                               const uint8_t *__restrict__ etype, uint32_t V,
                                               Bucket *__restrict__ out, float *__restrict__ vsum,
                                               uint2 *__restrict__ qs, uint2 *__restrict__ ql, uint32_t big_threshold,
-                                              bool to_ids)
+                                              bool to_ids, const uint64_t *__restrict__ nrow)
 {
     uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
@@ -428,6 +428,7 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
     }
     const uint32_t *cv = col + b;
     const uint8_t *tv = etype ? etype + b : nullptr;
+    const uint64_t *rv = nrow ? nrow + b : nullptr;   // packed rows of the list's neighbours
     Bucket *o = out + b;
     uint32_t sh = 0, lh = 0;
     while (sh != st) {
@@ -446,6 +447,9 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
             if (!to_ids) {   // local tables: the candidates' ids in the row fields
                 bk.prow = cv[se.x];
                 bk.arow = cv[le.x];
+            } else if (rv) {
+                bk.prow = rv[se.x];
+                bk.arow = rv[le.x];
             }
             float diff = ave - ps;
             pl -= diff;
@@ -458,6 +462,7 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
             bk.alias = bk.primary;
             bk.meta = tv ? (uint32_t)tv[se.x] * 0x101u : 0u;
             if (!to_ids) bk.prow = bk.arow = cv[se.x];
+            else if (rv) bk.prow = bk.arow = rv[se.x];
         }
         *o++ = bk;
     }
@@ -471,6 +476,7 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
         bk.alias = bk.primary;
         bk.meta = tv ? (uint32_t)tv[le.x] * 0x101u : 0u;
         if (!to_ids) bk.prow = bk.arow = cv[le.x];
+        else if (rv) bk.prow = bk.arow = rv[le.x];
         *o++ = bk;
     }
 }
@@ -503,17 +509,23 @@ __device__ __forceinline__ void wave_sync()
 }
 
 // A bucket built with local edge indices gets its edge types (meta) and, for
-// the engine's tables, its candidates' neighbour ids in place of the indices;
-// tables of local indices (walk programs) keep them and take the ids in the
-// row fields.  Done as each bucket is built, so every bucket is written once.
+// the engine's tables, its candidates' neighbour ids in place of the indices
+// and their packed rows (nrow, when the graph packs rows); tables of local
+// indices (walk programs) keep them and take the ids in the row fields.  Done
+// as each bucket is built, so every bucket is written once.
 __device__ __forceinline__ void finish_bucket(Bucket &bk, uint64_t b, const uint32_t *__restrict__ col,
-                                              const uint8_t *__restrict__ etype, bool to_ids)
+                                              const uint8_t *__restrict__ etype, bool to_ids,
+                                              const uint64_t *__restrict__ nrow)
 {
     const uint32_t pe = bk.primary, ae = bk.alias;
     bk.meta = etype ? ((uint32_t)etype[b + pe] | ((uint32_t)etype[b + ae] << 8)) : 0u;
     if (to_ids) {
         bk.primary = col[b + pe];
         bk.alias = col[b + ae];
+        if (nrow) {
+            bk.prow = nrow[b + pe];
+            bk.arow = nrow[b + ae];
+        }
     } else {
         bk.prow = col[b + pe];
         bk.arow = col[b + ae];
@@ -527,7 +539,7 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
                                                             const uint8_t *__restrict__ etype,
                                                             Bucket *__restrict__ out, float *__restrict__ vsum,
                                                             uint2 *__restrict__ qs, uint2 *__restrict__ ql,
-                                                            bool to_ids)
+                                                            bool to_ids, const uint64_t *__restrict__ nrow)
 {
     __shared__ uint2 win_s[AW_WAVES][AW_WIN], win_l[AW_WAVES][AW_WIN];
     __shared__ float stage[AW_WAVES][AW_SUM_CHUNK];
@@ -635,7 +647,7 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
                 bk.p = 1.0f;
                 bk.alias = se.x;
             }
-            finish_bucket(bk, b, col, etype, to_ids);
+            finish_bucket(bk, b, col, etype, to_ids, nrow);
             o[ob + lane] = bk;
         }
         const uint64_t ms = __ballot(to_s), ml = __ballot(to_l);
@@ -667,7 +679,7 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
         bk.meta = 0;
         bk.prow = 0;
         bk.arow = 0;
-        finish_bucket(bk, b, col, etype, to_ids);
+        finish_bucket(bk, b, col, etype, to_ids, nrow);
         o[ob + i] = bk;
     }
 }
@@ -703,34 +715,22 @@ __global__ void k_list_big(const uint64_t *row_ptr, uint32_t V, uint32_t *big, u
     if (row_ptr[v + 1] - row_ptr[v] >= AW_MIN) big[atomicAdd(n_big, 1u)] = (uint32_t)v;
 }
 
-// rows of each bucket's candidates (a separate pass keeps the per-vertex
-// construction above free of random reads).  Each bucket is read and written
-// back whole as its two 16-byte halves, so consecutive lanes rewrite whole
-// lines (rewriting only the row fields left every line partly written); a
-// candidate's row pair is one 16-byte read.
+// Packed row of every edge's neighbour, nrow[k] = row of col[k], for the
+// alias kernels to copy into buckets (each bucket's candidates are neighbours
+// of its list: E random row reads instead of two per bucket).
 typedef uint32_t rows_u32x4 __attribute__((ext_vector_type(4), aligned(4)));
 
 __device__ __forceinline__ uint64_t row_of(const uint64_t *row_ptr, uint32_t v)
 {
-    const rows_u32x4 q = *reinterpret_cast<const rows_u32x4 *>(row_ptr + v);
+    const rows_u32x4 q = *reinterpret_cast<const rows_u32x4 *>(row_ptr + v);   // row_ptr[v], row_ptr[v + 1]
     return pack_row((uint64_t)q.x | ((uint64_t)q.y << 32), (uint64_t)q.z | ((uint64_t)q.w << 32));
 }
 
-__global__ void k_bucket_rows(Bucket *b, uint64_t E, const uint64_t *row_ptr)
+__global__ void k_neighbour_rows(const uint32_t *__restrict__ col, uint64_t E, const uint64_t *__restrict__ row_ptr,
+                                 uint64_t *__restrict__ nrow)
 {
-    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= E) return;
-    uint4 *q = reinterpret_cast<uint4 *>(b + e);
-    uint4 h0 = q[0], h1 = q[1];   // {p, primary, prow}, {alias, meta, arow}
-    const uint32_t p = h0.y, a = h1.x;
-    const uint64_t pr = row_of(row_ptr, p);
-    const uint64_t ar = p == a ? pr : row_of(row_ptr, a);
-    h0.z = (uint32_t)pr;
-    h0.w = (uint32_t)(pr >> 32);
-    h1.z = (uint32_t)ar;
-    h1.w = (uint32_t)(ar >> 32);
-    q[0] = h0;
-    q[1] = h1;
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < E) nrow[k] = row_of(row_ptr, col[k]);
 }
 
 // uniform candidates as buckets: p = 1, alias = primary = col[e]
@@ -1156,6 +1156,13 @@ void build_alias(GraphStore &gs, hipStream_t s, bool local)
     if (!vsum) vsum = vs.p = dalloc<float>(gs.V);
     {
         DArr<uint2> qs(gs.E), ql(gs.E);
+        // the engine's tables carry their candidates' rows when the graph packs them
+        DArr<uint64_t> nrow;
+        if (!local && gs.packable && gs.E) {
+            nrow.p = dalloc<uint64_t>(gs.E);
+            hipLaunchKernelGGL(k_neighbour_rows, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.col, gs.E, gs.row_ptr,
+                               nrow.p);
+        }
         // lists of AW_MIN or more entries: one wave each; the rest: one lane each
         DArr<uint32_t> big(gs.V);
         DArr<unsigned int> n_big_d(1);
@@ -1167,11 +1174,9 @@ void build_alias(GraphStore &gs, hipStream_t s, bool local)
         KKW_HIP(hipStreamSynchronize(s));
         if (n_big)
             hipLaunchKernelGGL(k_alias_wave, dim3(grid_for(n_big, AW_WAVES)), dim3(64 * AW_WAVES), 0, s, big.p,
-                               n_big, gs.row_ptr, gs.col, gs.w, gs.etype, t.p, vsum, qs.p, ql.p, !local);
+                               n_big, gs.row_ptr, gs.col, gs.w, gs.etype, t.p, vsum, qs.p, ql.p, !local, nrow.p);
         hipLaunchKernelGGL(k_alias, dim3(grid_for(gs.V, 64)), dim3(64), 0, s, gs.row_ptr, gs.col, gs.w,
-                           gs.etype, gs.V, t.p, vsum, qs.p, ql.p, AW_MIN, !local);
-        if (!local && gs.packable && gs.E)
-            hipLaunchKernelGGL(k_bucket_rows, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, t.p, gs.E, gs.row_ptr);
+                           gs.etype, gs.V, t.p, vsum, qs.p, ql.p, AW_MIN, !local, nrow.p);
         KKW_HIP(hipGetLastError());
         KKW_HIP(hipStreamSynchronize(s));
     }
