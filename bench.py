@@ -208,27 +208,30 @@ def program_line(kk, eng, V, builtin_paths):
     from oracle import oracle as O
     with open(os.path.join(ROOT, "knightking_amd", "programs", "node2vec.cpp")) as f:
         src = f.read()
-    t0 = time.perf_counter()
-    prog = eng.program(src)
-    compile_s = time.perf_counter() - t0
     n = builtin_paths.vertices.shape[0]
     mask = np.arange(81)[None, :] < builtin_paths.lengths[:, None]
-    out = {"compile_s": round(compile_s, 2),
-           "adjacency": "kkw_adjacent (neighbour hash sets)"}
-    try:
-        up, lo, ip, iq, ov = O.node2vec_constants(2.0, 0.5)
-        params = [80, up, lo, ip, iq, ov, 1.0]
-        prog.run(1024, 80, use_weights=True, seed=1, second_order=True, params=params)   # tables, warm-up
-        for name, mode in (("plain", kk.MODE_PLAIN), ("persistent", kk.MODE_PERSISTENT)):
-            got = prog.run(V, 80, use_weights=True, seed=1, second_order=True, params=params, mode=mode)
-            kms = eng.timing()["kernel_ms"]
-            same = bool(np.array_equal(got.lengths, builtin_paths.lengths) and
-                        np.all((got.vertices[:n, :81] == builtin_paths.vertices[:, :81]) | ~mask))
-            out[name] = {"steps_per_s": round(got.steps / (kms / 1e3), 1), "kernel_ms": round(kms, 3),
-                         "same_paths_as_builtin": same}
-            del got
-    finally:
-        prog.close()
+    out = {"adjacency": "kkw_adjacent (neighbour hash sets)",
+           "tables": "the engine's buckets (ids + rows; the program reads no edge index)"}
+    up, lo, ip, iq, ov = O.node2vec_constants(2.0, 0.5)
+    params = [80, up, lo, ip, iq, ov, 1.0]
+    # the program as shipped, then rebuilt to draw from local-index tables
+    for prefix, options, modes in (("", None, (("plain", kk.MODE_PLAIN), ("persistent", kk.MODE_PERSISTENT))),
+                                   ("index_tables_", "-DKKW_EDGE_INDEX=1", (("persistent", kk.MODE_PERSISTENT),))):
+        t0 = time.perf_counter()
+        prog = eng.program(src, options)
+        out[prefix + "compile_s"] = round(time.perf_counter() - t0, 2)
+        try:
+            prog.run(1024, 80, use_weights=True, seed=1, second_order=True, params=params)   # tables, warm-up
+            for name, mode in modes:
+                got = prog.run(V, 80, use_weights=True, seed=1, second_order=True, params=params, mode=mode)
+                kms = eng.timing()["kernel_ms"]
+                same = bool(np.array_equal(got.lengths, builtin_paths.lengths) and
+                            np.all((got.vertices[:n, :81] == builtin_paths.vertices[:, :81]) | ~mask))
+                out[prefix + name] = {"steps_per_s": round(got.steps / (kms / 1e3), 1), "kernel_ms": round(kms, 3),
+                                      "same_paths_as_builtin": same}
+                del got
+        finally:
+            prog.close()
     log(f"node2vec as a walk program: {out}")
     return out
 

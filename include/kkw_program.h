@@ -44,6 +44,13 @@
  * and per-edge caller data is indexed the same way.  outlier_search returns
  * an index.  In update_state, w.step is already the new step
  * (walk.hpp:1039-1043); in the other functions it is the step being taken.
+ *
+ * A program whose functions never read e.index may say so with
+ * `#define KKW_EDGE_INDEX 0`.  Its walk loop then draws from the engine's own
+ * tables, whose buckets hold each candidate's vertex id and CSR row, so a
+ * step reads no row_ptr entry of the vertex it moves to; e.index is then
+ * KKW_NO_EDGE_INDEX for static draws (an outlier_search result keeps its
+ * index).  Paths are the same in both forms.
  */
 #ifndef KKW_PROGRAM_H
 #define KKW_PROGRAM_H
@@ -68,6 +75,7 @@ typedef __hip_internal::int64_t int64_t;
 
 #define KKW_PROGRAM_ARRAYS 8
 #define KKW_PROGRAM_PARAMS 8
+#define KKW_NO_EDGE_INDEX 0xFFFFFFFFFFFFFFFFull   /* e.index of a program built with KKW_EDGE_INDEX 0 */
 
 /* The graph and the caller's data, as the program's functions read them.  On
  * the device every pointer is a device address. */
@@ -82,7 +90,16 @@ typedef struct kkw_ctx {
     const uint32_t *adjset;    /* neighbour hash sets (below), or null */
     const void *array[KKW_PROGRAM_ARRAYS];   /* kkw_program_set_array slots (null if unset) */
     float param[KKW_PROGRAM_PARAMS];         /* kkw_prog_job.param */
+    /* Kept by the walk loop for kkw_adjacent, per walker: the CSR row of the
+     * vertex the walker has just left (row_v; KKW_NO_VERTEX before its first
+     * step), so a second-order program's adjacency test on that vertex reads
+     * no row_ptr entry.  Programs do not write them. */
+    uint32_t row_v;
+    uint32_t row_deg;
+    uint64_t row_beg;
 } kkw_ctx;
+
+#define KKW_NO_VERTEX 0xFFFFFFFFu
 
 /* 1 if u is in v's list, else 0 -- the adjacency test of a second-order
  * walk (node2vec's respond_query, src/apps/node2vec.hpp:158-166).  Reads the
@@ -92,7 +109,9 @@ typedef struct kkw_ctx {
  * (sorted lists) or a scan.  The answer does not depend on the method. */
 KKW_FN int kkw_adjacent(uint32_t v, uint32_t u, const struct kkw_ctx &c)
 {
-    const uint64_t b = c.row_ptr[v], e = c.row_ptr[v + 1];
+    const bool kept = v == c.row_v;   /* the row the walk loop already holds */
+    const uint64_t b = kept ? c.row_beg : c.row_ptr[v];
+    const uint64_t e = kept ? c.row_beg + c.row_deg : c.row_ptr[v + 1];
     if (e == b) return 0;
     if (c.adjset) {
         const uint32_t size = (uint32_t)(4 * (e - b) + 4);

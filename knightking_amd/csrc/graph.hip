@@ -750,11 +750,16 @@ __global__ void k_uniform_buckets(const uint32_t *col, const uint8_t *etype, con
     out[e] = b;
 }
 
+// One atomic per wave: the wave's maximum is taken with shuffles first.
 __global__ void k_max_degree(const uint64_t *row_ptr, uint32_t V, unsigned long long *out)
 {
-    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    atomicMax(out, (unsigned long long)(row_ptr[v + 1] - row_ptr[v]));
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long d = v < V ? (unsigned long long)(row_ptr[v + 1] - row_ptr[v]) : 0ull;
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(d, off, 64);
+        d = o > d ? o : d;
+    }
+    if ((threadIdx.x & 63) == 0 && d) atomicMax(out, d);
 }
 
 unsigned long long read_counter(unsigned long long *d, hipStream_t s)

@@ -24,6 +24,9 @@ static_assert(sizeof(kkw_prog_bucket) == sizeof(Bucket), "program bucket layout"
 static_assert(offsetof(kkw_prog_bucket, p) == offsetof(Bucket, p), "program bucket layout");
 static_assert(offsetof(kkw_prog_bucket, primary) == offsetof(Bucket, primary), "program bucket layout");
 static_assert(offsetof(kkw_prog_bucket, alias) == offsetof(Bucket, alias), "program bucket layout");
+static_assert(offsetof(kkw_prog_bucket, prow) == offsetof(Bucket, prow), "program bucket layout");
+static_assert(offsetof(kkw_prog_bucket, arow) == offsetof(Bucket, arow), "program bucket layout");
+static_assert(KKW_PROG_ROW_DEG_SHIFT == (unsigned)ROW_DEG_SHIFT, "packed row layout");
 
 struct kkw_program {
     kkw_engine *eng = nullptr;
@@ -32,6 +35,7 @@ struct kkw_program {
     hipFunction_t fn_p = nullptr;   // persistent form
     std::string log;
     bool uses_adjacency = false;   // the source calls kkw_adjacent: give it the hash sets
+    bool edge_index = true;        // reads e.index: local-index tables; else the engine's id + row tables
     void *arrays[KKW_PROGRAM_ARRAYS] = {};
     uint64_t array_bytes[KKW_PROGRAM_ARRAYS] = {};
     ~kkw_program()
@@ -122,6 +126,12 @@ int kkw_program_create(kkw_engine *eng, const char *source, const char *options,
         KKW_HIP(hipModuleLoadData(&p->mod, code.data()));
         KKW_HIP(hipModuleGetFunction(&p->fn, p->mod, "kkw_program_walk"));
         KKW_HIP(hipModuleGetFunction(&p->fn_p, p->mod, "kkw_program_walk_persistent"));
+        hipDeviceptr_t flag = nullptr;
+        size_t flag_bytes = 0;
+        KKW_HIP(hipModuleGetGlobal(&flag, &flag_bytes, p->mod, "kkw_prog_edge_index"));
+        int edge_index = 1;
+        KKW_HIP(hipMemcpyDtoH(&edge_index, flag, sizeof(edge_index)));
+        p->edge_index = edge_index != 0;
         *out = p.release();
         return KKW_OK;
     });
@@ -186,12 +196,24 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
         if (job->mode != KKW_MODE_PLAIN && job->mode != KKW_MODE_PERSISTENT)
             return fail(KKW_E_INVALID, "unknown mode");
         if (prog->uses_adjacency) auto_adj_index(eng);
-        if (job->use_weights && !gs.lbucket) {
+        if (prog->edge_index && job->use_weights && !gs.lbucket) {
             // local-index alias tables; the sums array keeps its address, so
             // batches captured earlier stay valid
             double t0 = now_ms();
             build_alias(gs, eng->stream, true);
             eng->timing.alias_ms = now_ms() - t0;
+        }
+        if (!prog->edge_index) {
+            // the engine's own tables, as its built-in walks build them
+            if (job->use_weights && !gs.bucket) {
+                eng->graph_version++;
+                double t0 = now_ms();
+                build_alias(gs, eng->stream);
+                eng->timing.alias_ms = now_ms() - t0;
+            } else if (!job->use_weights && !gs.ubucket) {
+                eng->graph_version++;
+                build_uniform_buckets(gs, eng->stream);
+            }
         }
         const uint32_t n = job->walker_num;
         const bool persistent = job->mode == KKW_MODE_PERSISTENT;
@@ -219,6 +241,7 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
         kkw_prog_args A;
         memset(&A, 0, sizeof(A));
         A.c.v_num = gs.V;
+        A.c.row_v = KKW_NO_VERTEX;   // each lane keeps its walker's row from here
         A.c.sorted = gs.sorted ? 1u : 0u;
         A.c.e_num = gs.E;
         A.c.adjset = gs.adjset;
@@ -228,7 +251,12 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
         A.c.etype = gs.etype;
         for (int k = 0; k < KKW_PROGRAM_ARRAYS; k++) A.c.array[k] = prog->arrays[k];
         for (int k = 0; k < KKW_PROGRAM_PARAMS; k++) A.c.param[k] = job->param[k];
-        A.lb = job->use_weights ? reinterpret_cast<const kkw_prog_bucket *>(gs.lbucket) : nullptr;
+        if (prog->edge_index) {
+            A.lb = job->use_weights ? reinterpret_cast<const kkw_prog_bucket *>(gs.lbucket) : nullptr;
+        } else {
+            A.lb = reinterpret_cast<const kkw_prog_bucket *>(job->use_weights ? gs.bucket : gs.ubucket);
+            A.packed_rows = gs.packable ? 1u : 0u;
+        }
         A.vsum = job->use_weights ? gs.vsum : nullptr;
         A.seed = job->seed;
         A.walker_begin = job->walker_begin;

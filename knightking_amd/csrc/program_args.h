@@ -4,9 +4,11 @@
 
 #include "kkw_program.h"
 
-// Layout of the library's 32-byte alias bucket; walk programs use tables whose
-// primary/alias fields hold local edge indices and whose prow/arow fields hold
-// the two candidates' vertex ids.
+// Layout of the library's 32-byte alias bucket.  Walk programs that read edge
+// indices use tables whose primary/alias fields hold local edge indices and
+// whose prow/arow fields hold the two candidates' vertex ids; programs built
+// with KKW_EDGE_INDEX 0 use the engine's tables (vertex ids, and the
+// candidates' packed rows when the graph packs them).
 struct kkw_prog_bucket {
     float p;
     uint32_t primary;
@@ -25,9 +27,13 @@ static_assert(sizeof(kkw_prog_bucket) == 32 && __builtin_offsetof(kkw_prog_bucke
 #define KKW_PROG_BLOCK 256u
 #define KKW_PROG_HALF_LINE 16u
 
+// A packed row is beg | deg << KKW_PROG_ROW_DEG_SHIFT (the engine's bucket rows).
+#define KKW_PROG_ROW_DEG_SHIFT 40u
+
 struct kkw_prog_args {
     kkw_ctx c;
-    const kkw_prog_bucket *lb;   // alias tables of the weights (local indices), or null: uniform
+    const kkw_prog_bucket *lb;   // alias tables of the weights (local indices), or null: uniform;
+                                 // KKW_EDGE_INDEX 0: the engine's alias or uniform buckets, never null
     const float *vsum;           // per-vertex sequential weight sums, null when lb is
     uint64_t seed;
     uint32_t walker_begin;
@@ -44,5 +50,5 @@ struct kkw_prog_args {
     unsigned long long *queue;   // persistent form: next work item (zero at launch)
     uint64_t n_chunks;           // persistent form: work items
     uint32_t line_rows;          // persistent form: rows of whole lines, written from LDS
-    uint32_t pad;
+    uint32_t packed_rows;        // KKW_EDGE_INDEX 0: buckets carry rows, beg | deg << 40
 };

@@ -26,6 +26,9 @@
 #ifndef KKW_OUTLIER
 #define KKW_OUTLIER 0
 #endif
+#ifndef KKW_EDGE_INDEX
+#define KKW_EDGE_INDEX 1
+#endif
 #if KKW_OUTLIER && !KKW_DYNAMIC
 #error "an outlier appendix needs a dynamic component (walk.hpp:251-261)"
 #endif
@@ -34,6 +37,13 @@
 #endif
 
 #define KKW_PROG_MAX_TRIALS (1u << 20)   // per step; guards degenerate programs
+
+// The host reads this after loading the module: 1 when the walk loop draws
+// from local-index tables (the program reads e.index), 0 when it draws from
+// the engine's tables of vertex ids and rows.
+extern "C" {
+__device__ int kkw_prog_edge_index = KKW_EDGE_INDEX;
+}
 
 typedef uint32_t kkw_prog_u4a __attribute__((ext_vector_type(4), aligned(4)));
 
@@ -45,9 +55,41 @@ __device__ __forceinline__ void kkw_prog_row(const uint64_t *rp, uint32_t v, uin
     deg = (uint32_t)(((uint64_t)q.z | ((uint64_t)q.w << 32)) - beg);
 }
 
+// The new vertex's row: from the accepted bucket when it carried one, else
+// from row_ptr.
+__device__ __forceinline__ void kkw_prog_next_row(const uint64_t *rp, uint32_t v, bool nrow_ok, uint64_t nrow,
+                                                  uint64_t &beg, uint32_t &deg)
+{
+    if (nrow_ok) {
+        beg = nrow & ((1ull << KKW_PROG_ROW_DEG_SHIFT) - 1ull);
+        deg = (uint32_t)(nrow >> KKW_PROG_ROW_DEG_SHIFT);
+    } else {
+        kkw_prog_row(rp, v, beg, deg);
+    }
+}
+
+// The walker leaves v (row beg, deg): the context keeps that row for
+// kkw_adjacent.
+__device__ __forceinline__ void kkw_prog_keep_row(kkw_ctx &c, uint32_t v, uint64_t beg, uint32_t deg)
+{
+    c.row_v = v;
+    c.row_beg = beg;
+    c.row_deg = deg;
+}
+
 // four path entries held in registers, stored as one aligned 16-byte group
 struct kkw_prog_group {
     uint32_t v[4];
+    // v[step & 3] = x by selects, so the group stays in registers (a
+    // dynamically indexed array would live in scratch memory)
+    __device__ __forceinline__ void set(uint32_t step, uint32_t x)
+    {
+        const uint32_t k = step & 3u;
+        v[0] = k == 0 ? x : v[0];
+        v[1] = k == 1 ? x : v[1];
+        v[2] = k == 2 ? x : v[2];
+        v[3] = k == 3 ? x : v[3];
+    }
     __device__ __forceinline__ void store(uint32_t *path, uint32_t step) const
     {
         *reinterpret_cast<uint4 *>(path + (step & ~3u)) = make_uint4(v[0], v[1], v[2], v[3]);
@@ -55,21 +97,24 @@ struct kkw_prog_group {
 };
 
 __device__ __forceinline__ bool kkw_prog_terminates(const WalkerState &s, const kkw_walker &w, uint32_t cur,
-                                                    uint32_t deg, uint32_t trial, const kkw_prog_args &A)
+                                                    uint32_t deg, uint32_t trial, const kkw_prog_args &A,
+                                                    const kkw_ctx &c)
 {
     if (deg == 0) return true;
-    const float ext = extension(s, w, cur, A.c);
+    const float ext = extension(s, w, cur, c);
     if (ext == 0.0f) return true;
     if (ext < 1.0f) return kkw::u01(kkw::draw(A.seed, w.id, w.step, trial, kkw::TAG_EXT).x) > ext;
     return false;
 }
 
-// One trial at cur; returns true with the accepted edge in `acc`.
+// One trial at cur; returns true with the accepted edge in `acc` and, when
+// the bucket carried it, the new vertex's packed row in `nrow` (nrow_ok).
 __device__ __forceinline__ bool kkw_prog_trial(const WalkerState &s, const kkw_walker &w, uint32_t cur,
                                                uint64_t beg, uint32_t deg, uint32_t trial, const kkw_prog_args &A,
-                                               kkw_edge &acc)
+                                               const kkw_ctx &c, kkw_edge &acc, uint64_t &nrow, bool &nrow_ok)
 {
-    const kkw_ctx &c = A.c;
+    nrow_ok = false;
+    nrow = 0;
 #if KKW_DYNAMIC
     const float up = upper_bound(cur, c);
 #endif
@@ -92,7 +137,7 @@ __device__ __forceinline__ bool kkw_prog_trial(const WalkerState &s, const kkw_w
                 e.index = (uint64_t)ei;
                 e.neighbour = c.col[ei];
                 e.reserved = 0;
-                const float sc = A.lb ? c.weight[ei] : 1.0f;
+                const float sc = A.vsum ? c.weight[ei] : 1.0f;   // vsum is set exactly for weighted draws
                 const float h = (rv - (float)idx * ov) / sc + up;
                 if (!(h <= dynamic_comp(s, w, cur, e, c))) return false;
                 acc = e;
@@ -105,6 +150,23 @@ __device__ __forceinline__ bool kkw_prog_trial(const WalkerState &s, const kkw_w
     const uint32_t k = kkw::bounded(r.x, r.y, deg);
     kkw_edge e;
     e.reserved = 0;
+#if !KKW_EDGE_INDEX
+    {
+        // the engine's buckets: {p, primary id, primary row}, {alias id, meta,
+        // alias row}; uniform buckets have p = 1 (always the primary, col[beg + k])
+        const uint32_t *bw = reinterpret_cast<const uint32_t *>(A.lb + beg + k);
+        const kkw_prog_u4a h0 = *reinterpret_cast<const kkw_prog_u4a *>(bw);
+        if (kkw::u01(r.z) < __uint_as_float(h0.x)) {
+            e.neighbour = h0.y;
+            nrow = (uint64_t)h0.z | ((uint64_t)h0.w << 32);
+        } else {
+            const kkw_prog_u4a h1 = *reinterpret_cast<const kkw_prog_u4a *>(bw + 4);
+            e.neighbour = h1.x;
+            nrow = (uint64_t)h1.z | ((uint64_t)h1.w << 32);
+        }
+        e.index = KKW_NO_EDGE_INDEX;
+    }
+#else
     if (A.lb) {
         // first half {p, local primary, primary's id} as one 16-byte load; the
         // alias half {local alias, meta, alias's id} only when the coin picks it
@@ -122,6 +184,7 @@ __device__ __forceinline__ bool kkw_prog_trial(const WalkerState &s, const kkw_w
         e.index = beg + k;
         e.neighbour = c.col[e.index];
     }
+#endif
 #if KKW_DYNAMIC
     const float dart = kkw::u01(r.w) * up;
     bool ok = false;
@@ -132,6 +195,7 @@ __device__ __forceinline__ bool kkw_prog_trial(const WalkerState &s, const kkw_w
     if (!ok) return false;
 #endif
     acc = e;
+    nrow_ok = !KKW_EDGE_INDEX && A.packed_rows != 0;
     return true;
 }
 
@@ -151,7 +215,8 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk(kk
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.walker_num) return;
-    const kkw_ctx &c = A.c;
+    kkw_ctx c = A.c;   // per lane: the kept row changes as the walker moves
+    kkw_prog_keep_row(c, KKW_NO_VERTEX, 0, 0);
     kkw_walker w;
     w.id = A.walker_begin + i;
     w.step = 0;
@@ -168,7 +233,7 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk(kk
     uint32_t trial = 0;
     for (;;) {
         if (trial == 0 || A.retest_ext) {
-            if (kkw_prog_terminates(s, w, cur, deg, A.retest_ext ? trial : 0u, A)) break;
+            if (kkw_prog_terminates(s, w, cur, deg, A.retest_ext ? trial : 0u, A, c)) break;
             if (w.step >= A.max_len) {
                 capped = true;
                 break;
@@ -179,16 +244,19 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk(kk
             break;
         }
         kkw_edge e;
-        if (!kkw_prog_trial(s, w, cur, beg, deg, trial, A, e)) {
+        uint64_t nrow;
+        bool nrow_ok;
+        if (!kkw_prog_trial(s, w, cur, beg, deg, trial, A, c, e, nrow, nrow_ok)) {
             trial++;
             continue;
         }
         w.step++;
         update_state(s, w, cur, e, c);
+        kkw_prog_keep_row(c, cur, beg, deg);
         cur = e.neighbour;
-        grp.v[w.step & 3u] = cur;
+        grp.set(w.step, cur);
         if ((w.step & 3u) == 3u) grp.store(path, w.step);
-        kkw_prog_row(c.row_ptr, cur, beg, deg);
+        kkw_prog_next_row(c.row_ptr, cur, nrow_ok, nrow, beg, deg);
         trial = 0;
     }
     if ((w.step & 3u) != 3u) grp.store(path, w.step);   // the last, partial group
@@ -220,7 +288,7 @@ __device__ __forceinline__ void kkw_prog_flush(const uint32_t *buf, uint32_t *pa
 // lanes are idle, so every wave reaches the exit.
 extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk_persistent(kkw_prog_args A)
 {
-    const kkw_ctx &c = A.c;
+    kkw_ctx c = A.c;   // per lane: the kept row changes as the walker moves
     const uint32_t lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     bool active = false, drained = false;
@@ -257,6 +325,7 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk_pe
                 w.id = A.walker_begin + i;
                 w.step = 0;
                 cur = kkw_prog_start(A, w.id);
+                kkw_prog_keep_row(c, KKW_NO_VERTEX, 0, 0);
                 init_state(s, w, cur, c);
                 path = A.paths + (uint64_t)i * A.stride;
                 if (lines) lbuf[0] = cur;
@@ -273,7 +342,7 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk_pe
         if (active) {
             bool done = false, capped = false;
             if (trial == 0 || A.retest_ext) {
-                if (kkw_prog_terminates(s, w, cur, deg, A.retest_ext ? trial : 0u, A)) {
+                if (kkw_prog_terminates(s, w, cur, deg, A.retest_ext ? trial : 0u, A, c)) {
                     done = true;
                 } else if (w.step >= A.max_len) {
                     done = capped = true;
@@ -282,19 +351,22 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk_pe
             if (!done && trial >= KKW_PROG_MAX_TRIALS) done = capped = true;
             if (!done) {
                 kkw_edge e;
-                if (kkw_prog_trial(s, w, cur, beg, deg, trial, A, e)) {
+                uint64_t nrow;
+                bool nrow_ok;
+                if (kkw_prog_trial(s, w, cur, beg, deg, trial, A, c, e, nrow, nrow_ok)) {
                     w.step++;
                     update_state(s, w, cur, e, c);
+                    kkw_prog_keep_row(c, cur, beg, deg);
                     cur = e.neighbour;
                     if (lines) {
                         lbuf[w.step & (KKW_PROG_HALF_LINE - 1)] = cur;
                         if ((w.step & (KKW_PROG_HALF_LINE - 1)) == KKW_PROG_HALF_LINE - 1)
                             kkw_prog_flush(lbuf, path, w.step);
                     } else {
-                        grp.v[w.step & 3u] = cur;
+                        grp.set(w.step, cur);
                         if ((w.step & 3u) == 3u) grp.store(path, w.step);
                     }
-                    kkw_prog_row(c.row_ptr, cur, beg, deg);
+                    kkw_prog_next_row(c.row_ptr, cur, nrow_ok, nrow, beg, deg);
                     trial = 0;
                 } else {
                     trial++;

@@ -1,5 +1,11 @@
 // DeepWalk as a walk program (src/apps/deepwalk.hpp:30-43): truncated static
 // walk, no dynamic component.  param[0] = walk length.
+// Reads no edge index: draws come from the engine's tables of vertex ids
+// and rows (KKW_EDGE_INDEX 0).  A -DKKW_EDGE_INDEX=1 build uses the
+// local-index tables instead; paths are the same.
+#ifndef KKW_EDGE_INDEX
+#define KKW_EDGE_INDEX 0
+#endif
 struct WalkerState {
     uint32_t unused;
 };

@@ -4,6 +4,12 @@
 // The adjacency test uses the engine's neighbour hash sets (kkw_adjacent).
 // param: 0 walk length, 1 upper, 2 lower, 3 1/p, 4 1/q, 5 overflow
 // (1/p - upper), 6 weighted (1) or not (0).
+// Reads no edge index: draws come from the engine's tables of vertex ids
+// and rows (KKW_EDGE_INDEX 0).  A -DKKW_EDGE_INDEX=1 build uses the
+// local-index tables instead; paths are the same.
+#ifndef KKW_EDGE_INDEX
+#define KKW_EDGE_INDEX 0
+#endif
 #define KKW_DYNAMIC 1
 #define KKW_LOWER 1
 #define KKW_OUTLIER 1

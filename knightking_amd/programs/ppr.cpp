@@ -1,5 +1,11 @@
 // PPR as a walk program (src/apps/ppr.hpp:30-59): extension = 1 - t, one
 // coin per step.  param[0] = 1 - t (as the float the library computes).
+// Reads no edge index: draws come from the engine's tables of vertex ids
+// and rows (KKW_EDGE_INDEX 0).  A -DKKW_EDGE_INDEX=1 build uses the
+// local-index tables instead; paths are the same.
+#ifndef KKW_EDGE_INDEX
+#define KKW_EDGE_INDEX 0
+#endif
 struct WalkerState {
     uint32_t unused;
 };

@@ -44,6 +44,9 @@ extern "C" {
 #ifndef KKW_LOWER
 #define KKW_LOWER 0
 #endif
+#ifndef KKW_EDGE_INDEX
+#define KKW_EDGE_INDEX 1   /* 0: the program never reads e.index of a static draw */
+#endif
 #ifndef KKW_OUTLIER
 #define KKW_OUTLIER 0
 #endif
@@ -127,6 +130,7 @@ bool trial_at(const WalkerState &s, const kkw_walker &w, uint32_t cur, uint64_t 
     e.index = beg + local;
     e.neighbour = c.col[e.index];
     e.reserved = 0;
+    if (!KKW_EDGE_INDEX) e.index = KKW_NO_EDGE_INDEX;
 #if KKW_DYNAMIC
     const float dart = orc_u01(r.w[3]) * up;
     bool ok = false;
@@ -154,6 +158,9 @@ extern "C" int orc_program_walk(uint32_t V, const uint64_t *row_ptr, const uint3
 {
     Job J;
     J.c.v_num = V;
+    J.c.row_v = KKW_NO_VERTEX;   // no kept row: kkw_adjacent reads row_ptr
+    J.c.row_deg = 0;
+    J.c.row_beg = 0;
     J.c.sorted = sorted ? 1u : 0u;
     J.c.adjset = 0;   // kkw_adjacent answers by binary search or scan here
     J.c.e_num = row_ptr[V];

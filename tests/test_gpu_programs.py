@@ -15,6 +15,9 @@ pytestmark = pytest.mark.gpu
 
 # both launch forms of a program: one lane per walker, and lanes refilled from a work queue
 MODES = [pytest.param(kk.MODE_PLAIN, id="plain"), pytest.param(kk.MODE_PERSISTENT, id="persistent")]
+# tables of the app programs: the engine's (ids + rows; the programs' own
+# KKW_EDGE_INDEX 0) and local-index tables (rebuilt with KKW_EDGE_INDEX 1)
+TABLES = [pytest.param(None, id="id_tables"), pytest.param("-DKKW_EDGE_INDEX=1", id="index_tables")]
 
 
 @pytest.fixture(scope="module")
@@ -27,13 +30,13 @@ def source(name):
         return f.read()
 
 
-def both(engine, build_dir, name, rp, col, w=None, et=None, arrays=(), **run):
+def both(engine, build_dir, name, rp, col, w=None, et=None, arrays=(), options=None, **run):
     """Run program `name` on the GPU and on the oracle; return both results."""
-    print(f"[gpu] program {name} V={len(rp) - 1} E={len(col)} {run}", flush=True)
+    print(f"[gpu] program {name} V={len(rp) - 1} E={len(col)} options={options} {run}", flush=True)
     engine.graph_from_csr(rp, col, w)
     if et is not None:
         engine.set_edge_types(et)
-    prog = engine.program(source(name))
+    prog = engine.program(source(name), options)
     try:
         for k, a in enumerate(arrays):
             prog.set_array(k, a)
@@ -56,33 +59,37 @@ def check(got, want):
     assert got.steps == int((lw.astype(np.int64) - 1).sum())
 
 
+@pytest.mark.parametrize("tables", TABLES)
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("weighted,L", [(True, 40), (False, 40), (True, 80)])
-def test_deepwalk_program(engine, build_dir, weighted, L, mode):
+def test_deepwalk_program(engine, build_dir, weighted, L, mode, tables):
     # L = 80: persistent rows of whole lines (half-line write-out from LDS)
     rp, col, w = O.rmat_graph(12, 16, seed=3, weighted=True)
     check(*both(engine, build_dir, "deepwalk", rp, col, w, walker_num=20000, max_length=L, params=[L],
-                use_weights=weighted, seed=9, mode=mode))
+                use_weights=weighted, seed=9, mode=mode, options=tables))
 
 
+@pytest.mark.parametrize("tables", TABLES)
 @pytest.mark.parametrize("mode", MODES)
-def test_ppr_program_uniform_start(engine, build_dir, mode):
+def test_ppr_program_uniform_start(engine, build_dir, mode, tables):
     rp, col, w = O.rmat_graph(12, 16, seed=4, weighted=True)
     t = 0.15
     ext = np.float32(1.0) - np.float32(t)
     check(*both(engine, build_dir, "ppr", rp, col, w, walker_num=30000, max_length=O.ppr_default_cap(t),
-                params=[ext], use_weights=True, seed=2, uniform_start=True, walker_begin=77, mode=mode))
+                params=[ext], use_weights=True, seed=2, uniform_start=True, walker_begin=77, mode=mode,
+                options=tables))
 
 
+@pytest.mark.parametrize("tables", TABLES)
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("p,q,weighted", [(2.0, 0.5, True), (0.5, 2.0, True), (0.25, 1.0, False)])
-def test_node2vec_program_matches_oracle_and_builtin_kernel(engine, build_dir, p, q, weighted, mode):
+def test_node2vec_program_matches_oracle_and_builtin_kernel(engine, build_dir, p, q, weighted, mode, tables):
     rp, col, w = O.rmat_graph(12, 16, seed=5, weighted=True)
     L = 40
     up, lo, ip, iq, ov = O.node2vec_constants(p, q)
     got, want = both(engine, build_dir, "node2vec", rp, col, w, walker_num=8192, max_length=L,
                      params=[L, up, lo, ip, iq, ov, 1.0 if weighted else 0.0], use_weights=weighted, seed=11,
-                     second_order=True, mode=mode)
+                     second_order=True, mode=mode, options=tables)
     check(got, want)
     (builtin,) = engine.run([kk.node2vec_job(8192, L, p, q, weighted, seed=11)], mode=kk.MODE_PLAIN)
     same_paths(got.vertices, got.lengths, builtin.vertices, builtin.lengths)
@@ -119,6 +126,58 @@ def test_return_bias_program_start_list_and_cap(engine, build_dir, karate, mode)
     rp, col, ww = O.csr_from_edges(34, src, dst, w, False, True)
     check(*both(engine, build_dir, "return_bias", rp, col, ww, walker_num=5000, max_length=7,
                 params=[1000], use_weights=True, seed=6, start_vertices=np.array([0, 33, 5], np.uint32), mode=mode))
+
+
+# A static draw of a program built with KKW_EDGE_INDEX 0 carries
+# KKW_NO_EDGE_INDEX; an outlier edge keeps its index.  The walker stops at
+# the first step whose edge broke that rule, so full-length paths say it held.
+EDGE_INDEX_PROBE = """
+#define KKW_EDGE_INDEX 0
+#define KKW_DYNAMIC 1
+#define KKW_OUTLIER 1
+struct WalkerState { uint32_t ok; };
+KKW_FN void init_state(WalkerState &s, const kkw_walker &, uint32_t, const kkw_ctx &) { s.ok = 1; }
+KKW_FN void update_state(WalkerState &s, const kkw_walker &, uint32_t cur, const kkw_edge &e, const kkw_ctx &c)
+{
+    const bool outlier = e.index != KKW_NO_EDGE_INDEX;
+    if (outlier && (e.index != c.row_ptr[cur] || e.neighbour != c.col[e.index])) s.ok = 0;
+}
+KKW_FN float extension(const WalkerState &s, const kkw_walker &w, uint32_t, const kkw_ctx &c)
+{
+    return (s.ok && w.step < (uint32_t)c.param[0]) ? 1.0f : 0.0f;
+}
+KKW_FN float dynamic_comp(const WalkerState &, const kkw_walker &, uint32_t, const kkw_edge &e, const kkw_ctx &)
+{
+    return e.index != KKW_NO_EDGE_INDEX ? 1e30f : 1.0f;   // outlier darts reach above the upper bound
+}
+KKW_FN float upper_bound(uint32_t, const kkw_ctx &) { return 1.0f; }
+KKW_FN void outlier_bound(const WalkerState &, const kkw_walker &, uint32_t, float &overflow, uint32_t &num,
+                          const kkw_ctx &)
+{
+    overflow = 0.5f;
+    num = 1;
+}
+KKW_FN int64_t outlier_search(const WalkerState &, const kkw_walker &, uint32_t cur, uint32_t, const kkw_ctx &c)
+{
+    return (int64_t)c.row_ptr[cur];   // the first edge of the list
+}
+"""
+
+
+@pytest.mark.parametrize("weighted", [True, False])
+def test_edge_index_sentinel_of_id_tables(engine, weighted):
+    rp, col, w = O.rmat_graph(10, 8, seed=7, weighted=True)
+    engine.graph_from_csr(rp, col, w)
+    prog = engine.program(EDGE_INDEX_PROBE)
+    try:
+        got = prog.run(4096, 20, params=[20], use_weights=weighted, seed=3, mode=kk.MODE_PERSISTENT)
+    finally:
+        prog.close()
+    deg = np.diff(rp.astype(np.int64))
+    starts = np.arange(4096) % (len(rp) - 1)
+    live = deg[starts] > 0
+    assert np.all(got.lengths[live] == 21)   # every step kept the rule
+    assert np.all(got.lengths[~live] == 1)
 
 
 def test_program_errors(engine):
