@@ -493,13 +493,19 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
 // entries go to the tails in pairing order (ballot again).  This is exactly the
 // sequential FIFO run k steps at a time.  Queue windows of AW_WIN entries live
 // in LDS; lists longer than that also keep their queues in global memory and
-// refill the windows from there.  The pairing works on local edge indices;
-// each bucket takes its neighbour ids and edge types as it is written
-// (finish_bucket; the tables of walk programs keep the indices and add the ids).
+// refill the windows from there.  The pairing works on local edge indices
+// and reads nothing from global memory, so a batch costs only its LDS round
+// trip: each bucket is written with its local indices, the list's first edge
+// in prow and AW_MARK in meta, and k_finish_wave_buckets then gives the
+// marked buckets their neighbour ids, rows and edge types in one edge-parallel
+// pass (finish_bucket; the tables of walk programs keep the indices and add
+// the ids).
 constexpr uint32_t AW_MIN = 32;
 constexpr uint32_t AW_WIN = 512;
 constexpr int AW_WAVES = 4;
 constexpr uint32_t AW_SUM_CHUNK = 256;
+constexpr uint32_t AW_Q_CHUNKS = 4;          // chunks of 64 weights per load group when queueing
+constexpr uint32_t AW_MARK = 0xFFFFFFFFu;   // meta of a bucket still holding local indices (real meta < 2^16)
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -534,12 +540,9 @@ __device__ __forceinline__ void finish_bucket(Bucket &bk, uint64_t b, const uint
 
 __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__restrict__ list, uint32_t n_list,
                                                             const uint64_t *__restrict__ row_ptr,
-                                                            const uint32_t *__restrict__ col,
                                                             const float *__restrict__ w,
-                                                            const uint8_t *__restrict__ etype,
                                                             Bucket *__restrict__ out, float *__restrict__ vsum,
-                                                            uint2 *__restrict__ qs, uint2 *__restrict__ ql,
-                                                            bool to_ids, const uint64_t *__restrict__ nrow)
+                                                            uint2 *__restrict__ qs, uint2 *__restrict__ ql)
 {
     __shared__ uint2 win_s[AW_WAVES][AW_WIN], win_l[AW_WAVES][AW_WIN];
     __shared__ float stage[AW_WAVES][AW_SUM_CHUNK];
@@ -579,27 +582,38 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
     if (lane == 0) vsum[v] = sum;
     const float ave = sum / (float)d;
 
-    // 2. small / large queues in edge order (walk.hpp:601-615)
+    // 2. small / large queues in edge order (walk.hpp:601-615); the weights
+    // of AW_Q_CHUNKS chunks of 64 are loaded together (one wait per group)
     uint32_t st = 0, lt = 0;
-    for (uint32_t base = 0; base < d; base += 64) {
-        const uint32_t e = base + lane;
-        const bool valid = e < d;
-        const float p = valid ? wv[e] : 0.0f;
-        const bool small = valid && p < ave, large = valid && !(p < ave);
-        const uint64_t ms = __ballot(small), ml = __ballot(large);
-        const uint2 ent = make_uint2(e, __float_as_uint(p));
-        if (small) {
-            const uint32_t pos = st + (uint32_t)__popcll(ms & lt_mask);
-            if (spill) S[pos] = ent;
-            if (pos < AW_WIN) sS[pos] = ent;
+    for (uint32_t gbase = 0; gbase < d; gbase += 64 * AW_Q_CHUNKS) {
+        float pc[AW_Q_CHUNKS];
+#pragma unroll
+        for (uint32_t j = 0; j < AW_Q_CHUNKS; j++) {
+            const uint32_t e = gbase + j * 64 + lane;
+            pc[j] = e < d ? wv[e] : 0.0f;
         }
-        if (large) {
-            const uint32_t pos = lt + (uint32_t)__popcll(ml & lt_mask);
-            if (spill) L[pos] = ent;
-            if (pos < AW_WIN) sL[pos] = ent;
+#pragma unroll
+        for (uint32_t j = 0; j < AW_Q_CHUNKS; j++) {   // chunks past the end queue nothing
+            const uint32_t base = gbase + j * 64;
+            const uint32_t e = base + lane;
+            const bool valid = e < d;
+            const float p = pc[j];
+            const bool small = valid && p < ave, large = valid && !(p < ave);
+            const uint64_t ms = __ballot(small), ml = __ballot(large);
+            const uint2 ent = make_uint2(e, __float_as_uint(p));
+            if (small) {
+                const uint32_t pos = st + (uint32_t)__popcll(ms & lt_mask);
+                if (spill) S[pos] = ent;
+                if (pos < AW_WIN) sS[pos] = ent;
+            }
+            if (large) {
+                const uint32_t pos = lt + (uint32_t)__popcll(ml & lt_mask);
+                if (spill) L[pos] = ent;
+                if (pos < AW_WIN) sL[pos] = ent;
+            }
+            st += (uint32_t)__popcll(ms);
+            lt += (uint32_t)__popcll(ml);
         }
-        st += (uint32_t)__popcll(ms);
-        lt += (uint32_t)__popcll(ml);
     }
     wave_sync();
 
@@ -610,10 +624,20 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
         const uint32_t av_s = (st < ws + AW_WIN ? st : ws + AW_WIN) - sh;
         const uint32_t av_l = paired ? (lt < wl + AW_WIN ? lt : wl + AW_WIN) - lh : 64u;
         if ((av_s < 64 && st > ws + AW_WIN) || (av_l < 64 && paired && lt > wl + AW_WIN)) {
-            // refill both windows from the global queues at the current heads
-            for (uint32_t i = lane; i < AW_WIN; i += 64) {
-                if (sh + i < st) sS[i] = S[sh + i];
-                if (lh + i < lt) sL[i] = L[lh + i];
+            // refill both windows from the global queues at the current heads:
+            // all loads first, then the LDS writes (one wait per refill)
+            uint2 rs[AW_WIN / 64], rl[AW_WIN / 64];
+#pragma unroll
+            for (uint32_t j = 0; j < AW_WIN / 64; j++) {
+                const uint32_t i = j * 64 + lane;
+                rs[j] = sh + i < st ? S[sh + i] : make_uint2(0, 0);
+                rl[j] = lh + i < lt ? L[lh + i] : make_uint2(0, 0);
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < AW_WIN / 64; j++) {
+                const uint32_t i = j * 64 + lane;
+                if (sh + i < st) sS[i] = rs[j];
+                if (lh + i < lt) sL[i] = rl[j];
             }
             ws = sh;
             wl = lh;
@@ -624,8 +648,8 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
         k = k < 64 ? k : 64;
         const bool act = lane < k;
         Bucket bk;
-        bk.meta = 0;
-        bk.prow = 0;
+        bk.meta = AW_MARK;
+        bk.prow = b;
         bk.arow = 0;
         bool to_s = false, to_l = false;
         uint2 ne = make_uint2(0, 0);
@@ -647,7 +671,6 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
                 bk.p = 1.0f;
                 bk.alias = se.x;
             }
-            finish_bucket(bk, b, col, etype, to_ids, nrow);
             o[ob + lane] = bk;
         }
         const uint64_t ms = __ballot(to_s), ml = __ballot(to_l);
@@ -676,12 +699,36 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
         bk.p = 1.0f;
         bk.primary = e;
         bk.alias = e;
-        bk.meta = 0;
-        bk.prow = 0;
+        bk.meta = AW_MARK;
+        bk.prow = b;
         bk.arow = 0;
-        finish_bucket(bk, b, col, etype, to_ids, nrow);
         o[ob + i] = bk;
     }
+}
+
+typedef uint32_t rows_u32x4 __attribute__((ext_vector_type(4), aligned(4)));
+
+// Buckets of the wave-built lists (meta == AW_MARK, prow = the list's first
+// edge): local indices become ids, rows and edge types (finish_bucket).  One
+// lane per bucket; buckets of the shorter lists are final and only read.
+__global__ void k_finish_wave_buckets(Bucket *__restrict__ out, uint64_t E, const uint32_t *__restrict__ col,
+                                      const uint8_t *__restrict__ etype, bool to_ids,
+                                      const uint64_t *__restrict__ nrow)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= E) return;
+    uint32_t *bw = reinterpret_cast<uint32_t *>(out + k);
+    const rows_u32x4 h1 = *reinterpret_cast<const rows_u32x4 *>(bw + 4);   // {alias, meta, arow}
+    if (h1.y != AW_MARK) return;
+    const rows_u32x4 h0 = *reinterpret_cast<const rows_u32x4 *>(bw);       // {p, primary, prow = list start}
+    Bucket bk;
+    bk.p = __uint_as_float(h0.x);
+    bk.primary = h0.y;
+    bk.alias = h1.x;
+    bk.prow = 0;
+    bk.arow = 0;
+    finish_bucket(bk, (uint64_t)h0.z | ((uint64_t)h0.w << 32), col, etype, to_ids, nrow);
+    out[k] = bk;
 }
 
 // Neighbour hash sets of lists with 1..ADJ_LDS_MAX_DEG entries: one wave per
@@ -718,8 +765,6 @@ __global__ void k_list_big(const uint64_t *row_ptr, uint32_t V, uint32_t *big, u
 // Packed row of every edge's neighbour, nrow[k] = row of col[k], for the
 // alias kernels to copy into buckets (each bucket's candidates are neighbours
 // of its list: E random row reads instead of two per bucket).
-typedef uint32_t rows_u32x4 __attribute__((ext_vector_type(4), aligned(4)));
-
 __device__ __forceinline__ uint64_t row_of(const uint64_t *row_ptr, uint32_t v)
 {
     const rows_u32x4 q = *reinterpret_cast<const rows_u32x4 *>(row_ptr + v);   // row_ptr[v], row_ptr[v + 1]
@@ -750,16 +795,28 @@ __global__ void k_uniform_buckets(const uint32_t *col, const uint8_t *etype, con
     out[e] = b;
 }
 
-// One atomic per wave: the wave's maximum is taken with shuffles first.
-__global__ void k_max_degree(const uint64_t *row_ptr, uint32_t V, unsigned long long *out)
+// Maximum degree: a bounded grid strides over the vertices, each block
+// reduces with shuffles and LDS, and only block maxima meet in one atomic
+// (same-address atomics serialise in L2).  Launched with 256 threads a block.
+__global__ void __launch_bounds__(256) k_max_degree(const uint64_t *row_ptr, uint32_t V, unsigned long long *out)
 {
-    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long d = v < V ? (unsigned long long)(row_ptr[v + 1] - row_ptr[v]) : 0ull;
+    __shared__ unsigned long long wmax[4];
+    unsigned long long d = 0;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long x = row_ptr[v + 1] - row_ptr[v];
+        d = x > d ? x : d;
+    }
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long o = __shfl_xor(d, off, 64);
         d = o > d ? o : d;
     }
-    if ((threadIdx.x & 63) == 0 && d) atomicMax(out, d);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = wmax[0];
+        for (int k = 1; k < 4; k++) m = wmax[k] > m ? wmax[k] : m;
+        if (m) atomicMax(out, m);
+    }
 }
 
 unsigned long long read_counter(unsigned long long *d, hipStream_t s)
@@ -796,7 +853,8 @@ void compute_packable(GraphStore &gs, hipStream_t s)
     if (gs.V == 0 || gs.E >= (1ull << ROW_DEG_SHIFT)) return;
     DArr<unsigned long long> mx(1);
     KKW_HIP(hipMemsetAsync(mx.p, 0, sizeof(unsigned long long), s));
-    hipLaunchKernelGGL(k_max_degree, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V, mx.p);
+    hipLaunchKernelGGL(k_max_degree, dim3(std::min<uint64_t>(grid_for(gs.V, 256), 2048)), dim3(256), 0, s, gs.row_ptr,
+                       gs.V, mx.p);
     KKW_HIP(hipGetLastError());
     gs.packable = read_counter(mx.p, s) < (1ull << (64 - ROW_DEG_SHIFT));
 }
@@ -1179,9 +1237,12 @@ void build_alias(GraphStore &gs, hipStream_t s, bool local)
         KKW_HIP(hipStreamSynchronize(s));
         if (n_big)
             hipLaunchKernelGGL(k_alias_wave, dim3(grid_for(n_big, AW_WAVES)), dim3(64 * AW_WAVES), 0, s, big.p,
-                               n_big, gs.row_ptr, gs.col, gs.w, gs.etype, t.p, vsum, qs.p, ql.p, !local, nrow.p);
+                               n_big, gs.row_ptr, gs.w, t.p, vsum, qs.p, ql.p);
         hipLaunchKernelGGL(k_alias, dim3(grid_for(gs.V, 64)), dim3(64), 0, s, gs.row_ptr, gs.col, gs.w,
                            gs.etype, gs.V, t.p, vsum, qs.p, ql.p, AW_MIN, !local, nrow.p);
+        if (n_big)
+            hipLaunchKernelGGL(k_finish_wave_buckets, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, t.p, gs.E, gs.col,
+                               gs.etype, !local, nrow.p);
         KKW_HIP(hipGetLastError());
         KKW_HIP(hipStreamSynchronize(s));
     }
