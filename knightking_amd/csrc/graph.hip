@@ -308,11 +308,14 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n)
     if (i < n) p[i] = v;
 }
 
-// Neighbour hash sets are built in two tiers by list length: lists of up to
-// ADJ_LDS_MAX_DEG entries one wave each in LDS (k_adjset_small: each table's
-// lines leave once, instead of every insert fetching and dirtying a random
-// line of HBM), longer lists one lane per edge with global atomics.
-constexpr uint32_t ADJ_LDS_MAX_DEG = 255;   // table of at most 4 * 255 + 4 = 1024 slots
+// Neighbour hash sets are built in three tiers by list length: lists of up
+// to ADJ_LDS_MAX_DEG entries one wave each in LDS (k_adjset_small), lists of
+// up to ADJ_MID_MAX_DEG one workgroup each in LDS (k_adjset_mid) -- each
+// table's lines leave once, instead of every insert fetching and dirtying a
+// random line of HBM -- and longer lists one lane per edge with global
+// atomics.
+constexpr uint32_t ADJ_LDS_MAX_DEG = 255;    // table of at most 4 * 255 + 4 = 1024 slots
+constexpr uint32_t ADJ_MID_MAX_DEG = 4095;   // table of at most 16384 slots (64 KiB of LDS)
 
 __device__ __forceinline__ void adj_insert(uint32_t *tab, uint32_t size, uint32_t u)
 {
@@ -336,7 +339,7 @@ __global__ void k_adjset_insert(const uint64_t *row_ptr, const uint32_t *col, ui
     }
     const uint32_t v = lo;
     const uint64_t b = row_ptr[v], deg = row_ptr[v + 1] - b;
-    if (deg <= ADJ_LDS_MAX_DEG) return;   // k_adjset_small's
+    if (deg <= ADJ_MID_MAX_DEG) return;   // k_adjset_small's or k_adjset_mid's
     adj_insert(t + adj_table_start(b, v), adj_table_size(deg), col[e]);
 }
 
@@ -753,6 +756,44 @@ __global__ void __launch_bounds__(256) k_adjset_small(const uint64_t *row_ptr, c
         for (uint32_t i = lane; i < size; i += 64) gt[i] = lt[i];
         wave_sync();   // the next vertex reuses the LDS table
     }
+}
+
+// Lists of ADJ_LDS_MAX_DEG + 1 .. MAX_DEG entries: one workgroup per list
+// (list ids from k_list_range, *n of them), the table built with LDS atomics
+// and written out in 16-byte groups (tables start 16-byte aligned and hold a
+// multiple of four slots).  A 128 KiB tier (lists up to 8191 entries, 1024
+// lanes) cut the atomics kernel from 10.2 to 8.5 ms but made every later
+// rebuild in the process take 0.2-0.5 s of host time, so tables stop at 64 KiB.
+template <uint32_t MAX_DEG, uint32_t BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_adjset_mid(const uint32_t *__restrict__ list, const unsigned int *n,
+                                                      const uint64_t *__restrict__ row_ptr,
+                                                      const uint32_t *__restrict__ col, uint32_t *__restrict__ t)
+{
+    __shared__ uint4 tab4[(4 * MAX_DEG + 4) / 4];
+    uint32_t *tab = reinterpret_cast<uint32_t *>(tab4);
+    const uint32_t count = *n;
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {   // workgroup-uniform
+        const uint32_t v = list[i];
+        const uint64_t b = row_ptr[v], e = row_ptr[v + 1];
+        const uint32_t size = adj_table_size(e - b);
+        for (uint32_t k = threadIdx.x; k < size; k += blockDim.x) tab[k] = ADJ_EMPTY;
+        __syncthreads();
+        for (uint64_t k = b + threadIdx.x; k < e; k += blockDim.x) adj_insert(tab, size, col[k]);
+        __syncthreads();
+        uint4 *gt = reinterpret_cast<uint4 *>(t + adj_table_start(b, v));
+        for (uint32_t k = threadIdx.x; k < size / 4; k += blockDim.x) gt[k] = tab4[k];
+        __syncthreads();   // the next list reuses the LDS table
+    }
+}
+
+// ids of the vertices whose lists hold lo .. hi entries, in no particular order
+__global__ void k_list_range(const uint64_t *row_ptr, uint32_t V, uint64_t lo, uint64_t hi, uint32_t *out,
+                             unsigned int *n)
+{
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const uint64_t d = row_ptr[v + 1] - row_ptr[v];
+    if (d >= lo && d <= hi) out[atomicAdd(n, 1u)] = (uint32_t)v;
 }
 
 __global__ void k_list_big(const uint64_t *row_ptr, uint32_t V, uint32_t *big, unsigned int *n_big)
@@ -1181,10 +1222,20 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
     if (gs.adjset) return;
     const uint64_t n = adj_table_start(gs.E, gs.V) + 4;   // every table ends by slot 4 (E + V)
     DArr<uint32_t> t(n);
+    DArr<uint32_t> mid;         // freed after the stream is synchronised below
+    DArr<unsigned int> n_mid;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, t.p, ADJ_EMPTY, n);
     if (gs.E) {
         const uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)gs.V + 3) / 4, 1u << 16);   // 4 waves each
         hipLaunchKernelGGL(k_adjset_small, dim3(blocks), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V, t.p);
+        mid.p = dalloc<uint32_t>(gs.V);
+        n_mid.p = dalloc<unsigned int>(1);
+        KKW_HIP(hipMemsetAsync(n_mid.p, 0, sizeof(unsigned int), s));
+        hipLaunchKernelGGL(k_list_range, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V,
+                           (uint64_t)ADJ_LDS_MAX_DEG + 1, (uint64_t)ADJ_MID_MAX_DEG, mid.p, n_mid.p);
+        // two 64 KiB workgroups fit a CU's LDS; the grid strides over the lists
+        hipLaunchKernelGGL((k_adjset_mid<ADJ_MID_MAX_DEG, 256>), dim3(std::min<uint64_t>(gs.V, 4096)), dim3(256), 0,
+                           s, mid.p, n_mid.p, gs.row_ptr, gs.col, t.p);
         hipLaunchKernelGGL(k_adjset_insert, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V,
                            gs.E, t.p);
     }
