@@ -301,19 +301,14 @@ __global__ void k_check_symmetric(const uint64_t *row_ptr, const uint32_t *col, 
     }
 }
 
-// neighbour hash sets (layout in kkw_dev.h)
-__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n)
-{
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-// Neighbour hash sets are built in three tiers by list length: lists of up
+// Neighbour hash sets (layout in kkw_dev.h) are built in three tiers by list
+// length, every table written by exactly one builder: lists of up
 // to ADJ_LDS_MAX_DEG entries one wave each in LDS (k_adjset_small), lists of
 // up to ADJ_MID_MAX_DEG one workgroup each in LDS (k_adjset_mid) -- each
 // table's lines leave once, instead of every insert fetching and dirtying a
 // random line of HBM -- and longer lists one lane per edge with global
-// atomics.
+// atomics into tables k_adjset_clear has emptied.  Empty lists get four empty
+// slots from k_adjset_small.
 constexpr uint32_t ADJ_LDS_MAX_DEG = 255;    // table of at most 4 * 255 + 4 = 1024 slots
 constexpr uint32_t ADJ_MID_MAX_DEG = 4095;   // table of at most 16384 slots (64 KiB of LDS)
 
@@ -746,7 +741,13 @@ __global__ void __launch_bounds__(256) k_adjset_small(const uint64_t *row_ptr, c
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
     for (uint64_t v = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave; v < V; v += waves) {   // wave-uniform
         const uint64_t b = row_ptr[v], e = row_ptr[v + 1];
-        if (e == b || e - b > ADJ_LDS_MAX_DEG) continue;
+        if (e == b) {   // an empty list's table: four empty slots
+            if (lane == 0)
+                *reinterpret_cast<uint4 *>(t + adj_table_start(b, v)) = make_uint4(ADJ_EMPTY, ADJ_EMPTY, ADJ_EMPTY,
+                                                                                   ADJ_EMPTY);
+            continue;
+        }
+        if (e - b > ADJ_LDS_MAX_DEG) continue;
         const uint32_t size = adj_table_size(e - b);
         for (uint32_t i = lane; i < size; i += 64) lt[i] = ADJ_EMPTY;
         wave_sync();
@@ -783,6 +784,23 @@ __global__ void __launch_bounds__(BLOCK) k_adjset_mid(const uint32_t *__restrict
         uint4 *gt = reinterpret_cast<uint4 *>(t + adj_table_start(b, v));
         for (uint32_t k = threadIdx.x; k < size / 4; k += blockDim.x) gt[k] = tab4[k];
         __syncthreads();   // the next list reuses the LDS table
+    }
+}
+
+// Tables of the lists k_adjset_insert fills (list ids from k_list_range, *n
+// of them) set to empty, one workgroup per table in 16-byte stores; every
+// other table is written whole by its builder.
+__global__ void __launch_bounds__(256) k_adjset_clear(const uint32_t *__restrict__ list, const unsigned int *n,
+                                                      const uint64_t *__restrict__ row_ptr, uint32_t *__restrict__ t)
+{
+    const uint32_t count = *n;
+    const uint4 empty = make_uint4(ADJ_EMPTY, ADJ_EMPTY, ADJ_EMPTY, ADJ_EMPTY);
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const uint32_t v = list[i];
+        const uint64_t b = row_ptr[v];
+        const uint32_t size = adj_table_size(row_ptr[v + 1] - b);
+        uint4 *gt = reinterpret_cast<uint4 *>(t + adj_table_start(b, v));
+        for (uint32_t k = threadIdx.x; k < size / 4; k += blockDim.x) gt[k] = empty;
     }
 }
 
@@ -1222,12 +1240,13 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
     if (gs.adjset) return;
     const uint64_t n = adj_table_start(gs.E, gs.V) + 4;   // every table ends by slot 4 (E + V)
     DArr<uint32_t> t(n);
-    DArr<uint32_t> mid;         // freed after the stream is synchronised below
-    DArr<unsigned int> n_mid;
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, t.p, ADJ_EMPTY, n);
-    if (gs.E) {
+    DArr<uint32_t> mid, big;    // freed after the stream is synchronised below
+    DArr<unsigned int> n_mid, n_big;
+    if (gs.V) {   // lists up to ADJ_LDS_MAX_DEG entries, and the empty lists' tables
         const uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)gs.V + 3) / 4, 1u << 16);   // 4 waves each
         hipLaunchKernelGGL(k_adjset_small, dim3(blocks), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V, t.p);
+    }
+    if (gs.E) {
         mid.p = dalloc<uint32_t>(gs.V);
         n_mid.p = dalloc<unsigned int>(1);
         KKW_HIP(hipMemsetAsync(n_mid.p, 0, sizeof(unsigned int), s));
@@ -1236,6 +1255,14 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
         // two 64 KiB workgroups fit a CU's LDS; the grid strides over the lists
         hipLaunchKernelGGL((k_adjset_mid<ADJ_MID_MAX_DEG, 256>), dim3(std::min<uint64_t>(gs.V, 4096)), dim3(256), 0,
                            s, mid.p, n_mid.p, gs.row_ptr, gs.col, t.p);
+        // longer lists: their tables cleared, then one lane per edge inserts
+        big.p = dalloc<uint32_t>(gs.V);
+        n_big.p = dalloc<unsigned int>(1);
+        KKW_HIP(hipMemsetAsync(n_big.p, 0, sizeof(unsigned int), s));
+        hipLaunchKernelGGL(k_list_range, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V,
+                           (uint64_t)ADJ_MID_MAX_DEG + 1, ~0ull, big.p, n_big.p);
+        hipLaunchKernelGGL(k_adjset_clear, dim3(std::min<uint64_t>(gs.V, 4096)), dim3(256), 0, s, big.p, n_big.p,
+                           gs.row_ptr, t.p);
         hipLaunchKernelGGL(k_adjset_insert, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V,
                            gs.E, t.p);
     }
