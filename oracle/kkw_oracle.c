@@ -2,7 +2,8 @@
  * kkw_oracle.c -- CPU restatement of KnightKing's random-walk hot path.
  *
  * TEST INFRASTRUCTURE ONLY (see kkw_oracle.h): used by tests/, by
- * __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker.
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, to compare the
+ * GPU results against.
  *
  * Build: gcc -O2 -fopenmp -ffp-contract=off -fPIC -shared (oracle/Makefile).
  * -ffp-contract=off keeps every float operation a separately rounded IEEE

@@ -1,10 +1,10 @@
 /*
  * kkw_oracle.h -- CPU restatement of KnightKing's random-walk hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is linked or
- * loaded by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline
- * leg, never by the product library (knightking_amd/libkkw.so), which must run
- * its HIP kernels or fail.
+ * TEST INFRASTRUCTURE ONLY.  This library is what GPU results are compared
+ * against: it is linked or loaded by tests/, by __graft_entry__.smoke() and
+ * by bench.py's cpu_baseline leg, never by the product library
+ * (knightking_amd/libkkw.so), which must run its HIP kernels or fail.
  *
  * Each function cites the reference file:line it restates (paths relative to
  * the KnightKing source tree).  The one deliberate difference from the

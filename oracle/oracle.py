@@ -2,7 +2,7 @@
 random-walk hot path).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, where it is the checker.  The product
+bench.py's cpu_baseline leg, which compare the GPU results with it.  The product
 (knightking_amd/libkkw.so) never calls it.
 """
 from __future__ import annotations

@@ -1,7 +1,7 @@
 /*
  * program_oracle.cpp -- CPU restatement of a walk program's walk loop.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for walk programs, compiled by the
+ * TEST INFRASTRUCTURE ONLY: the oracle for walk programs, compiled by the
  * tests together with one program's source (g++ -DKKW_PROGRAM_SOURCE=...,
  * oracle.program_oracle), never by the product library.
  *
