@@ -11,6 +11,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <cstdlib>
 #include <vector>
 
 namespace kkw {
@@ -306,11 +307,15 @@ __global__ void k_check_symmetric(const uint64_t *row_ptr, const uint32_t *col, 
 // to ADJ_LDS_MAX_DEG entries one wave each in LDS (k_adjset_small), lists of
 // up to ADJ_MID_MAX_DEG one workgroup each in LDS (k_adjset_mid) -- each
 // table's lines leave once, instead of every insert fetching and dirtying a
-// random line of HBM -- and longer lists one lane per edge with global
-// atomics into tables k_adjset_clear has emptied.  Empty lists get four empty
-// slots from k_adjset_small.
+// random line of HBM -- and longer lists in segments of ADJ_SEG slots, one
+// workgroup per segment in LDS (k_adjset_seg), the few keys whose probe runs
+// past a segment's end inserted afterwards with global atomics
+// (k_adjset_overflow).  Empty lists get four empty slots from
+// k_adjset_small.
 constexpr uint32_t ADJ_LDS_MAX_DEG = 255;    // table of at most 4 * 255 + 4 = 1024 slots
 constexpr uint32_t ADJ_MID_MAX_DEG = 4095;   // table of at most 16384 slots (64 KiB of LDS)
+constexpr uint32_t ADJ_SEG = 16384;          // slots of one LDS segment of a longer list's table
+constexpr uint64_t ADJ_OVERFLOW_CAP = 1u << 20;   // keys past a segment's end kept for k_adjset_overflow
 
 __device__ __forceinline__ void adj_insert(uint32_t *tab, uint32_t size, uint32_t u)
 {
@@ -322,20 +327,40 @@ __device__ __forceinline__ void adj_insert(uint32_t *tab, uint32_t size, uint32_
     }
 }
 
-__global__ void k_adjset_insert(const uint64_t *row_ptr, const uint32_t *col, uint32_t V, uint64_t E, uint32_t *t)
+// Keys of long lists whose probe ran past the end of their LDS segment
+// (k_adjset_seg), inserted with global atomics from their home slot on: the
+// probe passes the written segments and takes the first empty slot.  Filling
+// an empty slot never shortens another key's probe, so every key stays
+// reachable from its home slot.
+__global__ void k_adjset_overflow(const uint2 *__restrict__ ovf, const unsigned int *n_ovf, uint32_t cap,
+                                  const uint64_t *__restrict__ row_ptr, uint32_t *__restrict__ t)
 {
-    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = *n_ovf < cap ? *n_ovf : cap;   // beyond cap: k_adjset_insert_long
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint2 kv = ovf[i];   // {vertex, key}
+        const uint64_t b = row_ptr[kv.x];
+        adj_insert(t + adj_table_start(b, kv.x), adj_table_size(row_ptr[kv.x + 1] - b), kv.y);
+    }
+}
+
+// Every key of the lists longer than ADJ_MID_MAX_DEG inserted with global
+// atomics, one lane per edge (source by binary search over row_ptr): the
+// fallback when more keys ran past their segments than the overflow buffer
+// holds.  A key already in its table is found on its probe and left alone.
+__global__ void k_adjset_insert_long(const uint64_t *row_ptr, const uint32_t *col, uint32_t V, uint64_t E,
+                                     uint32_t *t)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     uint32_t lo = 0, hi = V - 1;   // source: last v with row_ptr[v] <= e
     while (lo < hi) {
-        uint32_t mid = lo + ((hi - lo + 1) >> 1);
+        const uint32_t mid = lo + ((hi - lo + 1) >> 1);
         if (row_ptr[mid] <= e) lo = mid;
         else hi = mid - 1;
     }
-    const uint32_t v = lo;
-    const uint64_t b = row_ptr[v], deg = row_ptr[v + 1] - b;
-    if (deg <= ADJ_MID_MAX_DEG) return;   // k_adjset_small's or k_adjset_mid's
-    adj_insert(t + adj_table_start(b, v), adj_table_size(deg), col[e]);
+    const uint64_t b = row_ptr[lo], deg = row_ptr[lo + 1] - b;
+    if (deg <= ADJ_MID_MAX_DEG) return;
+    adj_insert(t + adj_table_start(b, lo), adj_table_size(deg), col[e]);
 }
 
 // -------------------------------------------------------------- types ----
@@ -787,21 +812,57 @@ __global__ void __launch_bounds__(BLOCK) k_adjset_mid(const uint32_t *__restrict
     }
 }
 
-// Tables of the lists k_adjset_insert fills (list ids from k_list_range, *n
-// of them) set to empty, one workgroup per table in 16-byte stores; every
-// other table is written whole by its builder.
-__global__ void __launch_bounds__(256) k_adjset_clear(const uint32_t *__restrict__ list, const unsigned int *n,
-                                                      const uint64_t *__restrict__ row_ptr, uint32_t *__restrict__ t)
+// Tables of lists longer than ADJ_MID_MAX_DEG, one workgroup per (list,
+// segment) item: the workgroup reads the whole list, inserts the keys whose
+// home slot lies in its segment with LDS atomics (linear probing inside the
+// segment) and writes the segment out whole in 16-byte groups (segments start
+// at multiples of ADJ_SEG slots of 16-byte aligned tables, and table sizes
+// are multiples of four).  A key whose probe runs past the segment's end goes
+// to ovf (cap entries) for k_adjset_overflow.
+__global__ void __launch_bounds__(256) k_adjset_seg(const uint2 *__restrict__ items, uint32_t n_items,
+                                                    const uint32_t *__restrict__ list,
+                                                    const uint64_t *__restrict__ row_ptr,
+                                                    const uint32_t *__restrict__ col, uint32_t *__restrict__ t,
+                                                    uint2 *__restrict__ ovf, uint32_t cap, unsigned int *n_ovf)
 {
-    const uint32_t count = *n;
-    const uint4 empty = make_uint4(ADJ_EMPTY, ADJ_EMPTY, ADJ_EMPTY, ADJ_EMPTY);
-    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
-        const uint32_t v = list[i];
-        const uint64_t b = row_ptr[v];
-        const uint32_t size = adj_table_size(row_ptr[v + 1] - b);
-        uint4 *gt = reinterpret_cast<uint4 *>(t + adj_table_start(b, v));
-        for (uint32_t k = threadIdx.x; k < size / 4; k += blockDim.x) gt[k] = empty;
+    __shared__ uint4 tab4[ADJ_SEG / 4];
+    uint32_t *tab = reinterpret_cast<uint32_t *>(tab4);
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {   // workgroup-uniform
+        const uint2 item = items[it];   // {index into list, segment}
+        const uint32_t v = list[item.x];
+        const uint64_t b = row_ptr[v], e = row_ptr[v + 1];
+        const uint32_t size = adj_table_size(e - b);
+        const uint32_t s0 = item.y * ADJ_SEG;
+        const uint32_t w = size - s0 < ADJ_SEG ? size - s0 : ADJ_SEG;
+        for (uint32_t k = threadIdx.x; k < w; k += blockDim.x) tab[k] = ADJ_EMPTY;
+        __syncthreads();
+        for (uint64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
+            const uint32_t u = col[k];
+            const uint32_t h = adj_slot(u, size);
+            if (h < s0 || h - s0 >= w) continue;   // homed in another segment
+            for (uint32_t slot = h - s0;;) {
+                const uint32_t old = atomicCAS(tab + slot, ADJ_EMPTY, u);
+                if (old == ADJ_EMPTY || old == u) break;
+                if (++slot == w) {   // counted always, kept while there is room
+                    const unsigned int pos = atomicAdd(n_ovf, 1u);
+                    if (pos < cap) ovf[pos] = make_uint2(v, u);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        uint4 *gt = reinterpret_cast<uint4 *>(t + adj_table_start(b, v) + s0);
+        for (uint32_t k = threadIdx.x; k < w / 4; k += blockDim.x) gt[k] = tab4[k];
+        __syncthreads();   // the next item reuses the LDS segment
     }
+}
+
+// deg[i] = degree of vertex list[i]
+__global__ void k_list_degrees(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ row_ptr,
+                               uint32_t *__restrict__ deg)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) deg[i] = (uint32_t)(row_ptr[list[i] + 1] - row_ptr[list[i]]);
 }
 
 // ids of the vertices whose lists hold lo .. hi entries, in no particular order
@@ -1241,7 +1302,8 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
     const uint64_t n = adj_table_start(gs.E, gs.V) + 4;   // every table ends by slot 4 (E + V)
     DArr<uint32_t> t(n);
     DArr<uint32_t> mid, big;    // freed after the stream is synchronised below
-    DArr<unsigned int> n_mid, n_big;
+    DArr<unsigned int> n_mid, n_big, n_ovf;
+    DArr<uint2> items_d, ovf;
     if (gs.V) {   // lists up to ADJ_LDS_MAX_DEG entries, and the empty lists' tables
         const uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)gs.V + 3) / 4, 1u << 16);   // 4 waves each
         hipLaunchKernelGGL(k_adjset_small, dim3(blocks), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V, t.p);
@@ -1255,16 +1317,51 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
         // two 64 KiB workgroups fit a CU's LDS; the grid strides over the lists
         hipLaunchKernelGGL((k_adjset_mid<ADJ_MID_MAX_DEG, 256>), dim3(std::min<uint64_t>(gs.V, 4096)), dim3(256), 0,
                            s, mid.p, n_mid.p, gs.row_ptr, gs.col, t.p);
-        // longer lists: their tables cleared, then one lane per edge inserts
+        // longer lists: LDS segments, then the keys that ran past a segment
         big.p = dalloc<uint32_t>(gs.V);
         n_big.p = dalloc<unsigned int>(1);
         KKW_HIP(hipMemsetAsync(n_big.p, 0, sizeof(unsigned int), s));
         hipLaunchKernelGGL(k_list_range, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V,
                            (uint64_t)ADJ_MID_MAX_DEG + 1, ~0ull, big.p, n_big.p);
-        hipLaunchKernelGGL(k_adjset_clear, dim3(std::min<uint64_t>(gs.V, 4096)), dim3(256), 0, s, big.p, n_big.p,
-                           gs.row_ptr, t.p);
-        hipLaunchKernelGGL(k_adjset_insert, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V,
-                           gs.E, t.p);
+        unsigned int nb = 0;
+        KKW_HIP(hipMemcpyAsync(&nb, n_big.p, sizeof(nb), hipMemcpyDeviceToHost, s));
+        KKW_HIP(hipStreamSynchronize(s));
+        if (nb) {
+            DArr<uint32_t> deg_d(nb);
+            hipLaunchKernelGGL(k_list_degrees, dim3(grid_for(nb, 256)), dim3(256), 0, s, big.p, nb, gs.row_ptr,
+                               deg_d.p);
+            std::vector<uint32_t> deg(nb);
+            KKW_HIP(hipMemcpyAsync(deg.data(), deg_d.p, (size_t)nb * 4, hipMemcpyDeviceToHost, s));
+            KKW_HIP(hipStreamSynchronize(s));
+            std::vector<uint2> items;
+            uint64_t keys = 0;
+            for (uint32_t i = 0; i < nb; i++) {
+                const uint32_t segs = (uint32_t)((adj_table_size(deg[i]) + ADJ_SEG - 1) / ADJ_SEG);
+                for (uint32_t g = 0; g < segs; g++) items.push_back(make_uint2(i, g));
+                keys += deg[i];
+            }
+            // a few keys per table run past a segment; KKW_ADJ_OVERFLOW_CAP (tests) shrinks the buffer
+            uint64_t cap = std::min<uint64_t>(keys, ADJ_OVERFLOW_CAP);
+            if (const char *env = getenv("KKW_ADJ_OVERFLOW_CAP"))
+                cap = std::min<uint64_t>(cap, strtoull(env, nullptr, 10));
+            items_d.p = dalloc<uint2>(items.size());
+            ovf.p = dalloc<uint2>(cap);
+            n_ovf.p = dalloc<unsigned int>(1);
+            KKW_HIP(hipMemcpyAsync(items_d.p, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice, s));
+            KKW_HIP(hipMemsetAsync(n_ovf.p, 0, sizeof(unsigned int), s));
+            // two 64 KiB workgroups fit a CU's LDS; the grid strides over the items
+            hipLaunchKernelGGL(k_adjset_seg, dim3((unsigned)std::min<size_t>(items.size(), 8192)), dim3(256), 0, s,
+                               items_d.p, (uint32_t)items.size(), big.p, gs.row_ptr, gs.col, t.p, ovf.p, (uint32_t)cap,
+                               n_ovf.p);
+            hipLaunchKernelGGL(k_adjset_overflow, dim3(1024), dim3(256), 0, s, ovf.p, n_ovf.p, (uint32_t)cap,
+                               gs.row_ptr, t.p);
+            unsigned int no = 0;
+            KKW_HIP(hipMemcpyAsync(&no, n_ovf.p, sizeof(no), hipMemcpyDeviceToHost, s));
+            KKW_HIP(hipStreamSynchronize(s));   // also before the host vectors and deg_d go
+            if (no > cap)   // the buffer could not hold them all: insert every key of the long lists
+                hipLaunchKernelGGL(k_adjset_insert_long, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.row_ptr,
+                                   gs.col, gs.V, gs.E, t.p);
+        }
     }
     KKW_HIP(hipGetLastError());
     KKW_HIP(hipStreamSynchronize(s));

@@ -540,6 +540,82 @@ def test_node2vec_with_and_without_hash_sets(engine, index):
     engine.adj_index(True)
 
 
+ADJ_SEG = 16384   # slots of one LDS segment of a long list's hash set
+
+
+def adj_home(keys, size):
+    """Home slot of each key in a hash set of `size` slots (the layout the
+    library documents in include/kkw_program.h)."""
+    k = np.asarray(keys, np.uint64)
+    return (((k * np.uint64(0x9E3779B1)) & np.uint64(0xFFFFFFFF)) * np.uint64(size)) >> np.uint64(32)
+
+
+def segment_overflows(rp, col, min_deg=4096):
+    """Keys of lists of min_deg or more entries whose linear probe, run inside
+    their own ADJ_SEG-slot segment, passes the segment's end."""
+    rp = rp.astype(np.int64)
+    total = 0
+    for v in np.nonzero(np.diff(rp) >= min_deg)[0]:
+        keys = col[rp[v]:rp[v + 1]]
+        size = 4 * len(keys) + 4
+        h = adj_home(keys, size).astype(np.int64)
+        for s0 in range(0, size, ADJ_SEG):
+            w = min(ADJ_SEG, size - s0)
+            pos = -1
+            for x in np.sort(h[(h >= s0) & (h < s0 + w)] - s0):
+                pos = max(pos + 1, int(x))
+                total += pos >= w
+    return total
+
+
+def hub_graph(V=200000, extra=300000, seed=5):
+    """Undirected weighted graph whose hubs have lists far longer than one LDS
+    table: vertex 0 joined to every vertex, vertex 1 to 20,000 more (all
+    candidates hashing into the last 8 slots of the first segment of its table
+    among them, so keys run past a segment's end), plus random edges; no
+    duplicates, no self loops, lists sorted."""
+    rng = np.random.default_rng(seed)
+    cand = np.arange(2, V)
+    home = adj_home(cand, 4 * 20001 + 4)   # vertex 1's list: 20,000 picks and vertex 0
+    hot = cand[(home >= ADJ_SEG - 8) & (home < ADJ_SEG)]   # about 20 keys for 8 slots
+    rest = rng.permutation(np.setdiff1d(cand, hot))[:20000 - len(hot)]
+    picks = np.concatenate([hot, rest])
+    u = np.concatenate([np.zeros(V - 1, np.int64), np.ones(len(picks), np.int64), rng.integers(2, V, extra)])
+    v = np.concatenate([np.arange(1, V), picks, rng.integers(2, V, extra)])
+    keep = u != v
+    key = np.unique(np.minimum(u, v)[keep] * V + np.maximum(u, v)[keep])
+    src, dst = (key // V).astype(np.uint32), (key % V).astype(np.uint32)
+    w = rng.uniform(0.1, 2.0, len(src)).astype(np.float32)
+    return O.csr_from_edges(V, src, dst, w, undirected=True, sort_adj=True)
+
+
+@pytest.mark.parametrize("index", ["segments", "fallback", "off"])
+def test_node2vec_hash_sets_of_hub_lists(engine, index, monkeypatch):
+    """Adjacency tests against hub lists of 199,999 and 20,001 entries (their
+    hash sets span many 64 KiB LDS segments) answer as the binary search does:
+    paths bit-exact against the oracle, both launch forms.  "fallback" leaves
+    no room for keys that run past a segment, so every key of the long lists
+    is inserted again with global atomics."""
+    rp, col, w = hub_graph()
+    deg = np.diff(rp.astype(np.int64))
+    assert deg[0] == len(deg) - 1 and deg[1] == 20001
+    assert segment_overflows(rp, col) >= 8   # about 20 keys homed in 8 slots before a segment's end
+    engine.graph_from_csr(rp, col, w)
+    if index == "fallback":
+        monkeypatch.setenv("KKW_ADJ_OVERFLOW_CAP", "0")   # read when the hash sets are built
+    engine.adj_index(index != "off")
+    monkeypatch.delenv("KKW_ADJ_OVERFLOW_CAP", raising=False)
+    g = O.Graph(rp, col, w)
+    n = 30000
+    for weighted in (True, False):
+        op, ol, ot, st = O.walk(g, O.NODE2VEC, n, 20, weighted, p=2.0, q=0.5, seed=71)
+        for mode in (kk.MODE_PLAIN, kk.MODE_PERSISTENT):
+            print(f"[gpu] hub lists index={index} weighted={weighted} mode={mode}", flush=True)
+            (gpu,) = engine.run([kk.node2vec_job(n, 20, 2.0, 0.5, weighted, seed=71)], mode=mode)
+            assert_paths_equal(gpu, op, ol, ot, f"hubs index={index} weighted={weighted} mode={mode}")
+    engine.adj_index(True)
+
+
 TORCH_HANDOFF = r"""
 import sys
 import numpy as np
