@@ -416,17 +416,25 @@ __global__ void __launch_bounds__(256) k_vmask(const uint64_t *row_ptr, const ui
 // re-queued by its new value (617-634); leftovers get p = 1 and no alias
 // (635-647).  Buckets are stored in creation order.  An element sits in at most
 // one queue and its mass changes only when it is popped as the large partner,
-// so each queue entry carries its current mass with its index: every access
-// to the queues and the weights is then a sequential stream.
-__global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_ptr,
-                                              const uint32_t *__restrict__ col,
-                                              const float *__restrict__ w,
-                                              const uint8_t *__restrict__ etype, uint32_t V,
-                                              Bucket *__restrict__ out, float *__restrict__ vsum,
-                                              uint2 *__restrict__ qs, uint2 *__restrict__ ql, uint32_t big_threshold,
-                                              bool to_ids, const uint64_t *__restrict__ nrow)
+// so each queue entry carries its current mass with its index.
+//
+// This kernel takes the lists shorter than AW_MIN (k_alias_wave the rest).
+// Such a list never has more than AS_RING - 1 live queue entries, so each
+// lane keeps its two queues as rings in LDS (mass and local index apart,
+// rows padded against bank conflicts) and the pairing makes no global memory
+// round trip.  Buckets leave with local indices and the list's first edge in
+// prow, for k_finish_buckets.
+constexpr uint32_t AS_RING = 32;
+constexpr uint32_t AW_MIN = 32;   // lists of AW_MIN or more entries: k_alias_wave
+static_assert(AW_MIN <= AS_RING, "a short list's live queue entries must fit its ring");
+
+__global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_ptr, const float *__restrict__ w,
+                                              uint32_t V, Bucket *__restrict__ out, float *__restrict__ vsum)
 {
-    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float sp[64][AS_RING + 1], lp[64][AS_RING + 1];   // masses of the small / large rings
+    __shared__ uint8_t si[64][AS_RING + 4], li[64][AS_RING + 4];  // their local edge indices
+    const uint32_t lane = threadIdx.x;
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     const uint64_t b = row_ptr[v];
     const uint32_t d = (uint32_t)(row_ptr[v + 1] - b);
@@ -434,72 +442,66 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
         vsum[v] = 0.0f;
         return;
     }
-    if (d >= big_threshold) return;   // k_alias_wave's
+    if (d >= AW_MIN) return;   // k_alias_wave's
     const float *wv = w + b;
     float sum = 0.0f;
     for (uint32_t e = 0; e < d; e++) sum += wv[e];
     vsum[v] = sum;
     const float ave = sum / (float)d;
-    uint2 *S = qs + b;
-    uint2 *L = ql + b;
+    constexpr uint32_t M = AS_RING - 1;
     uint32_t st = 0, lt = 0;
     for (uint32_t e = 0; e < d; e++) {
-        float p = wv[e];
-        uint2 ent = make_uint2(e, __float_as_uint(p));
-        if (p < ave) S[st++] = ent;
-        else L[lt++] = ent;
+        const float p = wv[e];
+        if (p < ave) {
+            sp[lane][st & M] = p;
+            si[lane][st & M] = (uint8_t)e;
+            st++;
+        } else {
+            lp[lane][lt & M] = p;
+            li[lane][lt & M] = (uint8_t)e;
+            lt++;
+        }
     }
-    const uint32_t *cv = col + b;
-    const uint8_t *tv = etype ? etype + b : nullptr;
-    const uint64_t *rv = nrow ? nrow + b : nullptr;   // packed rows of the list's neighbours
     Bucket *o = out + b;
+    Bucket bk;
+    bk.meta = 0;
+    bk.prow = b;
+    bk.arow = 0;
     uint32_t sh = 0, lh = 0;
     while (sh != st) {
-        uint2 se = S[sh++];
-        float ps = __uint_as_float(se.y);
-        Bucket bk;
-        bk.prow = 0;
-        bk.arow = 0;
+        const float ps = sp[lane][sh & M];
+        const uint32_t se = si[lane][sh & M];
+        sh++;
+        bk.primary = se;
         if (lh != lt) {
-            uint2 le = L[lh++];
-            float pl = __uint_as_float(le.y);
+            float pl = lp[lane][lh & M];
+            const uint32_t le = li[lane][lh & M];
+            lh++;
             bk.p = ps / ave;
-            bk.primary = to_ids ? cv[se.x] : se.x;
-            bk.alias = to_ids ? cv[le.x] : le.x;
-            bk.meta = tv ? ((uint32_t)tv[se.x] | ((uint32_t)tv[le.x] << 8)) : 0u;
-            if (!to_ids) {   // local tables: the candidates' ids in the row fields
-                bk.prow = cv[se.x];
-                bk.arow = cv[le.x];
-            } else if (rv) {
-                bk.prow = rv[se.x];
-                bk.arow = rv[le.x];
-            }
-            float diff = ave - ps;
+            bk.alias = le;
+            const float diff = ave - ps;
             pl -= diff;
-            uint2 ne = make_uint2(le.x, __float_as_uint(pl));
-            if (pl < ave) S[st++] = ne;
-            else L[lt++] = ne;
+            if (pl < ave) {
+                sp[lane][st & M] = pl;
+                si[lane][st & M] = (uint8_t)le;
+                st++;
+            } else {
+                lp[lane][lt & M] = pl;
+                li[lane][lt & M] = (uint8_t)le;
+                lt++;
+            }
         } else {
             bk.p = 1.0f;
-            bk.primary = to_ids ? cv[se.x] : se.x;
-            bk.alias = bk.primary;
-            bk.meta = tv ? (uint32_t)tv[se.x] * 0x101u : 0u;
-            if (!to_ids) bk.prow = bk.arow = cv[se.x];
-            else if (rv) bk.prow = bk.arow = rv[se.x];
+            bk.alias = se;
         }
         *o++ = bk;
     }
     while (lh != lt) {
-        uint2 le = L[lh++];
-        Bucket bk;
-        bk.prow = 0;
-        bk.arow = 0;
+        const uint32_t le = li[lane][lh & M];
+        lh++;
         bk.p = 1.0f;
-        bk.primary = to_ids ? cv[le.x] : le.x;
-        bk.alias = bk.primary;
-        bk.meta = tv ? (uint32_t)tv[le.x] * 0x101u : 0u;
-        if (!to_ids) bk.prow = bk.arow = cv[le.x];
-        else if (rv) bk.prow = bk.arow = rv[le.x];
+        bk.primary = le;
+        bk.alias = le;
         *o++ = bk;
     }
 }
@@ -518,17 +520,14 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
 // in LDS; lists longer than that also keep their queues in global memory and
 // refill the windows from there.  The pairing works on local edge indices
 // and reads nothing from global memory, so a batch costs only its LDS round
-// trip: each bucket is written with its local indices, the list's first edge
-// in prow and AW_MARK in meta, and k_finish_wave_buckets then gives the
-// marked buckets their neighbour ids, rows and edge types in one edge-parallel
-// pass (finish_bucket; the tables of walk programs keep the indices and add
-// the ids).
-constexpr uint32_t AW_MIN = 32;
+// trip: each bucket is written with its local indices and the list's first
+// edge in prow, and k_finish_buckets then gives every bucket its neighbour
+// ids, rows and edge types in one edge-parallel pass (finish_bucket; the
+// tables of walk programs keep the indices and add the ids).
 constexpr uint32_t AW_WIN = 512;
 constexpr int AW_WAVES = 4;
 constexpr uint32_t AW_SUM_CHUNK = 256;
 constexpr uint32_t AW_Q_CHUNKS = 4;          // chunks of 64 weights per load group when queueing
-constexpr uint32_t AW_MARK = 0xFFFFFFFFu;   // meta of a bucket still holding local indices (real meta < 2^16)
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -671,7 +670,7 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
         k = k < 64 ? k : 64;
         const bool act = lane < k;
         Bucket bk;
-        bk.meta = AW_MARK;
+        bk.meta = 0;
         bk.prow = b;
         bk.arow = 0;
         bool to_s = false, to_l = false;
@@ -722,7 +721,7 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
         bk.p = 1.0f;
         bk.primary = e;
         bk.alias = e;
-        bk.meta = AW_MARK;
+        bk.meta = 0;
         bk.prow = b;
         bk.arow = 0;
         o[ob + i] = bk;
@@ -731,19 +730,17 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
 
 typedef uint32_t rows_u32x4 __attribute__((ext_vector_type(4), aligned(4)));
 
-// Buckets of the wave-built lists (meta == AW_MARK, prow = the list's first
-// edge): local indices become ids, rows and edge types (finish_bucket).  One
-// lane per bucket; buckets of the shorter lists are final and only read.
-__global__ void k_finish_wave_buckets(Bucket *__restrict__ out, uint64_t E, const uint32_t *__restrict__ col,
-                                      const uint8_t *__restrict__ etype, bool to_ids,
-                                      const uint64_t *__restrict__ nrow)
+// Every bucket as the alias kernels left it (local indices, prow = the list's
+// first edge) gets its neighbour ids, rows and edge types (finish_bucket), one
+// lane per bucket.
+__global__ void k_finish_buckets(Bucket *__restrict__ out, uint64_t E, const uint32_t *__restrict__ col,
+                                 const uint8_t *__restrict__ etype, bool to_ids, const uint64_t *__restrict__ nrow)
 {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= E) return;
     uint32_t *bw = reinterpret_cast<uint32_t *>(out + k);
-    const rows_u32x4 h1 = *reinterpret_cast<const rows_u32x4 *>(bw + 4);   // {alias, meta, arow}
-    if (h1.y != AW_MARK) return;
     const rows_u32x4 h0 = *reinterpret_cast<const rows_u32x4 *>(bw);       // {p, primary, prow = list start}
+    const rows_u32x4 h1 = *reinterpret_cast<const rows_u32x4 *>(bw + 4);   // {alias, -, -}
     Bucket bk;
     bk.p = __uint_as_float(h0.x);
     bk.primary = h0.y;
@@ -865,21 +862,36 @@ __global__ void k_list_degrees(const uint32_t *__restrict__ list, uint32_t n, co
     if (i < n) deg[i] = (uint32_t)(row_ptr[list[i] + 1] - row_ptr[list[i]]);
 }
 
-// ids of the vertices whose lists hold lo .. hi entries, in no particular order
-__global__ void k_list_range(const uint64_t *row_ptr, uint32_t V, uint64_t lo, uint64_t hi, uint32_t *out,
-                             unsigned int *n)
-{
-    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    const uint64_t d = row_ptr[v + 1] - row_ptr[v];
-    if (d >= lo && d <= hi) out[atomicAdd(n, 1u)] = (uint32_t)v;
-}
+// ids of the vertices whose lists hold lo .. hi entries, in no particular order.
+// One atomic per workgroup of LIST_BLOCK lanes (same-address atomics serialise
+// in L2): picks are counted by wave ballots, the waves' counts offset in LDS.
+constexpr uint32_t LIST_BLOCK = 1024;
 
-__global__ void k_list_big(const uint64_t *row_ptr, uint32_t V, uint32_t *big, unsigned int *n_big)
+__global__ void __launch_bounds__(LIST_BLOCK) k_list_range(const uint64_t *row_ptr, uint32_t V, uint64_t lo,
+                                                           uint64_t hi, uint32_t *out, unsigned int *n)
 {
-    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    if (row_ptr[v + 1] - row_ptr[v] >= AW_MIN) big[atomicAdd(n_big, 1u)] = (uint32_t)v;
+    __shared__ unsigned int wave_count[LIST_BLOCK / 64], block_base;
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool take = false;
+    if (v < V) {
+        const uint64_t d = row_ptr[v + 1] - row_ptr[v];
+        take = d >= lo && d <= hi;
+    }
+    const uint64_t m = __ballot(take);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wave_count[wave] = (unsigned int)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int total = 0;
+        for (uint32_t k = 0; k < blockDim.x / 64; k++) {
+            const unsigned int c = wave_count[k];
+            wave_count[k] = total;   // now the wave's offset in the block
+            total += c;
+        }
+        block_base = total ? atomicAdd(n, total) : 0u;
+    }
+    __syncthreads();
+    if (take) out[block_base + wave_count[wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)v;
 }
 
 // Packed row of every edge's neighbour, nrow[k] = row of col[k], for the
@@ -1312,7 +1324,7 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
         mid.p = dalloc<uint32_t>(gs.V);
         n_mid.p = dalloc<unsigned int>(1);
         KKW_HIP(hipMemsetAsync(n_mid.p, 0, sizeof(unsigned int), s));
-        hipLaunchKernelGGL(k_list_range, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V,
+        hipLaunchKernelGGL(k_list_range, dim3(grid_for(gs.V, LIST_BLOCK)), dim3(LIST_BLOCK), 0, s, gs.row_ptr, gs.V,
                            (uint64_t)ADJ_LDS_MAX_DEG + 1, (uint64_t)ADJ_MID_MAX_DEG, mid.p, n_mid.p);
         // two 64 KiB workgroups fit a CU's LDS; the grid strides over the lists
         hipLaunchKernelGGL((k_adjset_mid<ADJ_MID_MAX_DEG, 256>), dim3(std::min<uint64_t>(gs.V, 4096)), dim3(256), 0,
@@ -1321,7 +1333,7 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
         big.p = dalloc<uint32_t>(gs.V);
         n_big.p = dalloc<unsigned int>(1);
         KKW_HIP(hipMemsetAsync(n_big.p, 0, sizeof(unsigned int), s));
-        hipLaunchKernelGGL(k_list_range, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V,
+        hipLaunchKernelGGL(k_list_range, dim3(grid_for(gs.V, LIST_BLOCK)), dim3(LIST_BLOCK), 0, s, gs.row_ptr, gs.V,
                            (uint64_t)ADJ_MID_MAX_DEG + 1, ~0ull, big.p, n_big.p);
         unsigned int nb = 0;
         KKW_HIP(hipMemcpyAsync(&nb, n_big.p, sizeof(nb), hipMemcpyDeviceToHost, s));
@@ -1405,18 +1417,17 @@ void build_alias(GraphStore &gs, hipStream_t s, bool local)
         DArr<uint32_t> big(gs.V);
         DArr<unsigned int> n_big_d(1);
         KKW_HIP(hipMemsetAsync(n_big_d.p, 0, sizeof(unsigned int), s));
-        hipLaunchKernelGGL(k_list_big, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.V, big.p,
-                           n_big_d.p);
+        hipLaunchKernelGGL(k_list_range, dim3(grid_for(gs.V, LIST_BLOCK)), dim3(LIST_BLOCK), 0, s, gs.row_ptr, gs.V,
+                           (uint64_t)AW_MIN, ~0ull, big.p, n_big_d.p);
         unsigned int n_big = 0;
         KKW_HIP(hipMemcpyAsync(&n_big, n_big_d.p, sizeof(n_big), hipMemcpyDeviceToHost, s));
         KKW_HIP(hipStreamSynchronize(s));
         if (n_big)
             hipLaunchKernelGGL(k_alias_wave, dim3(grid_for(n_big, AW_WAVES)), dim3(64 * AW_WAVES), 0, s, big.p,
                                n_big, gs.row_ptr, gs.w, t.p, vsum, qs.p, ql.p);
-        hipLaunchKernelGGL(k_alias, dim3(grid_for(gs.V, 64)), dim3(64), 0, s, gs.row_ptr, gs.col, gs.w,
-                           gs.etype, gs.V, t.p, vsum, qs.p, ql.p, AW_MIN, !local, nrow.p);
-        if (n_big)
-            hipLaunchKernelGGL(k_finish_wave_buckets, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, t.p, gs.E, gs.col,
+        hipLaunchKernelGGL(k_alias, dim3(grid_for(gs.V, 64)), dim3(64), 0, s, gs.row_ptr, gs.w, gs.V, t.p, vsum);
+        if (gs.E)
+            hipLaunchKernelGGL(k_finish_buckets, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, t.p, gs.E, gs.col,
                                gs.etype, !local, nrow.p);
         KKW_HIP(hipGetLastError());
         KKW_HIP(hipStreamSynchronize(s));
