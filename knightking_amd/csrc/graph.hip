@@ -527,7 +527,7 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
 constexpr uint32_t AW_WIN = 512;
 constexpr int AW_WAVES = 4;
 constexpr uint32_t AW_SUM_CHUNK = 256;
-constexpr uint32_t AW_Q_CHUNKS = 4;          // chunks of 64 weights per load group when queueing
+constexpr uint32_t AW_Q_CHUNKS = 16;         // chunks of 64 weights per load group when queueing
 
 __device__ __forceinline__ void wave_sync()
 {
