@@ -268,16 +268,19 @@ __device__ __forceinline__ uint32_t pick(const DevGraph &g, const DevJob &J, con
     const Bucket *tab = use_alias ? g.bucket : g.ubucket;
     if (tab) {
         // first half {p, primary, prow}; the second {alias, meta, arow} only
-        // when the coin picks the alias or the edge type is wanted
+        // when the coin picks the alias, or -- issued with the first, one
+        // memory round trip -- when the edge type is wanted (metapath)
         const uint32_t *bp = reinterpret_cast<const uint32_t *>(tab + s.beg + k);
         const u32x4 a = load16(bp);
+        u32x4 h = {0u, 0u, 0u, 0u};
+        if (etype) h = load16(bp + 4);
         const bool prim = u01(r.z) < __uint_as_float(a.x);
         row_ok = g.packed_rows != 0;
         if (prim && !etype) {
             row = (uint64_t)a.z | ((uint64_t)a.w << 32);
             return a.y;
         }
-        const u32x4 h = load16(bp + 4);
+        if (!etype) h = load16(bp + 4);
         if (etype) *etype = prim ? (h.y & 0xffu) : ((h.y >> 8) & 0xffu);
         row = prim ? ((uint64_t)a.z | ((uint64_t)a.w << 32)) : ((uint64_t)h.z | ((uint64_t)h.w << 32));
         return prim ? a.y : h.x;
