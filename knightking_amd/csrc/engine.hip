@@ -1241,7 +1241,7 @@ int kkw_alias_download(kkw_engine *eng, float *p, uint32_t *primary, uint32_t *a
             if (p) p[i] = h[i].p;
             if (primary) primary[i] = h[i].primary;
             if (alias) alias[i] = h[i].alias;
-            if (meta) meta[i] = h[i].meta;
+            if (meta) meta[i] = h[i].meta & 0xFFFFu;   // the edge types (bits 16-31: the candidates' type masks)
         }
         if (vsum) KKW_HIP(hipMemcpy(vsum, gs.vsum, (size_t)gs.V * 4, hipMemcpyDeviceToHost));
         return KKW_OK;

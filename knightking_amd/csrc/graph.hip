@@ -536,20 +536,22 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// A bucket built with local edge indices gets its edge types (meta) and, for
-// the engine's tables, its candidates' neighbour ids in place of the indices
+// A bucket built with local edge indices gets its edge types (meta bits 0-15)
+// and, for the engine's tables, its candidates' neighbour ids in place of the
+// indices, their type masks (meta bits 16-31, when the graph has edge types)
 // and their packed rows (nrow, when the graph packs rows); tables of local
 // indices (walk programs) keep them and take the ids in the row fields.  Done
 // as each bucket is built, so every bucket is written once.
 __device__ __forceinline__ void finish_bucket(Bucket &bk, uint64_t b, const uint32_t *__restrict__ col,
                                               const uint8_t *__restrict__ etype, bool to_ids,
-                                              const uint64_t *__restrict__ nrow)
+                                              const uint64_t *__restrict__ nrow, const uint8_t *__restrict__ vmask)
 {
     const uint32_t pe = bk.primary, ae = bk.alias;
     bk.meta = etype ? ((uint32_t)etype[b + pe] | ((uint32_t)etype[b + ae] << 8)) : 0u;
     if (to_ids) {
         bk.primary = col[b + pe];
         bk.alias = col[b + ae];
+        if (etype && vmask) bk.meta |= ((uint32_t)vmask[bk.primary] << 16) | ((uint32_t)vmask[bk.alias] << 24);
         if (nrow) {
             bk.prow = nrow[b + pe];
             bk.arow = nrow[b + ae];
@@ -734,7 +736,8 @@ typedef uint32_t rows_u32x4 __attribute__((ext_vector_type(4), aligned(4)));
 // first edge) gets its neighbour ids, rows and edge types (finish_bucket), one
 // lane per bucket.
 __global__ void k_finish_buckets(Bucket *__restrict__ out, uint64_t E, const uint32_t *__restrict__ col,
-                                 const uint8_t *__restrict__ etype, bool to_ids, const uint64_t *__restrict__ nrow)
+                                 const uint8_t *__restrict__ etype, bool to_ids, const uint64_t *__restrict__ nrow,
+                                 const uint8_t *__restrict__ vmask)
 {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= E) return;
@@ -747,7 +750,7 @@ __global__ void k_finish_buckets(Bucket *__restrict__ out, uint64_t E, const uin
     bk.alias = h1.x;
     bk.prow = 0;
     bk.arow = 0;
-    finish_bucket(bk, (uint64_t)h0.z | ((uint64_t)h0.w << 32), col, etype, to_ids, nrow);
+    finish_bucket(bk, (uint64_t)h0.z | ((uint64_t)h0.w << 32), col, etype, to_ids, nrow, vmask);
     out[k] = bk;
 }
 
@@ -911,8 +914,8 @@ __global__ void k_neighbour_rows(const uint32_t *__restrict__ col, uint64_t E, c
 }
 
 // uniform candidates as buckets: p = 1, alias = primary = col[e]
-__global__ void k_uniform_buckets(const uint32_t *col, const uint8_t *etype, const uint64_t *row_ptr, uint64_t E,
-                                  int rows, Bucket *out)
+__global__ void k_uniform_buckets(const uint32_t *col, const uint8_t *etype, const uint8_t *vmask,
+                                  const uint64_t *row_ptr, uint64_t E, int rows, Bucket *out)
 {
     uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
@@ -922,6 +925,7 @@ __global__ void k_uniform_buckets(const uint32_t *col, const uint8_t *etype, con
     b.primary = u;
     b.alias = u;
     b.meta = etype ? (uint32_t)etype[e] * 0x101u : 0u;
+    if (etype && vmask) b.meta |= (uint32_t)vmask[u] * 0x01010000u;   // the candidate's type mask, twice
     b.prow = rows ? row_of(row_ptr, u) : 0;
     b.arow = b.prow;
     out[e] = b;
@@ -1386,8 +1390,8 @@ void build_uniform_buckets(GraphStore &gs, hipStream_t s)
     if (gs.ubucket) return;
     DArr<Bucket> t(gs.E);
     if (gs.E)
-        hipLaunchKernelGGL(k_uniform_buckets, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.col, gs.etype, gs.row_ptr,
-                           gs.E, gs.packable ? 1 : 0, t.p);
+        hipLaunchKernelGGL(k_uniform_buckets, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.col, gs.etype, gs.vmask,
+                           gs.row_ptr, gs.E, gs.packable ? 1 : 0, t.p);
     KKW_HIP(hipGetLastError());
     KKW_HIP(hipStreamSynchronize(s));
     gs.ubucket = t.release();
@@ -1428,7 +1432,7 @@ void build_alias(GraphStore &gs, hipStream_t s, bool local)
         hipLaunchKernelGGL(k_alias, dim3(grid_for(gs.V, 64)), dim3(64), 0, s, gs.row_ptr, gs.w, gs.V, t.p, vsum);
         if (gs.E)
             hipLaunchKernelGGL(k_finish_buckets, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, t.p, gs.E, gs.col,
-                               gs.etype, !local, nrow.p);
+                               gs.etype, !local, nrow.p, gs.vmask);
         KKW_HIP(hipGetLastError());
         KKW_HIP(hipStreamSynchronize(s));
     }

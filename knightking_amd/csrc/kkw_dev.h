@@ -17,8 +17,10 @@ enum : int { START_EQUAL = 0, START_LIST = 1, START_UNIFORM = 2 };
 // reads only the first half; the vertex id and the row (list begin | degree <<
 // 40) of the candidate are both there, so a draw and the move it leads to
 // cost one 16-byte random access.  meta = edge type of primary | edge type of
-// alias << 8.  Rows are 0 when the graph cannot pack them (2^40 or more edges,
-// or a degree of 2^24 or more).
+// alias << 8 | (on graphs with edge types) type mask of the primary vertex <<
+// 16 | type mask of the alias vertex << 24, a vertex's type mask being the OR
+// of its out-edges' types (metapath's termination test).  Rows are 0 when the
+// graph cannot pack them (2^40 or more edges, or a degree of 2^24 or more).
 struct __align__(32) Bucket {
     float p;
     uint32_t primary;

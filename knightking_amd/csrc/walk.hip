@@ -95,6 +95,7 @@ struct Walker {
     uint64_t pbeg, pend; // list of prev
     uint32_t step, trial;
     uint32_t scheme, state;
+    uint32_t cvm;      // metapath: type mask of cur (OR of its out-edges' types)
     bool lines;        // row written through the lane's LDS line buffer
     PathGroup pg;
 };
@@ -193,9 +194,11 @@ __device__ __forceinline__ void init_walker(const DevGraph &g, const DevJob &J, 
     s.scheme = 0;
     s.state = 0;
     const int kind = KIND == KIND_ANY ? J.kind : KIND;
+    s.cvm = 0;
     if (kind == KIND_METAPATH) {
         u32x4 r = draw(J.seed, s.wid, 0, 0, TAG_INIT);
         s.scheme = bounded(r.x, r.y, J.n_schemes);
+        s.cvm = g.vmask[s.cur];   // later vertices' masks come with their buckets
     }
     s.beg = g.row_ptr[s.cur];
     s.end = g.row_ptr[s.cur + 1];
@@ -258,6 +261,7 @@ __device__ __forceinline__ u32x4 load16(const uint32_t *p)
 // Candidate by alias bucket or uniform index (include/walk.hpp:1016-1029).
 // Uniform draws read the uniform buckets (p = 1: always the primary, which is
 // col[beg + k]) when they exist.  row_ok/row: the candidate's packed row.
+// etype (metapath): the edge's type | the candidate's type mask << 8.
 template <bool ALIAS_KNOWN, bool ALIAS>
 __device__ __forceinline__ uint32_t pick(const DevGraph &g, const DevJob &J, const Walker &s,
                                          uint32_t deg, const u32x4 &r, uint32_t *etype, bool &row_ok,
@@ -281,14 +285,15 @@ __device__ __forceinline__ uint32_t pick(const DevGraph &g, const DevJob &J, con
             return a.y;
         }
         if (!etype) h = load16(bp + 4);
-        if (etype) *etype = prim ? (h.y & 0xffu) : ((h.y >> 8) & 0xffu);
+        if (etype) *etype = prim ? ((h.y & 0xffu) | ((h.y >> 8) & 0xff00u)) : (((h.y >> 8) & 0xffu) | ((h.y >> 16) & 0xff00u));
         row = prim ? ((uint64_t)a.z | ((uint64_t)a.w << 32)) : ((uint64_t)h.z | ((uint64_t)h.w << 32));
         return prim ? a.y : h.x;
     }
     row_ok = false;
     row = 0;
-    if (etype) *etype = g.etype ? g.etype[s.beg + k] : 0u;
-    return g.col[s.beg + k];
+    const uint32_t c = g.col[s.beg + k];
+    if (etype) *etype = g.etype ? ((uint32_t)g.etype[s.beg + k] | ((uint32_t)g.vmask[c] << 8)) : 0u;
+    return c;
 }
 
 // One trial.  Returns true when the walker has ended (length written).
@@ -381,7 +386,7 @@ __device__ __forceinline__ bool advance(const DevGraph &g, const DevJob &J, Walk
                 ext = J.ppr_ext;                                  // ppr.hpp:51-55
             } else if (kind == KIND_METAPATH) {
                 ext = (s.step >= J.walk_length ||
-                       !(g.vmask[s.cur] & J.scheme_mask[s.scheme * 8 + s.state])) ? 0.0f : 1.0f;
+                       !(s.cvm & J.scheme_mask[s.scheme * 8 + s.state])) ? 0.0f : 1.0f;
             } else {
                 ext = s.step >= J.walk_length ? 0.0f : 1.0f;      // deepwalk.hpp:35-37
             }
@@ -412,10 +417,11 @@ __device__ __forceinline__ bool advance(const DevGraph &g, const DevJob &J, Walk
         uint32_t et = 0;
         uint32_t cand = pick<ALIAS_KNOWN, ALIAS>(g, J, s, deg, r, &et, row_ok, crow);
         float dart = u01(r.w) * 1.0f;
-        float pd = ((J.scheme_mask[s.scheme * 8 + s.state] >> et) & 1u) ? 1.0f : 0.0f;
+        float pd = ((J.scheme_mask[s.scheme * 8 + s.state] >> (et & 0xffu)) & 1u) ? 1.0f : 0.0f;
         if (dart <= pd) {                                          // walk.hpp:1030-1036
             uint32_t slen = J.scheme_len[s.scheme];
             move_to(g, J, s, cand, ls, row_ok, crow);
+            s.cvm = et >> 8;                                       // the new vertex's type mask
             s.state = (s.state + 1) % slen;                        // metapath.hpp:96-99
         } else {
             s.trial++;
