@@ -184,7 +184,8 @@ int kkw_prepare(kkw_engine *eng);
  * test use the binary search over sorted lists.  Same answers either way. */
 int kkw_graph_adj_index(kkw_engine *eng, int enable);
 /* Copy the alias buckets back (creation order per vertex) and the per-vertex
- * sequential weight sums; any pointer may be NULL. */
+ * sequential weight sums; any pointer may be NULL.  meta[i] is the edge type
+ * of the primary | the edge type of the alias << 8 (0 without edge types). */
 int kkw_alias_download(kkw_engine *eng, float *p, uint32_t *primary, uint32_t *alias,
                        uint32_t *meta, float *vsum);
 
