@@ -382,7 +382,8 @@ def main():
     tp = eng.timing()
     alias_s = tp["alias_ms"] / 1e3 if weighted else 0.0
     index_s = tp["index_ms"] / 1e3
-    log(f"alias tables in {alias_s:.2f}s, neighbour hash sets in {index_s:.2f}s")
+    prepare_s = tp["prepare_ms"] / 1e3   # both tables, built concurrently
+    log(f"alias tables in {alias_s:.3f}s, neighbour hash sets in {index_s:.3f}s, together {prepare_s:.3f}s")
 
     job_args = dict(seed=1)
     job = kk.node2vec_job(V, 80, 2.0, 0.5, weighted, seed=1)
@@ -519,8 +520,9 @@ def main():
         "alias_build_s": round(alias_s, 3),
         "neighbour_set_build_s": round(index_s, 3),
         # the paper's end-to-end time: static tables built, then one walk of all walkers
-        "end_to_end_s": round(alias_s + index_s + kernel_s, 4),
-        "end_to_end_steps_per_s": round(steps_per_launch / (alias_s + index_s + kernel_s), 1),
+        "prepare_s": round(prepare_s, 4),
+        "end_to_end_s": round(prepare_s + kernel_s, 4),
+        "end_to_end_steps_per_s": round(steps_per_launch / (prepare_s + kernel_s), 1),
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),

@@ -118,6 +118,9 @@ typedef struct kkw_timing {
      * vertex) with the sum over them of max(1, ceil(log2(deg(prev)+1)) - 3) */
     uint64_t steps, trials, searches, search_probes, row_loads, search_lines;
     uint64_t ref_queries, ref_query_lines;
+    double prepare_ms;       /* last kkw_prepare, host wall time: the alias tables and
+                                the neighbour hash sets build concurrently, so this is
+                                less than alias_ms + index_ms                        */
 } kkw_timing;
 
 enum kkw_flags { KKW_FLAG_STATS = 1 };

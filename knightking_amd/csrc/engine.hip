@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <list>
 #include <map>
 #include <memory>
@@ -971,6 +972,7 @@ int kkw_engine_create(const kkw_engine_opts *opts, kkw_engine **out)
         KKW_HIP(hipSetDevice(eng->device));
         KKW_HIP(hipDeviceGetAttribute(&eng->cu_count, hipDeviceAttributeMultiprocessorCount, eng->device));
         KKW_HIP(hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking));
+        KKW_HIP(hipStreamCreateWithFlags(&eng->aux_stream, hipStreamNonBlocking));
         *out = eng.release();
         return KKW_OK;
     });
@@ -983,6 +985,7 @@ int kkw_engine_destroy(kkw_engine *eng)
     (void)hipSetDevice(eng->device);
     eng->gs.release();
     if (eng->stream) (void)hipStreamDestroy(eng->stream);
+    if (eng->aux_stream) (void)hipStreamDestroy(eng->aux_stream);
     delete eng;
     return KKW_OK;
 }
@@ -1196,12 +1199,51 @@ int kkw_prepare(kkw_engine *eng)
         KKW_HIP(hipSetDevice(eng->device));
         if (!eng->gs.row_ptr) return fail(KKW_E_NOGRAPH, "no graph loaded");
         eng->graph_version++;
-        if (eng->gs.w) {
-            double t0 = now_ms();
-            build_alias(eng->gs, eng->stream);
+        const double t0 = now_ms();
+        if (eng->gs.w && eng->adj_index_auto && !eng->gs.adjset) {
+            // the two tables are independent: the hash sets build on the second
+            // stream from a helper thread (they write gs.adjset only, the alias
+            // build gs.bucket and gs.vsum) while the alias tables build here
+            eng->graph_version++;
+            std::exception_ptr set_err;
+            double index_ms = 0.0;
+            std::thread sets([&]() {
+                try {
+                    KKW_HIP(hipSetDevice(eng->device));
+                    const double t1 = now_ms();
+                    build_adj_index(eng->gs, eng->aux_stream);
+                    index_ms = now_ms() - t1;
+                } catch (...) {
+                    set_err = std::current_exception();
+                }
+            });
+            std::exception_ptr alias_err;
+            try {
+                build_alias(eng->gs, eng->stream);
+            } catch (...) {
+                alias_err = std::current_exception();
+            }
             eng->timing.alias_ms = now_ms() - t0;
+            sets.join();
+            if (alias_err) std::rethrow_exception(alias_err);
+            if (set_err) {
+                try {
+                    std::rethrow_exception(set_err);
+                } catch (const Error &e) {   // as auto_adj_index: out of memory leaves the binary search
+                    if (e.code != KKW_E_NOMEM) throw;
+                    eng->adj_index_auto = false;
+                }
+            } else {
+                eng->timing.index_ms = index_ms;
+            }
+        } else {
+            if (eng->gs.w) {
+                build_alias(eng->gs, eng->stream);
+                eng->timing.alias_ms = now_ms() - t0;
+            }
+            auto_adj_index(eng);
         }
-        auto_adj_index(eng);
+        eng->timing.prepare_ms = now_ms() - t0;
         return KKW_OK;
     });
 }

@@ -75,6 +75,7 @@ struct kkw_engine {
     uint64_t hbm_budget = 0;
     int cu_count = 256;
     hipStream_t stream = nullptr;
+    hipStream_t aux_stream = nullptr;   // kkw_prepare: the hash sets build here beside the alias tables
     std::recursive_mutex mu;
     kkw::GraphStore gs;
     kkw_timing timing{};
