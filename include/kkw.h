@@ -108,7 +108,7 @@ typedef struct kkw_paths {
 typedef struct kkw_timing {
     double kernel_ms;        /* walk kernel(s), HIP events on the engine stream      */
     double total_ms;         /* host wall time of the last kkw_run / batch launch     */
-    double alias_ms;         /* last kkw_prepare                                     */
+    double alias_ms;         /* last alias-table build (kkw_prepare or first use)    */
     double graph_ms;         /* last graph build                                     */
     double index_ms;         /* last neighbour hash-set build                        */
     /* instrumented runs (KKW_FLAG_STATS / KKW_BATCH_STATS): accepted steps, trials,
@@ -179,7 +179,8 @@ int kkw_graph_download(kkw_engine *eng, uint64_t *row_ptr, uint32_t *col, float 
 /* Static component.  Replaces init_alias_tables (include/walk.hpp:538-652), kept
  * resident across runs instead of being rebuilt per random_walk call.  Also
  * builds the neighbour hash sets node2vec's adjacency test uses (unless turned
- * off below); both are otherwise built on first use. */
+ * off below), at the same time on a second stream of the engine; both are
+ * otherwise built on first use.  Blocks until both are built. */
 int kkw_prepare(kkw_engine *eng);
 /* Neighbour hash sets for node2vec's "is the candidate a neighbour of the
  * previous vertex" test (the respond_query of src/apps/node2vec.hpp:158-166,
