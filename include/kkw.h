@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define KKW_ABI_VERSION 1
+#define KKW_ABI_VERSION 2   /* 2: kkw_timing.prepare_ms */
 
 enum kkw_status {
     KKW_OK = 0,
