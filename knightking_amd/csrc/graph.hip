@@ -519,8 +519,9 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
 // sequential FIFO run k steps at a time.  Queue windows of AW_WIN entries live
 // in LDS; lists longer than that also keep their queues in global memory and
 // refill the windows from there.  The pairing works on local edge indices
-// and reads nothing from global memory, so a batch costs only its LDS round
-// trip: each bucket is written with its local indices and the list's first
+// and reads global memory only to refill its windows (all loads before one
+// wait), so a batch costs its LDS round trip: each bucket is written with its
+// local indices and the list's first
 // edge in prow, and k_finish_buckets then gives every bucket its neighbour
 // ids, rows and edge types in one edge-parallel pass (finish_bucket; the
 // tables of walk programs keep the indices and add the ids).
