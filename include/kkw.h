@@ -198,6 +198,26 @@ int kkw_alias_download(kkw_engine *eng, float *p, uint32_t *primary, uint32_t *a
 int kkw_run(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mode, int out_where,
             int flags, kkw_paths *out);
 int kkw_paths_free(kkw_paths *paths);
+
+/* No counterpart in the reference.  kkw_run for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): the same jobs, modes and paths, written into device
+ * buffers the caller owns, with nothing allocated on the device.  Job i writes
+ * walker walker_begin + r's row at paths[i] + r * strides[i] (16-byte aligned;
+ * strides[i] a multiple of 4, at least the path capacity walk_length + 1, for
+ * PPR with walk_length 0 the default cap + 1) and its length word (as
+ * kkw_paths.lengths) at lengths[i][r].  The work-queue counter, the job table,
+ * start lists and scheme tables live in one fixed device array of
+ * KKW_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; jobs that
+ * need more return KKW_E_NOMEM before anything runs.  Tables are never built
+ * here: weighted jobs need the alias tables resident (kkw_prepare), otherwise
+ * KKW_E_NOGRAPH; node2vec without hash sets uses the binary search (same
+ * answers).  Blocks until the walk has finished on the engine stream.  Calls
+ * on engines of one GPU in one process share the array and are serialised by
+ * the library; a call never depends on what an earlier one left there. */
+#define KKW_NOALLOC_SCRATCH_BYTES (4u << 20)
+int kkw_run_noalloc(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mode,
+                    uint32_t *const *paths, const uint32_t *strides, uint32_t *const *lengths);
 /* PathSet::dump text format (include/path.hpp:85-109): per walker, optional
  * "id len", then " v" per vertex, then newline; fopen mode "w" or "a". */
 int kkw_paths_dump(const kkw_paths *paths, const char *file, const char *mode,

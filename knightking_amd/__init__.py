@@ -90,7 +90,8 @@ EXPORTED = [
     "kkw_last_error", "kkw_abi_version", "kkw_engine_create", "kkw_engine_destroy",
     "kkw_graph_from_edges", "kkw_graph_from_arrays", "kkw_graph_sort_adj", "kkw_graph_load_file", "kkw_graph_from_csr", "kkw_graph_rmat",
     "kkw_graph_set_edge_types", "kkw_graph_type_by_dst", "kkw_graph_info", "kkw_graph_download",
-    "kkw_prepare", "kkw_graph_adj_index", "kkw_alias_download", "kkw_run", "kkw_paths_free", "kkw_paths_dump",
+    "kkw_prepare", "kkw_graph_adj_index", "kkw_alias_download", "kkw_run", "kkw_run_noalloc", "kkw_paths_free",
+    "kkw_paths_dump",
     "kkw_last_timing", "kkw_batch_create", "kkw_batch_set_shape", "kkw_batch_launch",
     "kkw_batch_sync", "kkw_batch_paths", "kkw_batch_destroy", "kkw_autotune", "kkw_diag_stream_copy",
     "kkw_diag_random_gather",
@@ -130,6 +131,7 @@ def load_library(path: str = LIB_PATH):
         "kkw_graph_adj_index": (I32, [P, I32]),
         "kkw_alias_download": (I32, [P, P, P, P, P, P]),
         "kkw_run": (I32, [P, C.POINTER(_Job), I32, I32, I32, I32, C.POINTER(_Paths)]),
+        "kkw_run_noalloc": (I32, [P, C.POINTER(_Job), I32, I32, P, P, P]),
         "kkw_paths_free": (I32, [C.POINTER(_Paths)]),
         "kkw_paths_dump": (I32, [C.POINTER(_Paths), C.c_char_p, C.c_char_p, I32]),
         "kkw_last_timing": (I32, [P, C.POINTER(_Timing)]),
@@ -574,6 +576,18 @@ class Engine:
         outs = (_Paths * len(jobs))()
         _check(_lib.kkw_run(self.h, cj, len(jobs), mode, OUT_HOST, FLAG_STATS if stats else 0, outs))
         return [_take_paths(outs[i]) for i in range(len(jobs))]
+
+    def run_noalloc(self, jobs: Sequence[Job], mode: int, paths: Sequence[int], strides: Sequence[int],
+                    lengths: Sequence[int]) -> None:
+        """kkw_run_noalloc: walk into caller-owned device buffers (device
+        addresses as integers: job i's rows at paths[i], strides[i] entries
+        apart, its length words at lengths[i]); allocates nothing on the device."""
+        n = len(jobs)
+        cj = (_Job * n)(*[j.to_c() for j in jobs])
+        pp = (C.c_void_p * n)(*paths)
+        ss = (C.c_uint32 * n)(*strides)
+        ll = (C.c_void_p * n)(*lengths)
+        _check(_lib.kkw_run_noalloc(self.h, cj, n, mode, pp, ss, ll))
 
     def batch(self, jobs: Sequence[Job], mode: int = MODE_PERSISTENT, flags: int = 0) -> Batch:
         return Batch(self, jobs, mode, flags)

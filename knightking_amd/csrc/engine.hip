@@ -67,7 +67,11 @@ static void node2vec_constants(float p, float q, DevJob &d)
 // line_rows: the job runs in a persistent batch, whose kernels write rows of
 // whole 128-byte lines from LDS when the row stride is a multiple of 32
 // entries; rows are padded to that when the padding stays within a quarter.
-static PreparedJob prepare_job(kkw_engine *eng, const kkw_job &j, bool line_rows = false)
+// may_build = false: build no table (no device allocation); a weighted job
+// then needs the alias tables resident, while uniform draws without uniform
+// buckets read col[] directly and node2vec without hash sets searches the
+// sorted lists -- the same candidates and answers either way.
+static PreparedJob prepare_job(kkw_engine *eng, const kkw_job &j, bool line_rows = false, bool may_build = true)
 {
     GraphStore &gs = eng->gs;
     if (!gs.row_ptr) throw Error(KKW_E_NOGRAPH, "no graph loaded");
@@ -107,14 +111,16 @@ static PreparedJob prepare_job(kkw_engine *eng, const kkw_job &j, bool line_rows
         node2vec_constants(j.p, j.q, d);
         // neighbour hash sets for the adjacency test, unless the caller turned
         // them off (kkw_graph_adj_index(eng, 0): binary search of sorted lists)
-        auto_adj_index(eng);
+        if (may_build) auto_adj_index(eng);
     }
-    if (!j.use_weights && !gs.ubucket) {   // uniform draws read uniform buckets
+    if (!j.use_weights && !gs.ubucket && may_build) {   // uniform draws read uniform buckets
         eng->graph_version++;
         build_uniform_buckets(gs, eng->stream);
     }
     if (j.use_weights) {
         if (!gs.w) throw Error(KKW_E_INVALID, "use_weights on a graph without weights");
+        if (!gs.bucket && !may_build)
+            throw Error(KKW_E_NOGRAPH, "the alias tables are not resident (call kkw_prepare first)");
         if (!gs.bucket) {
             eng->graph_version++;
             double t0 = now_ms();
@@ -255,28 +261,58 @@ static void batch_rechunk(kkw_batch *b)
     }
 }
 
-static void batch_enqueue(kkw_batch *b)
+// The walk launches of one set of jobs: one plain launch per job, or one
+// persistent launch over the work queue (d_queue, cleared here by a fill
+// kernel) with the job passed by value (one job) or read from d_jobs.
+struct WalkLaunch {
+    int mode = KKW_MODE_PLAIN;
+    int block = 256, blocks_per_cu = 8, chunk = 64;
+    uint64_t n_chunks = 0;
+    bool line_rows = false;
+    bool stats = false;
+    DevStats *d_stats = nullptr;
+    unsigned long long *d_queue = nullptr;
+    DevJob *d_jobs = nullptr;
+};
+
+static void enqueue_walk(const kkw_engine *eng, const std::vector<PreparedJob> &jobs, const WalkLaunch &w,
+                         hipStream_t s)
 {
-    DevGraph g = b->eng->gs.view();
-    hipStream_t s = b->stream;
-    if (b->stats) launch_fill_u64((unsigned long long *)b->d_stats, 0ull, 8, s);
-    if (b->mode == KKW_MODE_PLAIN) {
-        for (auto &pj : b->jobs) launch_walk_plain(g, pj.dj, b->stats, b->d_stats, b->block, s);
+    DevGraph g = eng->gs.view();
+    if (w.stats) launch_fill_u64((unsigned long long *)w.d_stats, 0ull, 8, s);
+    if (w.mode == KKW_MODE_PLAIN) {
+        for (auto &pj : jobs) launch_walk_plain(g, pj.dj, w.stats, w.d_stats, w.block, s);
     } else {
-        if (b->n_chunks == 0) return;
-        launch_fill_u64(b->d_queue, 0ull, 1, s);
-        uint64_t want = (uint64_t)b->eng->cu_count * b->blocks_per_cu;
-        uint64_t waves_needed = b->n_chunks;  // one chunk per wave keeps every wave busy
-        uint64_t blocks_needed = (waves_needed * 64 + b->block - 1) / b->block;
+        if (w.n_chunks == 0) return;
+        launch_fill_u64(w.d_queue, 0ull, 1, s);
+        uint64_t want = (uint64_t)eng->cu_count * w.blocks_per_cu;
+        uint64_t waves_needed = w.n_chunks;  // one chunk per wave keeps every wave busy
+        uint64_t blocks_needed = (waves_needed * 64 + w.block - 1) / w.block;
         int grid = (int)std::max<uint64_t>(1, std::min(want, blocks_needed));
-        if (b->jobs.size() == 1)
-            launch_walk_persistent_single(g, b->jobs[0].dj, b->n_chunks, (uint32_t)b->chunk, b->d_queue, b->stats,
-                                          b->d_stats, grid, b->block, s);
+        if (jobs.size() == 1)
+            launch_walk_persistent_single(g, jobs[0].dj, w.n_chunks, (uint32_t)w.chunk, w.d_queue, w.stats,
+                                          w.d_stats, grid, w.block, s);
         else
-            launch_walk_persistent_mixed(g, b->d_jobs, (int)b->jobs.size(), b->n_chunks, (uint32_t)b->chunk,
-                                         b->d_queue, b->stats, b->d_stats, grid, b->block, b->line_rows, s);
+            launch_walk_persistent_mixed(g, w.d_jobs, (int)jobs.size(), w.n_chunks, (uint32_t)w.chunk, w.d_queue,
+                                         w.stats, w.d_stats, grid, w.block, w.line_rows, s);
     }
     KKW_HIP(hipGetLastError());
+}
+
+static void batch_enqueue(kkw_batch *b)
+{
+    WalkLaunch w;
+    w.mode = b->mode;
+    w.block = b->block;
+    w.blocks_per_cu = b->blocks_per_cu;
+    w.chunk = b->chunk;
+    w.n_chunks = b->n_chunks;
+    w.line_rows = b->line_rows;
+    w.stats = b->stats;
+    w.d_stats = b->d_stats;
+    w.d_queue = b->d_queue;
+    w.d_jobs = b->d_jobs;
+    enqueue_walk(b->eng, b->jobs, w, b->stream);
 }
 
 static void batch_launch(kkw_batch *b)
@@ -948,6 +984,134 @@ static void serve_loop(ServeState *S)
 }
 
 // ==========================================================================
+// No-allocation runs (kkw_run_noalloc)
+// ==========================================================================
+// Fixed device scratch of kkw_run_noalloc: the work-queue counter, the job
+// table, start lists and scheme tables.  Each GPU holds its own copy of the
+// array until the process ends.  Host code must never use the array's name as
+// an address: the device address of a GPU's copy comes from
+// hipGetSymbolAddress, taken once per device at engine creation.
+static __device__ __align__(256) unsigned char scratch[KKW_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_init_mu;
+std::map<int, void *> g_scratch_addr;                  // device ordinal -> address of its copy
+std::map<int, std::unique_ptr<std::mutex>> g_scratch_mu;   // serialises calls per device
+}  // namespace
+
+// The current device's copy of `scratch` (the caller has made eng->device current).
+static void *scratch_address(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_init_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) return it->second;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        throw Error(KKW_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_scratch_addr[device] = p;
+    g_scratch_mu[device] = std::make_unique<std::mutex>();
+    return p;
+}
+
+static std::mutex &scratch_mutex(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_init_mu);
+    return *g_scratch_mu.at(device);
+}
+
+static int run_noalloc(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mode, uint32_t *const *paths,
+                       const uint32_t *strides, uint32_t *const *lengths)
+{
+    if (!eng->noalloc_scratch) throw Error(KKW_E_DEVICE, "the engine has no scratch address");
+    std::lock_guard<std::mutex> slk(scratch_mutex(eng->device));
+    std::vector<PreparedJob> pjs;
+    for (int i = 0; i < n_jobs; i++) pjs.push_back(prepare_job(eng, jobs[i], false, /*may_build=*/false));
+    // the caller's buffers replace the batch's
+    for (int i = 0; i < n_jobs; i++) {
+        DevJob &d = pjs[i].dj;
+        const uint64_t need = (uint64_t)d.walk_length + 1;
+        if (strides[i] % 4 != 0 || strides[i] < need)
+            throw Error(KKW_E_INVALID, "job " + std::to_string(i) + ": stride must be a multiple of 4 and at least " +
+                                           std::to_string(need));
+        if (d.walker_num && (!paths[i] || !lengths[i]))
+            throw Error(KKW_E_INVALID, "job " + std::to_string(i) + ": null path or length buffer");
+        if ((uintptr_t)paths[i] % 16 != 0 || (uintptr_t)lengths[i] % 4 != 0)
+            throw Error(KKW_E_INVALID, "job " + std::to_string(i) + ": paths must be 16-byte aligned, lengths 4-byte");
+        pjs[i].stride = strides[i];
+        d.stride = strides[i];
+        d.paths = paths[i];
+        d.lens = lengths[i];
+    }
+    WalkLaunch w;
+    w.mode = mode;
+    // layout in the scratch array: [0, 256) queue counter, then 256-byte
+    // aligned job table, start lists and scheme tables
+    size_t off = 256;
+    auto take = [&](size_t n) {
+        size_t o = off;
+        off = al256(off + n);
+        return o;
+    };
+    const size_t jobs_off = (mode == KKW_MODE_PERSISTENT && n_jobs > 1) ? take(sizeof(DevJob) * n_jobs) : 0;
+    std::vector<size_t> starts_off(n_jobs, 0), len_off(n_jobs, 0), mask_off(n_jobs, 0);
+    for (int i = 0; i < n_jobs; i++) {
+        if (!pjs[i].starts.empty()) starts_off[i] = take(pjs[i].starts.size() * 4);
+        if (!pjs[i].scheme_len.empty()) {
+            len_off[i] = take(pjs[i].scheme_len.size() * 4);
+            mask_off[i] = take(pjs[i].scheme_mask.size());
+        }
+    }
+    if (off > KKW_NOALLOC_SCRATCH_BYTES)
+        throw Error(KKW_E_NOMEM, "these jobs need " + std::to_string(off) + " bytes of scratch; the fixed array holds " +
+                                     std::to_string((size_t)KKW_NOALLOC_SCRATCH_BYTES));
+    char *base = (char *)eng->noalloc_scratch;
+    hipStream_t st = eng->stream;
+    for (int i = 0; i < n_jobs; i++) {
+        PreparedJob &pj = pjs[i];
+        if (!pj.starts.empty()) {
+            KKW_HIP(hipMemcpyAsync(base + starts_off[i], pj.starts.data(), pj.starts.size() * 4,
+                                   hipMemcpyHostToDevice, st));
+            pj.dj.starts = (const uint32_t *)(base + starts_off[i]);
+        }
+        if (!pj.scheme_len.empty()) {
+            KKW_HIP(hipMemcpyAsync(base + len_off[i], pj.scheme_len.data(), pj.scheme_len.size() * 4,
+                                   hipMemcpyHostToDevice, st));
+            KKW_HIP(hipMemcpyAsync(base + mask_off[i], pj.scheme_mask.data(), pj.scheme_mask.size(),
+                                   hipMemcpyHostToDevice, st));
+            pj.dj.scheme_len = (const uint32_t *)(base + len_off[i]);
+            pj.dj.scheme_mask = (const uint8_t *)(base + mask_off[i]);
+        }
+    }
+    uint64_t acc = 0;
+    for (auto &pj : pjs) {
+        pj.dj.chunk_begin = acc;
+        acc += (pj.dj.walker_num + w.chunk - 1) / w.chunk;
+        w.line_rows = w.line_rows || (mode == KKW_MODE_PERSISTENT && pj.stride % 32 == 0);
+    }
+    w.n_chunks = acc;
+    w.d_queue = (unsigned long long *)base;
+    std::vector<DevJob> table;
+    if (jobs_off) {
+        for (auto &pj : pjs) table.push_back(pj.dj);
+        KKW_HIP(hipMemcpyAsync(base + jobs_off, table.data(), sizeof(DevJob) * table.size(), hipMemcpyHostToDevice, st));
+        w.d_jobs = (DevJob *)(base + jobs_off);
+    }
+    const double t0 = now_ms();
+    KKW_HIP(hipEventRecord(eng->noalloc_ev[0], st));
+    enqueue_walk(eng, pjs, w, st);
+    KKW_HIP(hipEventRecord(eng->noalloc_ev[1], st));
+    KKW_HIP(hipStreamSynchronize(st));   // also keeps the host sources of the copies alive until read
+    float ms = 0.0f;
+    KKW_HIP(hipEventElapsedTime(&ms, eng->noalloc_ev[0], eng->noalloc_ev[1]));
+    eng->timing.kernel_ms = ms;
+    eng->timing.total_ms = now_ms() - t0;
+    return KKW_OK;
+}
+
+// ==========================================================================
 // C ABI
 // ==========================================================================
 extern "C" {
@@ -973,6 +1137,11 @@ int kkw_engine_create(const kkw_engine_opts *opts, kkw_engine **out)
         KKW_HIP(hipDeviceGetAttribute(&eng->cu_count, hipDeviceAttributeMultiprocessorCount, eng->device));
         KKW_HIP(hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking));
         KKW_HIP(hipStreamCreateWithFlags(&eng->aux_stream, hipStreamNonBlocking));
+        // kkw_run_noalloc's scratch: this device's copy of the fixed array, and
+        // its events, set up here so that the call itself creates nothing
+        eng->noalloc_scratch = scratch_address(eng->device);
+        KKW_HIP(hipEventCreate(&eng->noalloc_ev[0]));
+        KKW_HIP(hipEventCreate(&eng->noalloc_ev[1]));
         *out = eng.release();
         return KKW_OK;
     });
@@ -986,6 +1155,8 @@ int kkw_engine_destroy(kkw_engine *eng)
     eng->gs.release();
     if (eng->stream) (void)hipStreamDestroy(eng->stream);
     if (eng->aux_stream) (void)hipStreamDestroy(eng->aux_stream);
+    for (hipEvent_t e : eng->noalloc_ev)
+        if (e) (void)hipEventDestroy(e);
     delete eng;
     return KKW_OK;
 }
@@ -1397,6 +1568,19 @@ int kkw_run(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mode, int out_
         batch_sync(b);
         for (int i = 0; i < n_jobs; i++) fill_paths_host(b->jobs[i], b->stream, &out[i]);
         return KKW_OK;
+    });
+}
+
+int kkw_run_noalloc(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mode, uint32_t *const *paths,
+                    const uint32_t *strides, uint32_t *const *lengths)
+{
+    return guarded([&]() -> int {
+        if (!eng || n_jobs < 0 || (n_jobs > 0 && (!jobs || !paths || !strides || !lengths)))
+            return fail(KKW_E_INVALID, "null argument");
+        if (mode != KKW_MODE_PLAIN && mode != KKW_MODE_PERSISTENT) return fail(KKW_E_INVALID, "unknown mode");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        return run_noalloc(eng, jobs, n_jobs, mode, paths, strides, lengths);
     });
 }
 

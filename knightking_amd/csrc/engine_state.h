@@ -84,6 +84,10 @@ struct kkw_engine {
     // before replaying a captured launch
     uint64_t graph_version = 1;
     std::unique_ptr<ServeState> serve;
+    // kkw_run_noalloc: this device's copy of the library's fixed scratch array
+    // (its device address, from hipGetSymbolAddress) and the two timing events
+    void *noalloc_scratch = nullptr;
+    hipEvent_t noalloc_ev[2] = {nullptr, nullptr};
 };
 
 // Neighbour hash sets when the engine builds them on its own (first node2vec
