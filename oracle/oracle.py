@@ -164,11 +164,14 @@ def alias_build(row_ptr, col, w, etype=None):
 
 
 def vmask_of(row_ptr, etype):
+    """Per vertex, the OR of (1 << type) over its out-edges (metapath.hpp:102-121)."""
     V = len(row_ptr) - 1
     vm = np.zeros(V, np.uint8)
     deg = np.diff(row_ptr.astype(np.int64))
-    src = np.repeat(np.arange(V), deg)
-    np.bitwise_or.at(vm, src, (1 << etype.astype(np.uint32)).astype(np.uint8))
+    nz = np.nonzero(deg > 0)[0]
+    if len(nz):
+        bits = np.left_shift(np.uint8(1), etype.astype(np.uint8))
+        vm[nz] = np.bitwise_or.reduceat(bits, row_ptr[:-1].astype(np.int64)[nz])
     return vm
 
 
@@ -192,6 +195,7 @@ class Graph:
     w: Optional[np.ndarray] = None
     etype: Optional[np.ndarray] = None
     alias: Optional[tuple] = None   # (p, prim, alias, meta, vsum)
+    vmask: Optional[np.ndarray] = None   # metapath type masks, computed on first use
 
     @property
     def V(self):
@@ -237,7 +241,9 @@ def walk(g: Graph, kind, walker_num, walk_length=0, use_weights=True, walker_beg
         for i, s in enumerate(schemes):
             masks[i, :len(s)] = s
         masks = np.ascontiguousarray(masks.reshape(-1))
-        vm = vmask_of(g.row_ptr, g.etype)
+        if g.vmask is None:
+            g.vmask = vmask_of(g.row_ptr, g.etype)
+        vm = g.vmask
         keep += [lens, masks, vm]
         j.n_schemes = len(schemes)
         j.scheme_len = lens.ctypes.data

@@ -193,6 +193,49 @@ def test_alias_matches_oracle_long_lists(engine, seed):
     assert np.array_equal(meta, meta_o)
 
 
+def _integer_weight_graph(seed):
+    """Undirected graph with integer weights rand() % 5 + 1 as the reference's
+    test generator draws them (src/tests/generator_helper.hpp:65-76): ties with
+    a list's average weight are common.  Hubs with 40-700 neighbours take the
+    wave-per-list alias build; the rest the lane-per-list one."""
+    rng = np.random.default_rng(seed)
+    V = 3000
+    pairs = [rng.integers(0, V, (12000, 2))]
+    for h in range(12):
+        k = int(rng.integers(40, 700))
+        pairs.append(np.stack([np.full(k, h), rng.integers(0, V, k)], 1))
+    e = np.concatenate(pairs).astype(np.int64)
+    e = e[e[:, 0] != e[:, 1]]
+    e = np.unique(np.sort(e, 1), axis=0)
+    w = (rng.integers(0, 5, len(e)) + 1).astype(np.float32)
+    return V, e[:, 0].astype(np.uint32), e[:, 1].astype(np.uint32), w
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_integer_weights_alias_and_walks(engine, seed):
+    """Alias tables and walks on integer weights, bit for bit against the oracle."""
+    V, src, dst, w = _integer_weight_graph(seed)
+    print(f"[gpu] integer weights, seed {seed}, {len(src)} undirected edges", flush=True)
+    engine.graph_from_edges(V, src, dst, w, make_undirected=True, sort_adj=True)
+    engine.prepare()
+    rp, col, wd, _ = engine.download()
+    p, prim, al, meta, vs = engine.alias_download()
+    p_o, prim_o, al_o, meta_o, vs_o = O.alias_build(rp, col, wd)
+    assert np.array_equal(vs.view(np.uint32), vs_o.view(np.uint32))
+    assert np.array_equal(p.view(np.uint32), p_o.view(np.uint32))
+    assert np.array_equal(prim, prim_o)
+    assert np.array_equal(al, al_o)
+    # ties decide buckets here: some list has a weight equal to its average
+    ave = np.repeat(vs_o / np.maximum(np.diff(rp.astype(np.int64)), 1), np.diff(rp.astype(np.int64)))
+    assert np.any(wd == ave.astype(np.float32))
+    g = O.Graph(rp, col, wd)
+    for mode in (kk.MODE_PLAIN, kk.MODE_PERSISTENT):
+        for job in (dict(kind=kk.DEEPWALK, walker_num=6000, walk_length=40, seed=60 + seed),
+                    dict(kind=kk.NODE2VEC, walker_num=6000, walk_length=40, p=2.0, q=0.5, seed=70 + seed)):
+            gpu, op, ol, ot = _walk_both(engine, g, mode, **dict(job))
+            assert_paths_equal(gpu, op, ol, ot, f"integer weights kind {job['kind']} mode {mode}")
+
+
 def _walk_both(engine, g, mode, **job):
     kind = job.pop("kind")
     n = job.pop("walker_num")
@@ -341,9 +384,13 @@ def test_node2vec_rmat14_distribution(engine):
 
 
 def test_mixed_batch_equals_single_jobs(engine):
-    """All kinds in one persistent launch give the same paths as each job alone."""
+    """All kinds in one persistent launch (the mixed-kind kernel) give each job
+    the oracle's paths, and the same paths as the job alone in a plain launch."""
+    from helpers import assert_rows_match_oracle, oracle_walk
     load_rmat(engine, 10)
     engine.type_by_dst(5, 9)
+    rp, col, w, et = engine.download(etype=True)
+    g = O.Graph(rp, col, w, et)
     jobs = [
         kk.deepwalk_job(700, 33, True, seed=31),
         kk.ppr_job(900, 0.05, True, seed=32, start_vertices=np.array([1, 2, 3], np.uint32)),
@@ -355,7 +402,9 @@ def test_mixed_batch_equals_single_jobs(engine):
         kk.deepwalk_job(800, 63, True, seed=38),   # 64-entry rows: half-line write-out
     ]
     mixed = engine.run(jobs, mode=kk.MODE_PERSISTENT)
-    for j, m in zip(jobs, mixed):
+    for i, (j, m) in enumerate(zip(jobs, mixed)):
+        op, ol, ot, _ = oracle_walk(g, j)
+        assert_rows_match_oracle(m, op, ol, ot, what=f"job {i} in the mix")
         (alone,) = engine.run([j], mode=kk.MODE_PLAIN)
         assert_same_rows(alone, m)
 
@@ -460,12 +509,20 @@ def test_mixed_jobs_rmat23_properties(engine):
     rp, col, _, et = engine.download(weights=False, etype=True)
     vtype = O.vertex_types(len(rp) - 1, 5, 3)
     assert np.array_equal(et, vtype[col])
+    from helpers import assert_rows_match_oracle, oracle_walk
     jobs, kinds = bench.mixed_jobs(kk)
     print(f"[gpu] 64 mixed jobs on R-MAT-23, {sum(j.walker_num for j in jobs)} walkers", flush=True)
     mixed = engine.run(jobs, mode=kk.MODE_PERSISTENT)
+    _, _, w, _ = engine.download(weights=True)
+    g = O.Graph(rp, col, w, et)
     rng = np.random.default_rng(2)
     us, vs = [], []
     for j, (job, kind, m) in enumerate(zip(jobs, kinds, mixed)):
+        # every job's rows of a strided sample of about 2000 walkers against the oracle
+        stride = max(1, job.walker_num // 2000)
+        n = (job.walker_num + stride - 1) // stride
+        op, ol, ot, _ = oracle_walk(g, job, walker_num=n, walker_stride=stride)
+        assert_rows_match_oracle(m, op, ol, ot, np.arange(n) * stride, f"job {j} ({kind}) in the mix")
         (alone,) = engine.run([job], mode=kk.MODE_PLAIN)
         assert_same_rows(alone, m, f"job {j} ({kind})")
         if kind != "metapath":

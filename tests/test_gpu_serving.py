@@ -57,8 +57,15 @@ def mixed_queries(V, n, seed):
 
 
 def test_results_same_alone_and_in_the_mix(served):
+    """Queries of every kind submitted from four threads: each served result
+    (packed or row form, through flights, packing and the pinned hand-back)
+    equals the oracle's walk of that query and the query run alone."""
+    from helpers import assert_rows_match_oracle, oracle_walk, packed_matches_oracle
+    from oracle import oracle as O
     eng = served
     V, _ = eng.info()
+    rp, col, w, et = eng.download(etype=True)
+    g = O.Graph(rp, col, w, et)
     jobs = mixed_queries(V, 40, 1)
     alone = [eng.run([j])[0] for j in jobs]
     eng.serve_start(max_walkers_in_flight=20000)
@@ -76,6 +83,11 @@ def test_results_same_alone_and_in_the_mix(served):
     for i, t in enumerate(tickets):
         got, lat = eng.wait(t, 60.0, packed=bool(i % 2))
         assert lat >= 0
+        op, ol, ot, _ = oracle_walk(g, jobs[i])
+        if i % 2:
+            assert packed_matches_oracle(got, op, ol, ot), f"query {i}: served result differs from the oracle"
+        else:
+            assert_rows_match_oracle(got, op, ol, ot, what=f"query {i} served")
         assert same(got, alone[i]), f"query {i} differs in the mix"
 
 
