@@ -33,6 +33,51 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def host_cpus():
+    """The host's CPU model and counts, and the threads the CPU baseline uses:
+    every CPU this process may run on (affinity mask, cgroup CPU quota), capped
+    by OMP_NUM_THREADS where the host sets it (the CPU share of one GPU on a
+    shared box)."""
+    info = {"cpu_model": None, "nproc": os.cpu_count(), "affinity_cpus": None, "cgroup_cpus": None,
+            "omp_num_threads": None}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["cpu_model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                info["cgroup_cpus"] = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        info["omp_num_threads"] = int(omp)
+    caps = [info[k] for k in ("nproc", "affinity_cpus", "cgroup_cpus", "omp_num_threads") if info[k]]
+    info["threads"] = max(1, min(caps)) if caps else 1
+    return info
+
+
+def source_digest():
+    """Digest of the sources that decide the walk kernel's memory traffic (the
+    kernels, the table layouts and their construction)."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in ("walk.hip", "graph.hip", "kkw_dev.h", "kkw_rng.h"):
+        with open(os.path.join(ROOT, "knightking_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def reference_model_bytes(t, weighted):
     """Algorithmic bytes of one launch under the per-step model of KnightKing's
     node2vec loop: per step 64 B (row_ptr pair) + 4 B (path entry); per trial
@@ -80,7 +125,7 @@ def cpu_baseline(eng, job_args, gpu_batch, V, seconds=20.0):
     p, prim, al, meta, vs = eng.alias_download()
     g = O.Graph(rp, col, w, None, (p, prim, al, meta, vs))
     log(f"cpu baseline: graph and alias tables copied to host in {time.perf_counter() - t0:.1f}s")
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    threads = host_cpus()["threads"]
     # calibrate on a small sample, then size the timed sample to ~`seconds`
     stride = 4096
     n = V // stride
@@ -205,14 +250,13 @@ def program_line(kk, eng, V, builtin_paths):
     (knightking_amd/programs/node2vec.cpp, compiled at run time), on the
     resident R-MAT-24 graph: kernel time of one run and equality with the
     built-in kernel's paths."""
-    from oracle import oracle as O
     with open(os.path.join(ROOT, "knightking_amd", "programs", "node2vec.cpp")) as f:
         src = f.read()
     n = builtin_paths.vertices.shape[0]
     mask = np.arange(81)[None, :] < builtin_paths.lengths[:, None]
     out = {"adjacency": "kkw_adjacent (neighbour hash sets)",
            "tables": "the engine's buckets (ids + rows; the program reads no edge index)"}
-    up, lo, ip, iq, ov = O.node2vec_constants(2.0, 0.5)
+    up, lo, ip, iq, ov = kk.node2vec_constants(2.0, 0.5)
     params = [80, up, lo, ip, iq, ov, 1.0]
     # the program as shipped, then rebuilt to draw from local-index tables
     for prefix, options, modes in (("", None, (("plain", kk.MODE_PLAIN), ("persistent", kk.MODE_PERSISTENT))),
@@ -261,7 +305,7 @@ def rmat14_line(kk, steps, warmup):
         rp, col, w, _ = eng.download(weights=True)
         g = O.Graph(rp, col, w)
         g.with_alias()
-        threads = max(1, min(os.cpu_count() or 1, 16))
+        threads = host_cpus()["threads"]
         reps, dt = 0, 0.0
         while dt < 2.0:
             t0 = time.perf_counter()
@@ -278,6 +322,81 @@ def rmat14_line(kk, steps, warmup):
         eng.close()
 
 
+def serve_stream(eng, jobs, gaps):
+    """Submit `jobs` at the given inter-arrival gaps while a second thread
+    collects results in submission order (steps and latency; paths are freed
+    unread); returns (latencies ms, steps, span s, error or None).  The
+    submitter sleeps to within 0.2 ms of each arrival and spins the rest, and
+    the interpreter switches threads every 0.1 ms, so the collector is never
+    starved of the interpreter lock while the submitter waits."""
+    n_q = len(jobs)
+    tickets, lat, steps, failure = [], [], [0], []
+    done = threading.Event()
+    c_jobs = [j.to_c() for j in jobs]   # descriptors built before the clock starts
+
+    def collect():
+        i = 0
+        try:
+            while i < n_q:
+                if i >= len(tickets):
+                    time.sleep(0.0001)
+                    continue
+                s, ms = eng.wait_steps(tickets[i], 30.0)
+                lat.append(ms)
+                steps[0] += s
+                i += 1
+        except Exception as e:   # a stalled or failed query ends this measurement
+            failure.append(repr(e))
+        done.set()
+
+    old_switch = sys.getswitchinterval()
+    sys.setswitchinterval(1e-4)
+    try:
+        th = threading.Thread(target=collect)
+        th.start()
+        t_start = time.perf_counter()
+        t_next = t_start
+        for i in range(n_q):
+            t_next += gaps[i]
+            ahead = t_next - time.perf_counter()
+            if ahead > 3e-4:
+                time.sleep(ahead - 2e-4)
+            while time.perf_counter() < t_next:
+                pass
+            tickets.append(eng.submit_c(c_jobs[i]))
+        done.wait()
+        th.join()
+    finally:
+        sys.setswitchinterval(old_switch)
+    return np.array(lat), steps[0], time.perf_counter() - t_start, (failure[0] if failure else None)
+
+
+def serve_session(eng, jobs, gaps):
+    """One serving session over a query stream, with the loop's flight trace:
+    returns the latency / throughput summary and the flights' occupancy."""
+    import tempfile
+    fd, trace = tempfile.mkstemp(suffix=".csv")
+    os.close(fd)
+    os.environ["KKW_SERVE_TRACE"] = trace
+    try:
+        eng.serve_start(max_walkers_in_flight=1 << 22)
+        try:
+            lat, steps, span, err = serve_stream(eng, jobs, gaps)
+        finally:
+            eng.serve_stop()
+        out = {"queries": len(jobs)}
+        if err:
+            out.update({"completed": len(lat), "error": err})
+        else:
+            out.update({"p50_ms": round(float(np.percentile(lat, 50)), 3),
+                        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                        "steps_per_s": round(steps / span, 1), "occupancy": flight_occupancy(trace)})
+        return out
+    finally:
+        os.environ.pop("KKW_SERVE_TRACE", None)
+        os.unlink(trace)
+
+
 def serve_latency(eng, rp, rates, seconds, seed=7):
     """Poisson PPR queries (t=0.15, 1e3..1e5 walkers log-uniform, source =
     source of a uniformly drawn directed edge) at each rate; latency from
@@ -286,61 +405,72 @@ def serve_latency(eng, rp, rates, seconds, seed=7):
     rng = np.random.default_rng(seed)
     E = int(rp[-1])
     out = {}
-    eng.serve_start(max_walkers_in_flight=1 << 22)
-    try:
-        for rate in rates:
-            n_q = int(rate * seconds)
-            gaps = rng.exponential(1.0 / rate, n_q)
-            edges = rng.integers(0, E, n_q)
-            srcs = np.searchsorted(rp, edges, side="right") - 1
-            walkers = np.round(10 ** (3 + 2 * rng.random(n_q))).astype(np.int64)
-            tickets = []
-            lat = []
-            steps = [0]
-            done = threading.Event()
+    for rate in rates:
+        n_q = int(rate * seconds)
+        gaps = rng.exponential(1.0 / rate, n_q)
+        srcs = np.searchsorted(rp, rng.integers(0, E, n_q), side="right") - 1
+        walkers = np.round(10 ** (3 + 2 * rng.random(n_q))).astype(np.int64)
+        jobs = [kk.ppr_job(int(walkers[i]), 0.15, True, seed=1000 + i,
+                           start_vertices=np.array([srcs[i]], np.uint32)) for i in range(n_q)]
+        out[str(rate)] = serve_session(eng, jobs, gaps)
+        log(f"serving {rate} qps: {out[str(rate)]}")
+        if "error" in out[str(rate)]:
+            break
+    return out
 
-            failure = []
 
-            def collect():
-                i = 0
-                try:
-                    while i < n_q:
-                        if i >= len(tickets):
-                            time.sleep(0.0002)
-                            continue
-                        paths, ms = eng.wait(tickets[i], 30.0, packed=True)
-                        lat.append(ms)
-                        steps[0] += paths.steps
-                        i += 1
-                except Exception as e:   # a stalled or failed query ends this rate's measurement
-                    failure.append(repr(e))
-                done.set()
+def flight_occupancy(trace_file):
+    """From the serving loop's flight trace: time-weighted mean number of
+    flights on the device (admission to completion seen) and the fraction of
+    the busy span with all four streams occupied."""
+    rows = np.loadtxt(trace_file, delimiter=",", skiprows=1, ndmin=2)
+    if rows.size == 0:
+        return None
+    t0, t1 = rows[:, 1], rows[:, 3]   # admit_ms, phase1_seen_ms
+    ev = np.concatenate([np.stack([t0, np.ones_like(t0)], 1), np.stack([t1, -np.ones_like(t1)], 1)])
+    ev = ev[np.lexsort((ev[:, 1], ev[:, 0]))]
+    level, last, area, full, span0 = 0, ev[0, 0], 0.0, 0.0, ev[0, 0]
+    for t, d in ev:
+        dt = t - last
+        area += level * dt
+        if level >= 4:
+            full += dt
+        level += int(d)
+        last = t
+    span = max(last - span0, 1e-9)
+    return {"flights": int(len(rows)), "mean_flights_in_flight": round(float(area / span), 3),
+            "frac_time_four_busy": round(float(full / span), 4),
+            "mean_queries_per_flight": round(float(rows[:, 6].mean()), 2)}
 
-            th = threading.Thread(target=collect)
-            th.start()
-            t_start = time.perf_counter()
-            t_next = t_start
-            for i in range(n_q):
-                t_next += gaps[i]
-                while time.perf_counter() < t_next:
-                    pass
-                job = kk.ppr_job(int(walkers[i]), 0.15, True, seed=1000 + i,
-                                 start_vertices=np.array([srcs[i]], np.uint32))
-                tickets.append(eng.submit(job))
-            done.wait()
-            th.join()
-            span = time.perf_counter() - t_start
-            if failure:
-                out[str(rate)] = {"queries": n_q, "completed": len(lat), "error": failure[0]}
-                log(f"serving {rate} qps: {out[str(rate)]}")
-                break
-            lat = np.array(lat)
-            out[str(rate)] = {"queries": n_q, "p50_ms": round(float(np.percentile(lat, 50)), 3),
-                              "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                              "steps_per_s": round(steps[0] / span, 1)}
-            log(f"serving {rate} qps: {out[str(rate)]}")
-    finally:
-        eng.serve_stop()
+
+def serve_mixed(eng, rp, rate, seconds, seed=8):
+    """Mixed queries at `rate`: kind uniform over PPR (t = 0.15, degree-
+    proportional source), DeepWalk (L = 80) and node2vec (p = 2, q = 0.5,
+    L = 80), all weighted, 1e3..1e5 walkers log-uniform; p50/p99 latency and
+    how many of the loop's four flights were busy (from its flight trace)."""
+    import knightking_amd as kk
+    rng = np.random.default_rng(seed)
+    E = int(rp[-1])
+    n_q = int(rate * seconds)
+    gaps = rng.exponential(1.0 / rate, n_q)
+    walkers = np.round(10 ** (3 + 2 * rng.random(n_q))).astype(np.int64)
+    kinds = rng.integers(0, 3, n_q)
+    srcs = np.searchsorted(rp, rng.integers(0, E, n_q), side="right") - 1
+    V = len(rp) - 1
+    jobs = []
+    for i in range(n_q):
+        begin = int(rng.integers(0, V))
+        if kinds[i] == 0:
+            jobs.append(kk.ppr_job(int(walkers[i]), 0.15, True, seed=3000 + i,
+                                   start_vertices=np.array([srcs[i]], np.uint32)))
+        elif kinds[i] == 1:
+            jobs.append(kk.deepwalk_job(int(walkers[i]), 80, True, seed=3000 + i, walker_begin=begin))
+        else:
+            jobs.append(kk.node2vec_job(int(walkers[i]), 80, 2.0, 0.5, True, seed=3000 + i, walker_begin=begin))
+    out = {"rate_qps": rate, "kinds": "PPR t=0.15 / DeepWalk L=80 / node2vec L=80, weighted",
+           "walkers": "10^(3+2U)"}
+    out.update(serve_session(eng, jobs, gaps))
+    log(f"mixed serving {rate} qps: {out}")
     return out
 
 
@@ -440,19 +570,36 @@ def main():
     accesses = t["row_loads"] + t["trials"] + t["search_lines"]
     access_rate = accesses / kernel_s
 
-    achieved = bytes_launch / kernel_s / 1e9
+    # algorithmic bytes: one 64-byte transaction per data-dependent random
+    # access + path and length bytes; the 128-byte L2-line figure beside it
+    achieved = touched / kernel_s / 1e9
+    achieved_128 = bytes_launch / kernel_s / 1e9
+    # HBM traffic per launch: measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE)
+    # on a run of this workload, reported only when that profile was taken on
+    # the same kernel and table-construction sources as this run
     traffic = None
+    traffic_source = None
     tf = os.path.join(ROOT, "profiles", "node2vec_rmat24_hbm_traffic.json")
     if os.path.exists(tf) and weighted and args.scale == 24:
         with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        if prof.get("source_digest") == source_digest():
+            traffic = prof.get("hbm_bytes_per_launch")
+            traffic_source = f"{prof.get('source')}: {prof.get('command')} (sources {prof['source_digest']})"
+        else:
+            traffic_source = (f"none: {tf} was profiled on sources {prof.get('source_digest')}, "
+                              f"this run's are {source_digest()}")
 
     cpu = None
     parity = None
     if not args.no_cpu and weighted:
         rate, threads, sample, parity = cpu_baseline(eng, job_args, pb, V)
+        hc = host_cpus()
         cpu = {"value": round(rate, 1), "unit": "walker steps/s", "cores": threads, "kind": "port",
-               "sample": sample}
+               "sample": sample, "cpu_model": hc["cpu_model"], "nproc": hc["nproc"],
+               "affinity_cpus": hc["affinity_cpus"], "cgroup_cpus": hc["cgroup_cpus"],
+               "omp_num_threads": hc["omp_num_threads"],
+               "threads_rule": "all CPUs the process may use (affinity, cgroup quota), capped by OMP_NUM_THREADS"}
         log(f"cpu baseline {rate / 1e6:.2f} M steps/s on {threads} threads; sample parity {parity}")
     run_extra = not args.no_extra and weighted and args.scale == 24
     builtin_paths = pb.paths(0) if run_extra else None
@@ -486,10 +633,18 @@ def main():
         eng.graph_rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=1, weighted=True)
         eng.prepare()
 
+    if cpu is not None and extra_rmat14 is not None:
+        # the reference's CPU-runnable case timed whole on the same cores
+        cpu["config1_whole_job"] = {
+            "workload": "node2vec p=2 q=0.5 weighted, R-MAT scale 14, 16384 walkers (one per vertex), L=80, seed 1",
+            "steps_per_s": extra_rmat14["cpu_oracle_steps_per_s"], "threads": extra_rmat14["cpu_threads"],
+            "gpu_plain_steps_per_s": extra_rmat14["plain_steps_per_s"], "bit_exact": extra_rmat14["bit_exact"]}
+
     serving = None
     if not args.no_serve:
         rp, _, _, _ = eng.download(weights=False)
         serving = serve_latency(eng, rp, [200, 500, 1000, 2000], args.serve_seconds)
+        serving["mixed"] = [serve_mixed(eng, rp, r, args.serve_seconds) for r in (2000, 8000)]
 
     line = {
         "metric": "walker steps/s, node2vec RMAT-24 1 GPU (batched vs plain); p99 PPR query latency",
@@ -530,10 +685,16 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "algorithmic_bytes_per_launch": int(bytes_launch),
-            "algorithmic_bytes_per_step": round(bytes_launch / max(steps_per_launch, 1), 1),
-            "bytes_model": "one 128 B L2 line per data-dependent random access + path/length bytes",
-            "bytes_per_step_64B_lines": round(touched / max(steps_per_launch, 1), 1),
+            "traffic_source": traffic_source,
+            "algorithmic_bytes_per_launch": int(touched),
+            "algorithmic_bytes_per_step": round(touched / max(steps_per_launch, 1), 1),
+            "bytes_model": "one 64 B transaction per data-dependent random access (row pair at a walker start, "
+                           "alias bucket, each distinct adjacency-probe block) + path and length bytes",
+            "random_line_ceiling_frac": None if gather_lps is None else
+            round(gather_lps * LINE / 1e9 / HBM_PEAK_GBS, 4),
+            "lines_128B": {"bytes_per_launch": int(bytes_launch),
+                           "bytes_per_step": round(bytes_launch / max(steps_per_launch, 1), 1),
+                           "achieved": round(achieved_128, 1), "frac": round(achieved_128 / HBM_PEAK_GBS, 4)},
             "reference_model_bytes_per_step": round(ref_bytes / max(steps_per_launch, 1), 1),
             "reference_model_gbs": round(ref_bytes / kernel_s / 1e9, 1),
             "traffic_gbs": None if traffic is None else round(traffic / kernel_s / 1e9, 1),

@@ -102,6 +102,37 @@ EXPORTED = [
 _lib = None
 
 
+def hip_runtimes(maps_text: Optional[str] = None) -> List[str]:
+    """Distinct HIP runtime libraries (libamdhip64) mapped into this process,
+    from /proc/self/maps (or the given text of such a file)."""
+    if maps_text is None:
+        try:
+            with open("/proc/self/maps") as f:
+                maps_text = f.read()
+        except OSError:
+            return []
+    paths = set()
+    for line in maps_text.splitlines():
+        parts = line.split()
+        if len(parts) >= 6 and "libamdhip64.so" in os.path.basename(parts[-1]):
+            paths.add(parts[-1])
+    return sorted(paths)
+
+
+def check_one_hip_runtime(where: str, maps_text: Optional[str] = None) -> None:
+    """Raise before any GPU call when two HIP runtimes are mapped: torch ships
+    its own libamdhip64, and a process that loaded libkkw (with the system
+    runtime) before torch initialised its GPU ends up with both, each driving
+    the GPU with its own state; GPU calls through the second one have crashed
+    (a segmentation fault inside a torch allocation).  Import torch and call
+    torch.cuda.init() before the first Engine, so libkkw binds to torch's."""
+    rt = hip_runtimes(maps_text)
+    if len(rt) > 1:
+        raise RuntimeError(f"{where}: two HIP runtimes are mapped in this process ({', '.join(rt)}); "
+                           "import torch and call torch.cuda.init() before the first Engine, or keep torch "
+                           "out of the process")
+
+
 def load_library(path: str = LIB_PATH):
     """Load libkkw.so (raises if it has not been built)."""
     global _lib
@@ -219,6 +250,23 @@ class Job:
             j.scheme_len = lens.ctypes.data_as(C.POINTER(C.c_uint32))
             j.scheme_mask = masks.ctypes.data_as(C.POINTER(C.c_uint8))
         return j
+
+
+def node2vec_constants(p: float, q: float) -> List[float]:
+    """node2vec's bounds and dynamic components as the engine computes them
+    for a job: upper = max(1, 1/q) and lower = min(1/p, 1, 1/q) in double,
+    stored as float (src/apps/node2vec.hpp:126-127); 1/p and 1/q in float
+    (node2vec.hpp:182-191); the outlier overflow 1/p - upper in double, stored
+    as float (node2vec.hpp:136).  Returns [upper, lower, 1/p, 1/q, overflow],
+    the parameters 1-5 of the node2vec walk program (programs/node2vec.cpp)."""
+    p32, q32 = np.float32(p), np.float32(q)
+    ip, iq = 1.0 / float(p32), 1.0 / float(q32)
+    upper = np.float32(max(1.0, iq))
+    lower = np.float32(min(ip, min(1.0, iq)))
+    inv_p = np.float32(1.0) / p32
+    inv_q = np.float32(1.0) / q32
+    overflow = np.float32(ip - float(upper))
+    return [float(upper), float(lower), float(inv_p), float(inv_q), float(overflow)]
 
 
 def node2vec_job(walker_num, walk_length, p, q, weighted=True, seed=1, **kw) -> Job:
@@ -432,6 +480,7 @@ class Batch:
         only one of the two, so import torch and initialise its GPU before the
         first Engine: libkkw then binds to the runtime torch loaded."""
         import torch
+        check_one_hip_runtime("Batch.paths_torch")
         if not torch.cuda.is_available():
             raise RuntimeError("torch does not see the GPU in this process: import torch and call "
                                "torch.cuda.init() before the first Engine, so that libkkw and torch "
@@ -469,6 +518,7 @@ class Engine:
 
     def __init__(self, device: int = 0, hbm_budget: int = 0, verbose: bool = False):
         load_library()
+        check_one_hip_runtime("Engine()")
         o = _Opts(device, 1 if verbose else 0, hbm_budget)
         h = C.c_void_p()
         _check(_lib.kkw_engine_create(C.byref(o), C.byref(h)))
@@ -624,6 +674,13 @@ class Engine:
         _check(_lib.kkw_submit(self.h, C.byref(cj), C.byref(t)))
         return t.value
 
+    def submit_c(self, cj: "_Job") -> int:
+        """submit() of a descriptor already built with Job.to_c() (the Job
+        must stay alive: the descriptor points into its arrays)."""
+        t = C.c_uint64()
+        _check(_lib.kkw_submit(self.h, C.byref(cj), C.byref(t)))
+        return t.value
+
     def poll(self, ticket: int) -> bool:
         d = C.c_int()
         _check(_lib.kkw_poll(self.h, ticket, C.byref(d)))
@@ -637,6 +694,16 @@ class Engine:
         lat = C.c_double()
         _check(_lib.kkw_wait(self.h, ticket, timeout_s, C.byref(p), C.byref(lat)))
         return (_take_packed(p) if packed else _take_paths(p)), lat.value
+
+    def wait_steps(self, ticket: int, timeout_s: float = -1.0):
+        """(steps, latency ms) of a served query; its paths are freed unread
+        (load generators that only measure)."""
+        p = _Paths()
+        lat = C.c_double()
+        _check(_lib.kkw_wait(self.h, ticket, timeout_s, C.byref(p), C.byref(lat)))
+        steps = int(p.steps)
+        _lib.kkw_paths_free(C.byref(p))
+        return steps, lat.value
 
     def cancel(self, ticket: int):
         _check(_lib.kkw_cancel(self.h, ticket))

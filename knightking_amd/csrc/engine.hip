@@ -151,6 +151,16 @@ static PreparedJob prepare_job(kkw_engine *eng, const kkw_job &j, bool line_rows
     return pj;
 }
 
+// Entry points that replace or free device graph storage refuse while the
+// serving loop runs: its flights read the tables between admission and
+// completion without holding the engine lock (tables the loop only adds, on
+// first use by a query, are never freed while it runs).
+static void refuse_while_serving(const kkw_engine *eng)
+{
+    if (eng->serve)
+        throw Error(KKW_E_BUSY, "the graph and its tables cannot change while serving (call kkw_serve_stop first)");
+}
+
 // --------------------------------------------------------------------------
 struct kkw_batch {
     kkw_engine *eng = nullptr;
@@ -551,6 +561,7 @@ struct Request {
     double t_submit = 0.0, t_done = 0.0;
     int status = 1;  // 1 pending, 0 done, <0 error
     bool cancelled = false;
+    uint32_t graph_V = 0;   // vertex count of the graph the query was checked against
     kkw_paths result{};
 };
 
@@ -716,7 +727,15 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
     memcpy(H + o_pjobs, pjs.data(), sizeof(PackJob) * nj);
     memset(H + o_queue, 0, sizeof(unsigned long long));
     // ---- enqueue: upload, walk, pack into pinned memory, lengths + offsets
-    DevGraph g = eng->gs.view();
+    DevGraph g;
+    {
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        g = eng->gs.view();
+    }
+    for (auto &r : f.reqs)   // what each query was admitted with is still there
+        if ((r->pj.dj.use_alias && !g.bucket) || (r->pj.dj.kind == KKW_METAPATH && !g.vmask) || !g.row_ptr ||
+            g.V != r->graph_V)
+            throw Error(KKW_E_NOGRAPH, "the graph or the tables a query needs are gone");
     KKW_HIP(hipMemcpyAsync(D, H, upload_bytes, hipMemcpyHostToDevice, st));
     if (chunk_acc) {
         uint64_t blocks_needed = (chunk_acc * 64 + 255) / 256;
@@ -1168,6 +1187,7 @@ int kkw_graph_from_edges(kkw_engine *eng, uint32_t v_num, const void *edges, uin
         if (!eng) return fail(KKW_E_INVALID, "engine is null");
         if (n_edges && !edges) return fail(KKW_E_INVALID, "edges is null");
         std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        refuse_while_serving(eng);
         KKW_HIP(hipSetDevice(eng->device));
         double t0 = now_ms();
         std::vector<uint32_t> src(n_edges), dst(n_edges);
@@ -1195,6 +1215,7 @@ int kkw_graph_from_arrays(kkw_engine *eng, uint32_t v_num, const uint32_t *src, 
         if (!eng) return fail(KKW_E_INVALID, "engine is null");
         if (n_edges && (!src || !dst)) return fail(KKW_E_INVALID, "src/dst is null");
         std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        refuse_while_serving(eng);
         KKW_HIP(hipSetDevice(eng->device));
         double t0 = now_ms();
         eng->graph_version++;
@@ -1210,6 +1231,7 @@ int kkw_graph_sort_adj(kkw_engine *eng)
     return guarded([&]() -> int {
         if (!eng) return fail(KKW_E_INVALID, "engine is null");
         std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        refuse_while_serving(eng);
         KKW_HIP(hipSetDevice(eng->device));
         eng->graph_version++;
         sort_adjacency(eng->gs, eng->stream);
@@ -1272,6 +1294,7 @@ int kkw_graph_from_csr(kkw_engine *eng, uint32_t v_num, const uint64_t *row_ptr,
     return guarded([&]() -> int {
         if (!eng || !row_ptr || (!col && row_ptr[v_num])) return fail(KKW_E_INVALID, "null argument");
         std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        refuse_while_serving(eng);
         KKW_HIP(hipSetDevice(eng->device));
         double t0 = now_ms();
         for (uint32_t v = 0; v < v_num; v++)
@@ -1291,6 +1314,7 @@ int kkw_graph_rmat(kkw_engine *eng, int scale, int edge_factor, double a, double
         if (!(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0))
             return fail(KKW_E_INVALID, "R-MAT probabilities must be non-negative with a+b+c <= 1");
         std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        refuse_while_serving(eng);
         KKW_HIP(hipSetDevice(eng->device));
         double t0 = now_ms();
         auto thr = [](double x) -> uint32_t {
@@ -1310,6 +1334,7 @@ int kkw_graph_set_edge_types(kkw_engine *eng, const uint8_t *etype)
     return guarded([&]() -> int {
         if (!eng || !etype) return fail(KKW_E_INVALID, "null argument");
         std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        refuse_while_serving(eng);
         KKW_HIP(hipSetDevice(eng->device));
         eng->graph_version++;
         set_edge_types(eng->gs, etype, eng->stream);
@@ -1322,6 +1347,7 @@ int kkw_graph_type_by_dst(kkw_engine *eng, int ntypes, uint64_t seed)
     return guarded([&]() -> int {
         if (!eng) return fail(KKW_E_INVALID, "engine is null");
         std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        refuse_while_serving(eng);
         KKW_HIP(hipSetDevice(eng->device));
         eng->graph_version++;
         type_by_dst(eng->gs, ntypes, seed, eng->stream);
@@ -1367,6 +1393,7 @@ int kkw_prepare(kkw_engine *eng)
     return guarded([&]() -> int {
         if (!eng) return fail(KKW_E_INVALID, "engine is null");
         std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        refuse_while_serving(eng);
         KKW_HIP(hipSetDevice(eng->device));
         if (!eng->gs.row_ptr) return fail(KKW_E_NOGRAPH, "no graph loaded");
         eng->graph_version++;
@@ -1424,6 +1451,7 @@ int kkw_graph_adj_index(kkw_engine *eng, int enable)
     return guarded([&]() -> int {
         if (!eng) return fail(KKW_E_INVALID, "engine is null");
         std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        refuse_while_serving(eng);
         KKW_HIP(hipSetDevice(eng->device));
         eng->adj_index_auto = enable != 0;
         eng->graph_version++;
@@ -1772,7 +1800,9 @@ int kkw_serve_start(kkw_engine *eng, const kkw_serve_opts *opts)
 {
     return guarded([&]() -> int {
         if (!eng) return fail(KKW_E_INVALID, "engine is null");
+        std::lock_guard<std::recursive_mutex> elk(eng->mu);
         if (eng->serve) return fail(KKW_E_INVALID, "serving already started");
+        if (!eng->gs.row_ptr) return fail(KKW_E_NOGRAPH, "no graph loaded");
         auto S = std::make_unique<ServeState>();
         S->eng = eng;
         if (const char *tp = getenv("KKW_SERVE_TRACE")) S->trace_path = tp;
@@ -1809,14 +1839,15 @@ int kkw_submit(kkw_engine *eng, const kkw_job *job, uint64_t *ticket)
 {
     return guarded([&]() -> int {
         if (!eng || !job || !ticket) return fail(KKW_E_INVALID, "null argument");
+        // the engine lock is held until the request is queued, so that
+        // kkw_serve_stop cannot release the serving state in between
+        std::lock_guard<std::recursive_mutex> elk(eng->mu);
         ServeState *S = eng->serve.get();
         if (!S) return fail(KKW_E_INVALID, "serving not started");
         auto r = std::make_shared<Request>();
-        {
-            std::lock_guard<std::recursive_mutex> lk(eng->mu);
-            KKW_HIP(hipSetDevice(eng->device));
-            r->pj = prepare_job(eng, *job);
-        }
+        KKW_HIP(hipSetDevice(eng->device));
+        r->pj = prepare_job(eng, *job);
+        r->graph_V = eng->gs.V;
         // prepare_job keeps host copies; pointers inside src must not be used later
         r->pj.src.starts = r->pj.starts.empty() ? nullptr : r->pj.starts.data();
         r->pj.src.scheme_len = r->pj.scheme_len.empty() ? nullptr : r->pj.scheme_len.data();
@@ -1913,6 +1944,7 @@ int kkw_serve_stop(kkw_engine *eng)
         if (s) (void)hipStreamDestroy(s);
     if (S->dev_base) (void)hipFree(S->dev_base);
     S->pin.reset();   // freed once the last served result using it is freed
+    std::lock_guard<std::recursive_mutex> elk(eng->mu);   // graph changes are allowed again from here
     eng->serve.reset();
     return KKW_OK;
 }

@@ -29,12 +29,16 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_no_gpu_means_no_engine():
-    import torch
-    if torch.cuda.is_available():
-        pytest.skip("a GPU is present")
-    with pytest.raises(kk.KkwError) as e:
-        kk.Engine()
-    assert e.value.code == kk.KKW_E_DEVICE
+    """Without a GPU the engine cannot be created (no CPU fallback).  torch is
+    kept out of this process: its bundled HIP runtime beside libkkw's is what
+    Engine() refuses (test below)."""
+    try:
+        eng = kk.Engine()
+    except kk.KkwError as e:
+        assert e.code == kk.KKW_E_DEVICE
+        return
+    eng.close()
+    pytest.skip("a GPU is present")
 
 
 def test_struct_layouts_match_header(tmp_path):
@@ -58,3 +62,20 @@ def test_struct_layouts_match_header(tmp_path):
         want.append(ctypes.sizeof(py))
         want += [getattr(py, f).offset for f, _ in py._fields_]
     assert got == want
+
+
+def test_two_hip_runtimes_are_refused_before_any_gpu_call():
+    """/proc/self/maps listing torch's bundled HIP runtime beside the system's
+    (libkkw loaded before torch initialised its GPU) makes Engine() and
+    paths_torch refuse, naming both; one runtime mapped twice is fine."""
+    sys_rt = "/opt/rocm-7.2.0/lib/libamdhip64.so.7.2.0"
+    torch_rt = "/usr/local/lib/python3.10/dist-packages/torch/lib/libamdhip64.so"
+    line = "7f0000000000-7f0000100000 r-xp 00000000 08:01 123 {}"
+    one = "\n".join([line.format(sys_rt), line.format(sys_rt), line.format("/usr/lib/libc.so.6")])
+    assert kk.hip_runtimes(one) == [sys_rt]
+    kk.check_one_hip_runtime("test", one)
+    two = one + "\n" + line.format(torch_rt)
+    assert kk.hip_runtimes(two) == sorted([sys_rt, torch_rt])
+    with pytest.raises(RuntimeError) as e:
+        kk.check_one_hip_runtime("Engine()", two)
+    assert sys_rt in str(e.value) and torch_rt in str(e.value)

@@ -86,7 +86,7 @@ def test_ppr_program_uniform_start(engine, build_dir, mode, tables):
 def test_node2vec_program_matches_oracle_and_builtin_kernel(engine, build_dir, p, q, weighted, mode, tables):
     rp, col, w = O.rmat_graph(12, 16, seed=5, weighted=True)
     L = 40
-    up, lo, ip, iq, ov = O.node2vec_constants(p, q)
+    up, lo, ip, iq, ov = kk.node2vec_constants(p, q)
     got, want = both(engine, build_dir, "node2vec", rp, col, w, walker_num=8192, max_length=L,
                      params=[L, up, lo, ip, iq, ov, 1.0 if weighted else 0.0], use_weights=weighted, seed=11,
                      second_order=True, mode=mode, options=tables)

@@ -185,3 +185,34 @@ def test_results_collected_out_of_order_under_pinned_pressure(engine):
     finally:
         eng.serve_stop()
         eng.close()
+
+
+def test_graph_cannot_change_while_serving(served):
+    """Entry points that replace or free device tables refuse with KKW_E_BUSY
+    while the loop serves (its flights read the tables without the engine
+    lock); queries keep their answers, and the calls work again after stop."""
+    eng = served
+    V, E = eng.info()
+    job = kk.node2vec_job(3000, 20, 2.0, 0.5, True, seed=9)
+    (ref,) = eng.run([job])
+    eng.serve_start(max_walkers_in_flight=1 << 16)
+    t = eng.submit(job)
+    changes = {
+        "graph_rmat": lambda: eng.graph_rmat(8, 8, seed=1, weighted=True),
+        "graph_from_edges": lambda: eng.graph_from_edges(4, np.array([0, 1], np.uint32), np.array([1, 2], np.uint32)),
+        "prepare": eng.prepare,
+        "adj_index_off": lambda: eng.adj_index(False),
+        "type_by_dst": lambda: eng.type_by_dst(3, 1),
+        "set_edge_types": lambda: eng.set_edge_types(np.zeros(E, np.uint8)),
+        "sort_adj": eng.sort_adj,
+    }
+    for name, call in changes.items():
+        with pytest.raises(kk.KkwError) as e:
+            call()
+        assert e.value.code == kk.KKW_E_BUSY, name
+    got, _ = eng.wait(t, 60.0)
+    assert same(got, ref)
+    eng.serve_stop()
+    eng.adj_index(False)   # allowed again
+    eng.adj_index(True)
+    assert eng.info() == (V, E)

@@ -145,3 +145,16 @@ def test_node2vec_constants_follow_reference_rounding():
     assert up == 1.0 and lo == 0.5 and ip == 4.0 and ov == 3.0
     up, lo, ip, iq, ov = O.node2vec_constants(3.0, 3.0)
     assert up == 1.0 and lo == np.float32(1 / 3.0) and iq == np.float32(1) / np.float32(3)
+
+
+def test_package_node2vec_constants_equal_the_oracle():
+    """The host constants knightking_amd hands to walk programs are the
+    oracle's, bit for bit, over a grid of p and q (including values whose
+    reciprocals round differently in float and double)."""
+    import knightking_amd as kk
+    vals = [0.1, 0.25, 0.3, 0.5, 0.7, 1.0, 1.5, 2.0, 3.0, 7.0, 1e-3, 123.456]
+    for p in vals:
+        for q in vals:
+            got = np.array(kk.node2vec_constants(p, q), np.float32)
+            want = np.array(O.node2vec_constants(p, q), np.float32)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (p, q, got, want)
