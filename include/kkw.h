@@ -193,6 +193,12 @@ int kkw_graph_adj_index(kkw_engine *eng, int enable);
 int kkw_alias_download(kkw_engine *eng, float *p, uint32_t *primary, uint32_t *alias,
                        uint32_t *meta, float *vsum);
 
+/* The start vertex walker `walker` of a job seeded `seed` takes under
+ * KKW_START_UNIFORM on a graph of v_num vertices (get_uniform_dist_func,
+ * include/walk.hpp:413-421): the device's own draw, evaluated on the host.
+ * Returns 0 when v_num is 0. */
+uint32_t kkw_uniform_start(uint64_t seed, uint32_t walker, uint32_t v_num);
+
 /* Run jobs to completion.  Replaces WalkEngine::random_walk (include/walk.hpp:423-436)
  * as called by the four apps.  out[i] receives job i's paths. */
 int kkw_run(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mode, int out_where,

@@ -36,12 +36,18 @@ struct edge_traits<WeightedMetaData> {
     static const bool weighted = true, typed = true;
     static float weight(const WeightedMetaData &x) { return x.weight; }
     static int meta(const WeightedMetaData &x) { return x.meta_info; }
+    static void set(WeightedMetaData &x, float w, int m)
+    {
+        x.weight = w;
+        x.meta_info = m;
+    }
 };
 template <>
 struct edge_traits<UnweightedMetaData> {
     static const bool weighted = false, typed = true;
     static float weight(const UnweightedMetaData &) { return 1.0f; }
     static int meta(const UnweightedMetaData &x) { return x.meta_info; }
+    static void set(UnweightedMetaData &x, float, int m) { x.meta_info = m; }
 };
 }  // namespace knightking_detail
 

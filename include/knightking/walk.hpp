@@ -21,8 +21,11 @@
 #include <assert.h>
 #include <stdio.h>
 #include <string.h>
+#include <sys/mman.h>
 
 #include <functional>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -31,6 +34,7 @@
 #include "../kkw.h"
 #include "path.hpp"
 #include "type.hpp"
+#include "util.hpp"
 
 template <typename edge_data_t>
 struct AdjUnit {
@@ -50,6 +54,22 @@ template <typename edge_data_t>
 struct AdjList {
     AdjUnit<edge_data_t> *begin;
     AdjUnit<edge_data_t> *end;
+};
+
+// The CSR as KnightKing's graph engine exposes it (include/graph.hpp:77-96):
+// one AdjList {begin, end} per vertex over an array of AdjUnits.
+template <typename edge_data_t>
+struct EdgeContainer {
+    AdjList<edge_data_t> *adj_lists;
+    AdjUnit<edge_data_t> *adj_units;
+    EdgeContainer() : adj_lists(nullptr), adj_units(nullptr) {}
+    EdgeContainer(const EdgeContainer &) = delete;
+    EdgeContainer &operator=(const EdgeContainer &) = delete;
+    ~EdgeContainer()
+    {
+        delete[] adj_lists;
+        delete[] adj_units;
+    }
 };
 
 template <typename walker_data_t>
@@ -88,18 +108,21 @@ inline void check(int rc)
     if (rc != KKW_OK) throw std::runtime_error(std::string("kkw: ") + kkw_last_error());
 }
 
-// Static weight and edge type of an edge record, per edge data type.
+// Static weight and edge type of an edge record, per edge data type, and
+// how the host CSR mirror fills one from the device's weight and type.
 template <typename T>
 struct edge_traits {
     static const bool weighted = false, typed = false;
     static float weight(const T &) { return 1.0f; }
     static int meta(const T &) { return 0; }
+    static void set(T &, float, int) {}
 };
 template <>
 struct edge_traits<real_t> {
     static const bool weighted = true, typed = false;
     static float weight(const real_t &x) { return x; }
     static int meta(const real_t &) { return 0; }
+    static void set(real_t &x, float w, int) { x = w; }
 };
 
 // A device job descriptor attached to a transition config by an app function.
@@ -115,9 +138,13 @@ struct EqualDist {
     vertex_id_t v_num;
     vertex_id_t operator()(walker_id_t w) const { return w % v_num; }
 };
+// get_uniform_dist_func: the start vertex the device draws for walker w in
+// the engine's next random_walk call (seed = engine seed + calls so far).
 struct UniformDist {
     vertex_id_t v_num;
-    vertex_id_t operator()(walker_id_t w) const { return w % v_num; }  // placeholder; drawn on the device
+    const uint64_t *seed;
+    const uint64_t *calls;
+    vertex_id_t operator()(walker_id_t w) const { return kkw_uniform_start(*seed + *calls, w, v_num); }
 };
 
 }  // namespace knightking_detail
@@ -331,11 +358,44 @@ class WalkEngine {
     edge_id_t e_num_ = 0;
     uint64_t seed_ = 1;
     uint64_t calls_ = 0;
+    // host mirror of the device CSR, made on first use of csr / degrees
+    std::unique_ptr<EdgeContainer<edge_data_t> > mirror_;
+    std::vector<vertex_id_t> out_deg_, in_deg_;
+    uint64_t mirror_sig_ = 0;
+    std::recursive_mutex mirror_mu_;
 
 public:
     typedef knightking_detail::edge_traits<edge_data_t> traits;
 
-    WalkEngine()
+    // KnightKing's public graph members (include/graph.hpp:152-156): `csr`
+    // (csr->adj_lists[v].begin / end over AdjUnits) and the degree arrays.
+    // They read a host mirror of the device CSR, downloaded on first use.
+    // The mirror may be edited in place as KnightKing apps do (node2vec sorts
+    // every list through process_vertices): the next walk uploads it when its
+    // contents changed.
+    class CsrHandle {
+        WalkEngine *g_;
+
+    public:
+        explicit CsrHandle(WalkEngine *g) : g_(g) {}
+        EdgeContainer<edge_data_t> *operator->() const { return g_->host_csr(); }
+        EdgeContainer<edge_data_t> &operator*() const { return *g_->host_csr(); }
+        operator EdgeContainer<edge_data_t> *() const { return g_->host_csr(); }
+    };
+    class DegreeHandle {
+        WalkEngine *g_;
+        bool out_;
+
+    public:
+        DegreeHandle(WalkEngine *g, bool out) : g_(g), out_(out) {}
+        vertex_id_t &operator[](size_t v) const { return g_->host_degree(out_)[v]; }
+        operator vertex_id_t *() const { return g_->host_degree(out_); }
+    };
+    CsrHandle csr;
+    DegreeHandle vertex_out_degree;
+    DegreeHandle vertex_in_degree;
+
+    WalkEngine() : csr(this), vertex_out_degree(this, true), vertex_in_degree(this, false)
     {
         kkw_engine_opts o{};
         knightking_detail::check(kkw_engine_create(&o, &eng_));
@@ -349,6 +409,59 @@ public:
     void set_seed(uint64_t s) { seed_ = s; calls_ = 0; }
     vertex_id_t get_vertex_num() { return v_num_; }
     edge_id_t get_edge_num() { return e_num_; }
+    int get_worker_num() { return 1; }
+
+    // One process drives the whole graph: the local partition is every vertex
+    // (include/graph.hpp:164-201).
+    vertex_id_t get_local_vertex_begin() { return 0; }
+    vertex_id_t get_local_vertex_end() { return v_num_; }
+    vertex_id_t get_vertex_begin(partition_id_t p) { return p == 0 ? 0 : v_num_; }
+    vertex_id_t get_vertex_end(partition_id_t) { return v_num_; }
+    bool is_local_vertex(vertex_id_t v) { return v < v_num_; }
+    bool is_valid_edge(Edge<edge_data_t> e) { return e.src < v_num_ && e.dst < v_num_; }
+
+    // Vertex arrays (include/graph.hpp:203-234): zero-filled anonymous
+    // mappings, as KnightKing allocates them.
+    template <typename T>
+    T *alloc_array(size_t num)
+    {
+        void *p = mmap(NULL, sizeof(T) * (num ? num : 1), PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+        if (p == MAP_FAILED) throw std::bad_alloc();
+        return static_cast<T *>(p);
+    }
+    template <typename T>
+    void dealloc_array(T *array, size_t num)
+    {
+        munmap(array, sizeof(T) * (num ? num : 1));
+    }
+    template <typename T>
+    T *alloc_vertex_array()
+    {
+        return alloc_array<T>(v_num_);
+    }
+    template <typename T>
+    void dealloc_vertex_array(T *array)
+    {
+        dealloc_array(array, v_num_);
+    }
+
+    // process_vertices (include/graph.hpp:920-941): `process` on every vertex
+    // on the host, summed.  Closures may read and reorder csr.
+    template <typename reducer_data_t>
+    reducer_data_t process_vertices(std::function<reducer_data_t(vertex_id_t)> process)
+    {
+        (void)host_csr();
+        reducer_data_t reducer = 0;
+        for (vertex_id_t v = 0; v < v_num_; v++) reducer += process(v);
+        return reducer;
+    }
+
+    // The generator user closures draw from on the host (walk.hpp:398-401).
+    StdRandNumGenerator *get_thread_local_rand_gen()
+    {
+        static thread_local StdRandNumGenerator gen;
+        return &gen;
+    }
 
     // GraphEngine::load_graph (include/graph.hpp:408-530), storage.hpp:35-157 formats.
     void load_graph(vertex_id_t v_num, const char *graph_path, bool load_as_undirected = false,
@@ -366,6 +479,7 @@ public:
             read_text(f, src, dst, w);
         }
         fclose(f);
+        drop_mirror();
         knightking_detail::check(kkw_graph_from_arrays(
             eng_, v_num, src.data(), dst.data(), traits::weighted ? w.data() : nullptr,
             traits::typed ? et.data() : nullptr, src.size(), load_as_undirected ? 1 : 0, 0));
@@ -376,7 +490,12 @@ public:
         e_num_ = e;
     }
 
-    void sort_adjacency() { knightking_detail::check(kkw_graph_sort_adj(eng_)); }
+    void sort_adjacency()
+    {
+        sync_mirror();
+        knightking_detail::check(kkw_graph_sort_adj(eng_));
+        drop_mirror();
+    }
 
     std::function<vertex_id_t(walker_id_t)> get_equal_dist_func()
     {
@@ -384,7 +503,7 @@ public:
     }
     std::function<vertex_id_t(walker_id_t)> get_uniform_dist_func()
     {
-        return knightking_detail::UniformDist{v_num_};
+        return knightking_detail::UniformDist{v_num_, &seed_, &calls_};
     }
 
     // WalkEngine::random_walk (include/walk.hpp:423-436, epochs 698-878).
@@ -394,6 +513,7 @@ public:
     {
         WalkConfig local;
         WalkConfig *wc = walk_config_param ? walk_config_param : &local;
+        sync_mirror();
         knightking_detail::DeviceDescriptor &dd = transition_config->device;
         if (!dd.present)
             throw std::logic_error("random_walk: this transition config has no device descriptor; "
@@ -468,6 +588,7 @@ public:
         WalkConfig local;
         WalkConfig *wc = walk_config_param ? walk_config_param : &local;
         if (program->max_length == 0) throw std::invalid_argument("WalkProgram: max_length must be set");
+        sync_mirror();
         if (wc->device_consumer_flag)
             throw std::invalid_argument("WalkProgram: paths come back to the host; no device consumer");
         kkw_program *prog = nullptr;
@@ -530,6 +651,105 @@ public:
     }
 
 private:
+    void drop_mirror()
+    {
+        std::lock_guard<std::recursive_mutex> lk(mirror_mu_);
+        mirror_.reset();
+        out_deg_.clear();
+        in_deg_.clear();
+    }
+    EdgeContainer<edge_data_t> *host_csr()
+    {
+        std::lock_guard<std::recursive_mutex> lk(mirror_mu_);
+        if (!mirror_) download_mirror();
+        return mirror_.get();
+    }
+    vertex_id_t *host_degree(bool out)
+    {
+        std::lock_guard<std::recursive_mutex> lk(mirror_mu_);
+        if (!mirror_) download_mirror();
+        return out ? out_deg_.data() : in_deg_.data();
+    }
+    void download_mirror()
+    {
+        if (v_num_ == 0) throw std::logic_error("no graph loaded");
+        std::vector<uint64_t> rp((size_t)v_num_ + 1);
+        std::vector<uint32_t> col(e_num_);
+        std::vector<float> w(traits::weighted ? e_num_ : 0);
+        std::vector<uint8_t> et(traits::typed ? e_num_ : 0);
+        knightking_detail::check(kkw_graph_download(eng_, rp.data(), col.data(), traits::weighted ? w.data() : nullptr,
+                                                    traits::typed ? et.data() : nullptr));
+        std::unique_ptr<EdgeContainer<edge_data_t> > ec(new EdgeContainer<edge_data_t>());
+        ec->adj_units = new AdjUnit<edge_data_t>[e_num_ ? e_num_ : 1];
+        ec->adj_lists = new AdjList<edge_data_t>[v_num_];
+        for (edge_id_t e = 0; e < e_num_; e++) {
+            ec->adj_units[e].neighbour = col[e];
+            traits::set(unit_data(ec->adj_units[e]), traits::weighted ? w[e] : 1.0f, traits::typed ? et[e] : 0);
+        }
+        out_deg_.assign(v_num_, 0);
+        in_deg_.assign(v_num_, 0);
+        for (vertex_id_t v = 0; v < v_num_; v++) {
+            ec->adj_lists[v].begin = ec->adj_units + rp[v];
+            ec->adj_lists[v].end = ec->adj_units + rp[v + 1];
+            out_deg_[v] = (vertex_id_t)(rp[v + 1] - rp[v]);
+        }
+        for (edge_id_t e = 0; e < e_num_; e++) in_deg_[col[e]]++;
+        mirror_ = std::move(ec);
+        mirror_sig_ = signature();
+    }
+    template <typename U>
+    static auto unit_data(AdjUnit<U> &u) -> decltype((u.data)) { return u.data; }
+    // FNV-1a over every unit's neighbour, weight and type, in CSR order
+    uint64_t signature() const
+    {
+        uint64_t h = 1469598103934665603ull;
+        auto mix = [&h](uint32_t x) {
+            for (int b = 0; b < 4; b++) {
+                h ^= (x >> (8 * b)) & 0xffu;
+                h *= 1099511628211ull;
+            }
+        };
+        for (edge_id_t e = 0; e < e_num_; e++) {
+            AdjUnit<edge_data_t> &u = mirror_->adj_units[e];
+            mix(u.neighbour);
+            if (traits::weighted) {
+                float x = traits::weight(unit_data(u));
+                uint32_t bits;
+                memcpy(&bits, &x, 4);
+                mix(bits);
+            }
+            if (traits::typed) mix((uint32_t)traits::meta(unit_data(u)));
+        }
+        return h;
+    }
+    // A mirror edited on the host goes back to the device before the next walk.
+    void sync_mirror()
+    {
+        std::lock_guard<std::recursive_mutex> lk(mirror_mu_);
+        if (!mirror_) return;
+        const uint64_t sig = signature();
+        if (sig == mirror_sig_) return;
+        std::vector<uint64_t> rp((size_t)v_num_ + 1, 0);
+        std::vector<uint32_t> col(e_num_);
+        std::vector<float> w(traits::weighted ? e_num_ : 0);
+        std::vector<uint8_t> et(traits::typed ? e_num_ : 0);
+        edge_id_t o = 0;
+        for (vertex_id_t v = 0; v < v_num_; v++) {
+            for (AdjUnit<edge_data_t> *p = mirror_->adj_lists[v].begin; p < mirror_->adj_lists[v].end; p++, o++) {
+                if (o >= e_num_) throw std::logic_error("csr mirror: lists hold more units than the graph");
+                col[o] = p->neighbour;
+                if (traits::weighted) w[o] = traits::weight(unit_data(*p));
+                if (traits::typed) et[o] = (uint8_t)traits::meta(unit_data(*p));
+            }
+            rp[v + 1] = o;
+        }
+        if (o != e_num_) throw std::logic_error("csr mirror: lists hold fewer units than the graph");
+        knightking_detail::check(kkw_graph_from_csr(eng_, v_num_, rp.data(), col.data(),
+                                                    traits::weighted ? w.data() : nullptr));
+        if (traits::typed) knightking_detail::check(kkw_graph_set_edge_types(eng_, et.data()));
+        mirror_sig_ = sig;
+    }
+
     template <typename T = edge_data_t>
     void push(const Edge<T> &e, std::vector<uint32_t> &src, std::vector<uint32_t> &dst, std::vector<float> &w,
               std::vector<uint8_t> &et)
@@ -547,9 +767,10 @@ private:
     }
     void read_text(FILE *f, std::vector<uint32_t> &src, std::vector<uint32_t> &dst, std::vector<float> &w)
     {
-        // read_edge_txt (include/storage.hpp:110-122): "%u %u" or "%u %u %f"
-        static_assert(std::is_same<edge_data_t, EmptyData>::value || std::is_same<edge_data_t, real_t>::value,
-                      "text edge lists hold unweighted or real-weighted edges only");
+        // read_edge_txt (include/storage.hpp:106-121): "%u %u" or "%u %u %f";
+        // other edge types cannot be read from text (a run-time error there too)
+        if (!std::is_same<edge_data_t, EmptyData>::value && !std::is_same<edge_data_t, real_t>::value)
+            throw std::invalid_argument("load_graph: this edge type does not support reading from text");
         unsigned a, b;
         float x;
         for (;;) {

@@ -90,7 +90,8 @@ EXPORTED = [
     "kkw_last_error", "kkw_abi_version", "kkw_engine_create", "kkw_engine_destroy",
     "kkw_graph_from_edges", "kkw_graph_from_arrays", "kkw_graph_sort_adj", "kkw_graph_load_file", "kkw_graph_from_csr", "kkw_graph_rmat",
     "kkw_graph_set_edge_types", "kkw_graph_type_by_dst", "kkw_graph_info", "kkw_graph_download",
-    "kkw_prepare", "kkw_graph_adj_index", "kkw_alias_download", "kkw_run", "kkw_run_noalloc", "kkw_paths_free",
+    "kkw_prepare", "kkw_graph_adj_index", "kkw_alias_download", "kkw_uniform_start", "kkw_run", "kkw_run_noalloc",
+    "kkw_paths_free",
     "kkw_paths_dump",
     "kkw_last_timing", "kkw_batch_create", "kkw_batch_set_shape", "kkw_batch_launch",
     "kkw_batch_sync", "kkw_batch_paths", "kkw_batch_destroy", "kkw_autotune", "kkw_diag_stream_copy",
@@ -163,6 +164,7 @@ def load_library(path: str = LIB_PATH):
         "kkw_alias_download": (I32, [P, P, P, P, P, P]),
         "kkw_run": (I32, [P, C.POINTER(_Job), I32, I32, I32, I32, C.POINTER(_Paths)]),
         "kkw_run_noalloc": (I32, [P, C.POINTER(_Job), I32, I32, P, P, P]),
+        "kkw_uniform_start": (U32, [U64, U32, U32]),
         "kkw_paths_free": (I32, [C.POINTER(_Paths)]),
         "kkw_paths_dump": (I32, [C.POINTER(_Paths), C.c_char_p, C.c_char_p, I32]),
         "kkw_last_timing": (I32, [P, C.POINTER(_Timing)]),
@@ -250,6 +252,12 @@ class Job:
             j.scheme_len = lens.ctypes.data_as(C.POINTER(C.c_uint32))
             j.scheme_mask = masks.ctypes.data_as(C.POINTER(C.c_uint8))
         return j
+
+
+def uniform_start(seed: int, walker: int, v_num: int) -> int:
+    """Start vertex of `walker` under START_UNIFORM for a job seeded `seed`
+    (kkw_uniform_start: the kernels' draw, evaluated on the host)."""
+    return int(load_library().kkw_uniform_start(seed, walker, v_num))
 
 
 def node2vec_constants(p: float, q: float) -> List[float]:

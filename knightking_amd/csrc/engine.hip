@@ -1599,6 +1599,13 @@ int kkw_run(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mode, int out_
     });
 }
 
+uint32_t kkw_uniform_start(uint64_t seed, uint32_t walker, uint32_t v_num)
+{
+    if (v_num == 0) return 0;
+    const u32x4 r = draw(seed, walker, 0, 0, TAG_INIT);   // the kernels' start_vertex()
+    return bounded(r.z, r.w, v_num);
+}
+
 int kkw_run_noalloc(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mode, uint32_t *const *paths,
                     const uint32_t *strides, uint32_t *const *lengths)
 {

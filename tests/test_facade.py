@@ -71,3 +71,24 @@ def test_facade_matches_oracle(tmp_path, karate):
     assert [x[0] for x in got] == list(range(68))
     for i, (wid, n, verts) in enumerate(got):
         assert n == la[i] and verts == pa[i, :la[i]].tolist()
+    # metapath on binary Edge<WeightedMetaData> records, type (src + 2 dst) % 3
+    et = ((np.repeat(np.arange(34), np.diff(rp2.astype(np.int64))) + 2 * col2.astype(np.int64)) % 3).astype(np.uint8)
+    gm = O.Graph(rp2, col2, w2, et)
+    masks = [[0b101, 0b110], [0b011], [0b100, 0b001, 0b010]]   # get_scheme_mask of the schemes written
+    paths, lens, _, _ = O.walk(gm, O.METAPATH, 68, 12, True, seed=1, schemes=masks)
+    got = read_dump(prefix + "_meta.0", True)
+    assert [x[0] for x in got] == list(range(68))
+    for i, (wid, n, verts) in enumerate(got):
+        assert n == lens[i] and verts == paths[i, :lens[i]].tolist(), f"metapath walker {i}"
+    assert (tmp_path / "out_schemes.txt").read_text().split() == \
+        "3 3 2 1 0 1 0 1 1 1 1 1 0 3 0 0 1 1 0 0 0 1 0".split()
+    vm = np.loadtxt(prefix + "_meta_vmask.txt", dtype=np.int64)
+    assert np.array_equal(vm[:, 0], O.vmask_of(rp2, et))
+    assert np.array_equal(vm[:, 1], np.diff(rp2.astype(np.int64)))
+    assert np.array_equal(vm[:, 2], np.bincount(col2, minlength=34))
+    # lists sorted on the host through csr, then walked on the device
+    rp3, col3, w3 = O.csr_from_edges(34, src, dst, w, False, True)
+    paths, lens, _, _ = O.walk(O.Graph(rp3, col3, w3), O.DEEPWALK, 34, 12, True, seed=1)
+    got = read_dump(prefix + "_sorted_dw.0", True)
+    for i, (wid, n, verts) in enumerate(got):
+        assert n == lens[i] and verts == paths[i, :lens[i]].tolist(), f"sorted deepwalk walker {i}"

@@ -158,3 +158,16 @@ def test_package_node2vec_constants_equal_the_oracle():
             got = np.array(kk.node2vec_constants(p, q), np.float32)
             want = np.array(O.node2vec_constants(p, q), np.float32)
             assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (p, q, got, want)
+
+
+def test_uniform_start_is_the_kernels_draw():
+    """kkw_uniform_start (get_uniform_dist_func on the host) is the start draw
+    of the kernels: Philox at (walker, 0, 0, INIT), words z, w mapped to
+    [0, v_num) -- checked against the oracle's Philox and bounded map."""
+    import knightking_amd as kk
+    for seed in (1, 7, (3 << 40) + 5):
+        for v_num in (1, 34, 1000, 1 << 24, 0xFFFFFFFF):
+            for wid in (0, 1, 33, 12345, 0xFFFFFFFE):
+                r = O.philox([wid, 0, 0, 2], [seed & 0xFFFFFFFF, seed >> 32])
+                assert kk.uniform_start(seed, wid, v_num) == O.bounded(int(r[2]), int(r[3]), v_num)
+    assert kk.uniform_start(1, 5, 0) == 0
