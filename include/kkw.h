@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define KKW_ABI_VERSION 2   /* 2: kkw_timing.prepare_ms */
+#define KKW_ABI_VERSION 3   /* 2: kkw_timing.prepare_ms; 3: kkw_timing.envelope_violations */
 
 enum kkw_status {
     KKW_OK = 0,
@@ -121,6 +121,12 @@ typedef struct kkw_timing {
     double prepare_ms;       /* last kkw_prepare, host wall time: the alias tables and
                                 the neighbour hash sets build concurrently, so this is
                                 less than alias_ms + index_ms                        */
+    /* last kkw_program_run: trials whose dynamic component left the rejection
+     * envelope (the check of include/walk.hpp:1003-1005): an outlier candidate
+     * above overflow / static + upper + 1e-6, or, for a program without an
+     * outlier pair, a static draw above the upper bound.  0 for a correct
+     * program; anything else means biased paths. */
+    uint64_t envelope_violations;
 } kkw_timing;
 
 enum kkw_flags { KKW_FLAG_STATS = 1 };

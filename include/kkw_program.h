@@ -38,6 +38,31 @@
  *   KKW_FN int64_t outlier_search(const WalkerState &s, const kkw_walker &w, uint32_t cur,
  *                                 uint32_t idx, const kkw_ctx &c);  // edge or -1
  *
+ * A second-order transition (SecondOrderTransitionConfig, include/walk.hpp:
+ * 273-363, the query / response supersteps of walk.hpp:1082-1381) defines,
+ * after `#define KKW_SECOND_ORDER 1` (with KKW_DYNAMIC), its query and
+ * response data and the three functions that pass them:
+ *
+ *   struct Query { ... };                // stateQuery<query_data_t>::data
+ *   struct Response { ... };             // stateResponse<response_data_t>::data
+ *   KKW_FN bool post_query(const WalkerState &s, const kkw_walker &w, uint32_t cur,
+ *                          const kkw_edge &e, uint32_t &target, Query &q,
+ *                          const kkw_ctx &c);    // post_query_func: true = emit(target, q)
+ *   KKW_FN void respond_query(uint32_t v, const Query &q, Response &r,
+ *                             const kkw_ctx &c);  // respond_query_func at vertex v
+ *   KKW_FN float dynamic_comp(const WalkerState &s, const kkw_walker &w, const Response &r,
+ *                             uint32_t cur, const kkw_edge &e, const kkw_ctx &c);
+ *
+ * in place of the first-order dynamic_comp.  A trial the lower bound does
+ * not accept posts its query, the target vertex answers it, and the dart is
+ * compared with dynamic_comp given the answer (walk.hpp:1226-1257,
+ * 1336-1364).  On one GPU the query and the answer are function calls: the
+ * target's data is read directly (kkw_ctx holds the whole graph).  A trial
+ * whose post_query sends nothing sees the walker's previous response (value-
+ * initialised when the walker starts), as the reference's per-walker
+ * response cache holds it.  The extension is re-tested on every retry
+ * (walk.hpp:1161-1173), whatever kkw_prog_job.second_order says.
+ *
  * `e` is the candidate edge, KnightKing's AdjUnit: e.neighbour the vertex
  * it leads to, e.index its global CSR position -- its weight is
  * c.weight[e.index] (when the graph has weights), its type c.etype[e.index],

@@ -76,7 +76,7 @@ class _Timing(C.Structure):
         ("graph_ms", C.c_double), ("index_ms", C.c_double), ("steps", C.c_uint64), ("trials", C.c_uint64),
         ("searches", C.c_uint64), ("search_probes", C.c_uint64), ("row_loads", C.c_uint64),
         ("search_lines", C.c_uint64), ("ref_queries", C.c_uint64), ("ref_query_lines", C.c_uint64),
-        ("prepare_ms", C.c_double),
+        ("prepare_ms", C.c_double), ("envelope_violations", C.c_uint64),
     ]
 
 

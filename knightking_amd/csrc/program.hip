@@ -233,7 +233,8 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
             out->priv = (void *)1;
             return KKW_OK;
         }
-        DevBuf paths((size_t)n * stride * 4), lens((size_t)n * 4), queue(sizeof(unsigned long long));
+        // queue word, then the envelope-violation counter
+        DevBuf paths((size_t)n * stride * 4), lens((size_t)n * 4), queue(2 * sizeof(unsigned long long));
         DevBuf starts(job->start_mode == KKW_START_LIST ? (size_t)job->n_starts * 4 : 0);
         if (job->start_mode == KKW_START_LIST)
             KKW_HIP(hipMemcpyAsync(starts.p, job->starts, (size_t)job->n_starts * 4, hipMemcpyHostToDevice,
@@ -273,12 +274,13 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
         A.chunk = chunk;
         A.n_chunks = ((uint64_t)n + chunk - 1) / chunk;
         A.queue = (unsigned long long *)queue.p;
+        A.violations = (unsigned long long *)queue.p + 1;
         A.line_rows = line_rows ? 1u : 0u;
         // persistent: 8 blocks of 256 per CU, fewer when there are fewer work items
         const uint64_t blocks_needed = (A.n_chunks * 64 + KKW_PROG_BLOCK - 1) / KKW_PROG_BLOCK;
         const uint32_t grid = persistent ? (uint32_t)std::min<uint64_t>((uint64_t)eng->cu_count * 8, blocks_needed)
                                          : (n + KKW_PROG_BLOCK - 1) / KKW_PROG_BLOCK;
-        if (persistent) KKW_HIP(hipMemsetAsync(queue.p, 0, sizeof(unsigned long long), eng->stream));
+        KKW_HIP(hipMemsetAsync(queue.p, 0, 2 * sizeof(unsigned long long), eng->stream));
         void *args[] = {&A};
         hipEvent_t e0, e1;
         KKW_HIP(hipEventCreate(&e0));
@@ -311,9 +313,12 @@ int kkw_program_run(kkw_program *prog, const kkw_prog_job *job, kkw_paths *out)
             steps += (out->lengths[i] & 0x7fffffffu) - 1;
             trunc += out->lengths[i] >> 31;
         }
+        unsigned long long viol = 0;
+        KKW_HIP(hipMemcpy(&viol, A.violations, sizeof(viol), hipMemcpyDeviceToHost));
         out->steps = steps;
         out->truncated = trunc;
         eng->timing.steps = steps;
+        eng->timing.envelope_violations = viol;
         eng->timing.total_ms = now_ms() - t0;
         return KKW_OK;
     });

@@ -51,4 +51,5 @@ struct kkw_prog_args {
     uint64_t n_chunks;           // persistent form: work items
     uint32_t line_rows;          // persistent form: rows of whole lines, written from LDS
     uint32_t packed_rows;        // KKW_EDGE_INDEX 0: buckets carry rows, beg | deg << 40
+    unsigned long long *violations;   // trials whose dynamic component left its envelope (zero at launch)
 };

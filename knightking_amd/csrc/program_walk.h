@@ -1,7 +1,9 @@
 // program_walk.h -- the walk loop of a walk program, compiled at run time
 // (hipRTC) after the program's own functions (include/kkw_program.h).
 //
-// One walker per lane, the first-order epoch of include/walk.hpp:954-1064:
+// One walker per lane, the first-order epoch of include/walk.hpp:954-1064
+// (and, for a program with a query / response pair, the second-order one of
+// walk.hpp:1135-1379 with the query answered in place):
 // per step the extension test (once, or on every retry for a second-order
 // program, walk.hpp:1161-1173), then trials until one is accepted -- the
 // outlier appendix first when the program has one (981-1014), else a static
@@ -28,6 +30,12 @@
 #endif
 #ifndef KKW_EDGE_INDEX
 #define KKW_EDGE_INDEX 1
+#endif
+#ifndef KKW_SECOND_ORDER
+#define KKW_SECOND_ORDER 0
+#endif
+#if KKW_SECOND_ORDER && !KKW_DYNAMIC
+#error "a second-order transition needs a dynamic component (walk.hpp:339-351)"
 #endif
 #if KKW_OUTLIER && !KKW_DYNAMIC
 #error "an outlier appendix needs a dynamic component (walk.hpp:251-261)"
@@ -77,6 +85,31 @@ __device__ __forceinline__ void kkw_prog_keep_row(kkw_ctx &c, uint32_t v, uint64
     c.row_deg = deg;
 }
 
+// The walker's last response (a second-order program) and the dynamic
+// component of a candidate: post the query, have the target answer it, and
+// evaluate dynamic_comp with the answer (walk.hpp:1226-1257, 1336-1364).
+#if KKW_SECOND_ORDER
+typedef Response kkw_prog_resp;
+#else
+struct kkw_prog_resp {
+};
+#endif
+#if KKW_DYNAMIC
+__device__ __forceinline__ float kkw_prog_dynamic(const WalkerState &s, const kkw_walker &w, uint32_t cur,
+                                                  const kkw_edge &e, kkw_prog_resp &resp, const kkw_ctx &c)
+{
+#if KKW_SECOND_ORDER
+    uint32_t target = 0;
+    Query q;
+    if (post_query(s, w, cur, e, target, q, c)) respond_query(target, q, resp, c);
+    return dynamic_comp(s, w, resp, cur, e, c);
+#else
+    (void)resp;
+    return dynamic_comp(s, w, cur, e, c);
+#endif
+}
+#endif
+
 // four path entries held in registers, stored as one aligned 16-byte group
 struct kkw_prog_group {
     uint32_t v[4];
@@ -109,10 +142,18 @@ __device__ __forceinline__ bool kkw_prog_terminates(const WalkerState &s, const 
 
 // One trial at cur; returns true with the accepted edge in `acc` and, when
 // the bucket carried it, the new vertex's packed row in `nrow` (nrow_ok).
+// env counts envelope violations (include/walk.hpp:1003-1005): an outlier
+// candidate whose dynamic component exceeds overflow / static + upper (+1e-6,
+// the reference's tolerance), or -- for a program without an outlier pair,
+// where nothing covers the excess -- a static draw whose dynamic component
+// exceeds the upper bound.  Either biases the walk; the count is reported.
 __device__ __forceinline__ bool kkw_prog_trial(const WalkerState &s, const kkw_walker &w, uint32_t cur,
                                                uint64_t beg, uint32_t deg, uint32_t trial, const kkw_prog_args &A,
-                                               const kkw_ctx &c, kkw_edge &acc, uint64_t &nrow, bool &nrow_ok)
+                                               const kkw_ctx &c, kkw_prog_resp &resp, kkw_edge &acc,
+                                               uint64_t &nrow, bool &nrow_ok, uint32_t &env)
 {
+    (void)resp;
+    (void)env;
     nrow_ok = false;
     nrow = 0;
 #if KKW_DYNAMIC
@@ -139,7 +180,9 @@ __device__ __forceinline__ bool kkw_prog_trial(const WalkerState &s, const kkw_w
                 e.reserved = 0;
                 const float sc = A.vsum ? c.weight[ei] : 1.0f;   // vsum is set exactly for weighted draws
                 const float h = (rv - (float)idx * ov) / sc + up;
-                if (!(h <= dynamic_comp(s, w, cur, e, c))) return false;
+                const float pd = kkw_prog_dynamic(s, w, cur, e, resp, c);
+                env += ov / sc + up + 1e-6f < pd ? 1u : 0u;
+                if (!(h <= pd)) return false;
                 acc = e;
                 return true;
             }
@@ -191,7 +234,11 @@ __device__ __forceinline__ bool kkw_prog_trial(const WalkerState &s, const kkw_w
 #if KKW_LOWER
     ok = dart <= lower_bound(cur, c);
 #endif
-    if (!ok) ok = dart <= dynamic_comp(s, w, cur, e, c);
+    if (!ok) {
+        const float pd = kkw_prog_dynamic(s, w, cur, e, resp, c);
+        env += !KKW_OUTLIER && pd > up ? 1u : 0u;
+        ok = dart <= pd;
+    }
     if (!ok) return false;
 #endif
     acc = e;
@@ -223,6 +270,8 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk(kk
     uint32_t cur = kkw_prog_start(A, w.id);
     WalkerState s;
     init_state(s, w, cur, c);
+    kkw_prog_resp resp = kkw_prog_resp();
+    const bool retest = A.retest_ext || KKW_SECOND_ORDER;
     uint32_t *path = A.paths + (uint64_t)i * A.stride;   // stride is a multiple of 4
     kkw_prog_group grp;
     grp.v[0] = grp.v[1] = grp.v[2] = grp.v[3] = cur;
@@ -230,10 +279,10 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk(kk
     uint32_t deg;
     kkw_prog_row(c.row_ptr, cur, beg, deg);
     bool capped = false;
-    uint32_t trial = 0;
+    uint32_t trial = 0, env = 0;
     for (;;) {
-        if (trial == 0 || A.retest_ext) {
-            if (kkw_prog_terminates(s, w, cur, deg, A.retest_ext ? trial : 0u, A, c)) break;
+        if (trial == 0 || retest) {
+            if (kkw_prog_terminates(s, w, cur, deg, retest ? trial : 0u, A, c)) break;
             if (w.step >= A.max_len) {
                 capped = true;
                 break;
@@ -246,7 +295,7 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk(kk
         kkw_edge e;
         uint64_t nrow;
         bool nrow_ok;
-        if (!kkw_prog_trial(s, w, cur, beg, deg, trial, A, c, e, nrow, nrow_ok)) {
+        if (!kkw_prog_trial(s, w, cur, beg, deg, trial, A, c, resp, e, nrow, nrow_ok, env)) {
             trial++;
             continue;
         }
@@ -261,6 +310,7 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk(kk
     }
     if ((w.step & 3u) != 3u) grp.store(path, w.step);   // the last, partial group
     A.lens[i] = (w.step + 1) | (capped ? 0x80000000u : 0u);
+    if (env) atomicAdd(A.violations, (unsigned long long)env);
 }
 
 // Path write-out of the persistent form when rows are whole 128-byte lines
@@ -293,13 +343,15 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk_pe
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     bool active = false, drained = false;
     uint32_t pool_next = 0, pool_end = 0;
-    uint32_t i = 0, cur = 0, deg = 0, trial = 0;
+    uint32_t i = 0, cur = 0, deg = 0, trial = 0, env = 0;
     uint64_t beg = 0;
     uint32_t *path = nullptr;
     kkw_walker w;
     w.id = 0;
     w.step = 0;
     WalkerState s;
+    kkw_prog_resp resp = kkw_prog_resp();
+    const bool retest = A.retest_ext || KKW_SECOND_ORDER;
     kkw_prog_group grp;
     uint32_t *const lbuf = kkw_prog_lds + threadIdx.x * (KKW_PROG_HALF_LINE + 1);
     const bool lines = A.line_rows != 0;
@@ -327,6 +379,7 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk_pe
                 cur = kkw_prog_start(A, w.id);
                 kkw_prog_keep_row(c, KKW_NO_VERTEX, 0, 0);
                 init_state(s, w, cur, c);
+                resp = kkw_prog_resp();
                 path = A.paths + (uint64_t)i * A.stride;
                 if (lines) lbuf[0] = cur;
                 else grp.v[0] = grp.v[1] = grp.v[2] = grp.v[3] = cur;
@@ -341,8 +394,8 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk_pe
         if (__ballot(active) == 0ull) break;
         if (active) {
             bool done = false, capped = false;
-            if (trial == 0 || A.retest_ext) {
-                if (kkw_prog_terminates(s, w, cur, deg, A.retest_ext ? trial : 0u, A, c)) {
+            if (trial == 0 || retest) {
+                if (kkw_prog_terminates(s, w, cur, deg, retest ? trial : 0u, A, c)) {
                     done = true;
                 } else if (w.step >= A.max_len) {
                     done = capped = true;
@@ -353,7 +406,7 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk_pe
                 kkw_edge e;
                 uint64_t nrow;
                 bool nrow_ok;
-                if (kkw_prog_trial(s, w, cur, beg, deg, trial, A, c, e, nrow, nrow_ok)) {
+                if (kkw_prog_trial(s, w, cur, beg, deg, trial, A, c, resp, e, nrow, nrow_ok, env)) {
                     w.step++;
                     update_state(s, w, cur, e, c);
                     kkw_prog_keep_row(c, cur, beg, deg);
@@ -383,4 +436,5 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk_pe
             }
         }
     }
+    if (env) atomicAdd(A.violations, (unsigned long long)env);
 }

@@ -5,7 +5,9 @@
  * tests together with one program's source (g++ -DKKW_PROGRAM_SOURCE=...,
  * oracle.program_oracle), never by the product library.
  *
- * Restates the first-order epoch of include/walk.hpp:954-1064 (per walker):
+ * Restates the first-order epoch of include/walk.hpp:954-1064 (per walker),
+ * and for a program with a query / response pair the second-order one of
+ * walk.hpp:1135-1379 (query posted, answered at its target, verdict):
  *   - extension_comp once per step (954-973), or on every retry for a
  *     second-order program (walk.hpp:1161-1173); stop when it is 0, or when it
  *     is below 1 and the extension coin exceeds it;
@@ -50,8 +52,37 @@ extern "C" {
 #ifndef KKW_OUTLIER
 #define KKW_OUTLIER 0
 #endif
+#ifndef KKW_SECOND_ORDER
+#define KKW_SECOND_ORDER 0
+#endif
 
 namespace {
+
+/* The walker's last response; a second-order program's query is posted to
+ * its target and answered there before dynamic_comp sees the answer
+ * (walk.hpp:1226-1257 post, 1285-1294 respond, 1336-1364 verdict). */
+#if KKW_SECOND_ORDER
+typedef Response Resp;
+#else
+struct Resp {
+};
+#endif
+#if KKW_DYNAMIC
+float dynamic_of(const WalkerState &s, const kkw_walker &w, uint32_t cur, const kkw_edge &e, Resp &resp,
+                 const kkw_ctx &c)
+{
+#if KKW_SECOND_ORDER
+    uint32_t target = 0;
+    Query q;
+    const bool sent = post_query(s, w, cur, e, target, q, c);
+    if (sent) respond_query(target, q, resp, c);
+    return dynamic_comp(s, w, resp, cur, e, c);
+#else
+    (void)resp;
+    return dynamic_comp(s, w, cur, e, c);
+#endif
+}
+#endif
 
 const uint32_t kMaxTrials = 1u << 20;
 
@@ -90,10 +121,11 @@ bool terminates(const WalkerState &s, const kkw_walker &w, uint32_t cur, uint32_
 }
 
 bool trial_at(const WalkerState &s, const kkw_walker &w, uint32_t cur, uint64_t beg, uint32_t deg,
-              uint32_t trial, const Job &J, kkw_edge &acc)
+              uint32_t trial, const Job &J, Resp &resp, kkw_edge &acc)
 {
     const kkw_ctx &c = J.c;
     (void)s;
+    (void)resp;
 #if KKW_DYNAMIC
     const float up = upper_bound(cur, c);
 #endif
@@ -116,7 +148,7 @@ bool trial_at(const WalkerState &s, const kkw_walker &w, uint32_t cur, uint64_t 
             e.reserved = 0;
             const float sc = J.bp ? c.weight[ei] : 1.0f;
             const float h = (rv - (float)idx * ov) / sc + up;
-            if (!(h <= dynamic_comp(s, w, cur, e, c))) return false;
+            if (!(h <= dynamic_of(s, w, cur, e, resp, c))) return false;
             acc = e;
             return true;
         }
@@ -137,7 +169,7 @@ bool trial_at(const WalkerState &s, const kkw_walker &w, uint32_t cur, uint64_t 
 #if KKW_LOWER
     ok = dart <= lower_bound(cur, c);
 #endif
-    if (!ok) ok = dart <= dynamic_comp(s, w, cur, e, c);
+    if (!ok) ok = dart <= dynamic_of(s, w, cur, e, resp, c);
     if (!ok) return false;
 #endif
     acc = e;
@@ -194,6 +226,8 @@ extern "C" int orc_program_walk(uint32_t V, const uint64_t *row_ptr, const uint3
         }
         WalkerState s;
         init_state(s, w, cur, J.c);
+        Resp resp = Resp();
+        const int retest = retest_ext || KKW_SECOND_ORDER;
         uint32_t *path = paths + (uint64_t)i * stride;
         path[0] = cur;
         int capped = 0;
@@ -201,8 +235,8 @@ extern "C" int orc_program_walk(uint32_t V, const uint64_t *row_ptr, const uint3
         for (;;) {
             const uint64_t beg = row_ptr[cur];
             const uint32_t deg = (uint32_t)(row_ptr[cur + 1] - beg);
-            if (trial == 0 || retest_ext) {
-                if (terminates(s, w, cur, deg, retest_ext ? trial : 0u, J)) break;
+            if (trial == 0 || retest) {
+                if (terminates(s, w, cur, deg, retest ? trial : 0u, J)) break;
                 if (w.step >= max_len) {
                     capped = 1;
                     break;
@@ -213,7 +247,7 @@ extern "C" int orc_program_walk(uint32_t V, const uint64_t *row_ptr, const uint3
                 break;
             }
             kkw_edge e;
-            if (!trial_at(s, w, cur, beg, deg, trial, J, e)) {
+            if (!trial_at(s, w, cur, beg, deg, trial, J, resp, e)) {
                 trial++;
                 continue;
             }

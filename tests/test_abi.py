@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
     lib = kk.load_library()
     missing = [s for s in declared_symbols() if not hasattr(lib, s)]
     assert not missing, missing
-    assert lib.kkw_abi_version() == 2
+    assert lib.kkw_abi_version() == 3
 
 
 def test_no_gpu_means_no_engine():

@@ -95,6 +95,33 @@ def test_node2vec_program_matches_oracle_and_builtin_kernel(engine, build_dir, p
     same_paths(got.vertices, got.lengths, builtin.vertices, builtin.lengths)
 
 
+@pytest.mark.parametrize("tables", TABLES)
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("p,q,weighted", [(2.0, 0.5, True), (0.5, 2.0, True), (0.25, 1.0, False)])
+def test_node2vec_query_program_matches_oracle_and_builtin_kernel(engine, build_dir, p, q, weighted, mode, tables):
+    """node2vec through the second-order query / response functions: the
+    device loop against the oracle's loop and the built-in kernel."""
+    rp, col, w = O.rmat_graph(12, 16, seed=5, weighted=True)
+    L = 40
+    up, lo, ip, iq, ov = kk.node2vec_constants(p, q)
+    got, want = both(engine, build_dir, "node2vec_query", rp, col, w, walker_num=8192, max_length=L,
+                     params=[L, up, lo, ip, iq, ov, 1.0 if weighted else 0.0], use_weights=weighted, seed=11,
+                     mode=mode, options=tables)
+    check(got, want)
+    (builtin,) = engine.run([kk.node2vec_job(8192, L, p, q, weighted, seed=11)], mode=kk.MODE_PLAIN)
+    same_paths(got.vertices, got.lengths, builtin.vertices, builtin.lengths)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("weighted", [True, False])
+def test_second_order_answers_kept_between_queries(engine, build_dir, weighted, mode):
+    """A response carrying data, queries sent on even steps only, the kept
+    answer seen on odd ones: device loop and oracle loop agree bit for bit."""
+    rp, col, w = O.rmat_graph(11, 16, seed=8, weighted=True)
+    check(*both(engine, build_dir, "edge_weight_query", rp, col, w, walker_num=10000, max_length=30, params=[30],
+                use_weights=weighted, seed=21, mode=mode))
+
+
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("weighted", [True, False])
 def test_tag_walk_program(engine, build_dir, weighted, mode):
@@ -201,5 +228,48 @@ def test_program_errors(engine):
         with pytest.raises(kk.KkwError) as e:
             prog.run(10, 5, params=[5], mode=7)
         assert e.value.code == kk.KKW_E_INVALID
+    finally:
+        prog.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_envelope_violations_are_counted(engine, mode):
+    """kkw_timing.envelope_violations (the reference's check at
+    include/walk.hpp:1003-1005): 0 for correct programs -- the return-biased
+    walk (upper bound 2) and node2vec with its outlier appendix (p = 0.5) --
+    and the number of offending trials when the upper bound is set too low
+    (1.5 under a dynamic component of 2 for the return edge)."""
+    rp, col, w = O.rmat_graph(10, 16, seed=6, weighted=True)
+    engine.graph_from_csr(rp, col, w)
+    src = source("return_bias")
+    prog = engine.program(src)
+    try:
+        prog.run(4000, 20, use_weights=True, seed=3, params=[20], mode=mode)
+        assert engine.timing()["envelope_violations"] == 0
+    finally:
+        prog.close()
+    up, lo, ip, iq, ov = kk.node2vec_constants(0.5, 2.0)
+    prog = engine.program(source("node2vec"))
+    try:
+        prog.run(4000, 20, use_weights=True, seed=3, params=[20, up, lo, ip, iq, ov, 1.0], second_order=True,
+                 mode=mode)
+        assert engine.timing()["envelope_violations"] == 0
+    finally:
+        prog.close()
+    bad = src.replace("KKW_FN float upper_bound(uint32_t, const kkw_ctx &) { return 2.0f; }",
+                      "KKW_FN float upper_bound(uint32_t, const kkw_ctx &) { return 1.5f; }")
+    assert bad != src
+    prog = engine.program(bad)
+    try:
+        got = prog.run(4000, 20, use_weights=True, seed=3, params=[20], mode=mode)
+        n = engine.timing()["envelope_violations"]
+        # every accepted or rejected trial of the return edge after step 0 is one
+        rows = got.vertices[:, :21]
+        returns = 0
+        for i in range(4000):
+            n_i = int(got.lengths[i])
+            if n_i >= 3:
+                returns += int(np.sum(rows[i, 2:n_i] == rows[i, :n_i - 2]))
+        assert n >= returns > 0
     finally:
         prog.close()

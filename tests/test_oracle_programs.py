@@ -94,6 +94,42 @@ def test_node2vec_program_is_the_builtin(build_dir, p, q, weighted):
     assert np.array_equal(ta, tb)
 
 
+@pytest.mark.parametrize("p,q,weighted", [(2.0, 0.5, True), (0.5, 2.0, True), (0.25, 1.0, False)])
+def test_node2vec_query_program_is_the_builtin(build_dir, p, q, weighted):
+    """node2vec written with the second-order query / response functions
+    (knightking_amd/programs/node2vec_query.cpp: the query goes to the
+    previous vertex, which answers by adjacency) gives the built-in walk bit
+    for bit, outlier appendix included."""
+    rp, col, w = O.rmat_graph(10, 16, seed=5, weighted=True)
+    L = 40
+    g = O.Graph(rp, col, w)
+    pb, lb, tb, _ = O.walk(g, O.NODE2VEC, 3000, L, weighted, p=p, q=q, seed=11)
+    up, lo, ip, iq, ov = O.node2vec_constants(p, q)
+    pa, la, ta = O.program_walk(plib("node2vec_query", build_dir), rp, col, w,
+                                params=[L, up, lo, ip, iq, ov, 1.0 if weighted else 0.0], use_weights=weighted,
+                                seed=11, walker_num=3000, max_length=L)
+    same_paths(pa, la, pb, lb)
+    assert np.array_equal(ta, tb)
+
+
+def test_query_sent_to_the_wrong_vertex_is_caught(build_dir, tmp_path):
+    """A planted mistake in the query's target (the candidate itself instead
+    of the previous vertex) changes the paths: the comparison above can fail."""
+    src = open(program_path("node2vec_query")).read()
+    assert "target = s.prev;" in src
+    bad = tmp_path / "node2vec_query_bad.cpp"
+    bad.write_text(src.replace("target = s.prev;", "target = e.neighbour;", 1))
+    rp, col, w = O.rmat_graph(10, 16, seed=5, weighted=True)
+    up, lo, ip, iq, ov = O.node2vec_constants(2.0, 0.5)
+    g = O.Graph(rp, col, w)
+    pb, lb, _, _ = O.walk(g, O.NODE2VEC, 3000, 40, True, p=2.0, q=0.5, seed=11)
+    pa, la, _ = O.program_walk(O.program_oracle(str(bad), build_dir), rp, col, w,
+                               params=[40, up, lo, ip, iq, ov, 1.0], use_weights=True, seed=11, walker_num=3000,
+                               max_length=40)
+    mask = np.arange(41)[None, :] < lb[:, None]
+    assert not (np.array_equal(la, lb) and np.all((pa[:, :41] == pb) | ~mask))
+
+
 # ----------------------------------------------------- closed forms ----
 def tagged_graph(V, E, seed, weighted, sort_by_tag=False):
     """Random undirected graph, one random tag per CSR entry (both directions
