@@ -371,7 +371,7 @@ def serve_stream(eng, jobs, gaps):
     return np.array(lat), steps[0], time.perf_counter() - t_start, (failure[0] if failure else None)
 
 
-def serve_session(eng, jobs, gaps):
+def serve_session(eng, jobs, gaps, round_steps=0):
     """One serving session over a query stream, with the loop's flight trace:
     returns the latency / throughput summary and the flights' occupancy."""
     import tempfile
@@ -379,7 +379,7 @@ def serve_session(eng, jobs, gaps):
     os.close(fd)
     os.environ["KKW_SERVE_TRACE"] = trace
     try:
-        eng.serve_start(max_walkers_in_flight=1 << 22)
+        eng.serve_start(max_walkers_in_flight=1 << 22, round_steps=round_steps)
         try:
             lat, steps, span, err = serve_stream(eng, jobs, gaps)
         finally:
@@ -443,7 +443,7 @@ def flight_occupancy(trace_file):
             "mean_queries_per_flight": round(float(rows[:, 6].mean()), 2)}
 
 
-def serve_mixed(eng, rp, rate, seconds, seed=8):
+def serve_mixed(eng, rp, rate, seconds, seed=8, round_steps=0):
     """Mixed queries at `rate`: kind uniform over PPR (t = 0.15, degree-
     proportional source), DeepWalk (L = 80) and node2vec (p = 2, q = 0.5,
     L = 80), all weighted, 1e3..1e5 walkers log-uniform; p50/p99 latency and
@@ -468,8 +468,8 @@ def serve_mixed(eng, rp, rate, seconds, seed=8):
         else:
             jobs.append(kk.node2vec_job(int(walkers[i]), 80, 2.0, 0.5, True, seed=3000 + i, walker_begin=begin))
     out = {"rate_qps": rate, "kinds": "PPR t=0.15 / DeepWalk L=80 / node2vec L=80, weighted",
-           "walkers": "10^(3+2U)"}
-    out.update(serve_session(eng, jobs, gaps))
+           "walkers": "10^(3+2U)", "round_steps": round_steps}
+    out.update(serve_session(eng, jobs, gaps, round_steps))
     log(f"mixed serving {rate} qps: {out}")
     return out
 
@@ -644,7 +644,8 @@ def main():
     if not args.no_serve:
         rp, _, _, _ = eng.download(weights=False)
         serving = serve_latency(eng, rp, [200, 500, 1000, 2000], args.serve_seconds)
-        serving["mixed"] = [serve_mixed(eng, rp, r, args.serve_seconds) for r in (2000, 8000)]
+        serving["mixed"] = [serve_mixed(eng, rp, r, args.serve_seconds, round_steps=k)
+                            for r, k in ((2000, 0), (8000, 0), (8000, 8))]
 
     line = {
         "metric": "walker steps/s, node2vec RMAT-24 1 GPU (batched vs plain); p99 PPR query latency",

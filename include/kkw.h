@@ -311,8 +311,14 @@ int kkw_diag_stream_copy(kkw_engine *eng, uint64_t bytes, int iters, double *ms_
 int kkw_diag_random_gather(kkw_engine *eng, uint64_t bytes, int reads, int iters, double *lines_per_s);
 
 /* Serving: a resident loop admits queries (usually PPR) by free HBM and slot
- * count, batches everything admitted into rounds of the persistent kernel, and
- * retires queries as their last walker finishes. */
+ * count into flights -- one persistent-kernel launch sequence each, up to four
+ * on the device at once -- and retires queries as their last walker finishes.
+ * round_steps = 0: a flight walks to the end in one launch.  round_steps = n:
+ * a flight walks in launches (rounds) of at most n steps per walker; walkers
+ * still going are suspended at a step boundary and continued by the next
+ * launch, so the persistent kernel of a long flight gives the GPU back between
+ * rounds and flights admitted meanwhile run in between.  Paths are the same
+ * either way. */
 typedef struct kkw_serve_opts {
     uint32_t max_walkers_in_flight;  /* slot budget (walkers)                        */
     uint32_t round_steps;            /* walker steps per kernel round (0 = to end)    */

@@ -672,8 +672,10 @@ class Engine:
         return r.value
 
     # ---- serving ---------------------------------------------------------
-    def serve_start(self, max_walkers_in_flight=1 << 22, queue_capacity=1 << 16):
-        o = _ServeOpts(max_walkers_in_flight, 0, queue_capacity, 0)
+    def serve_start(self, max_walkers_in_flight=1 << 22, queue_capacity=1 << 16, round_steps=0):
+        """Start the serving loop; round_steps > 0 walks each flight in kernel
+        launches of at most that many steps per walker (same paths)."""
+        o = _ServeOpts(max_walkers_in_flight, round_steps, queue_capacity, 0)
         _check(_lib.kkw_serve_start(self.h, C.byref(o)))
 
     def submit(self, job: Job) -> int:

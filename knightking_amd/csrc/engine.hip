@@ -580,6 +580,17 @@ struct Flight {
     size_t o_lens = 0, o_offs = 0; // device offsets of lens_all and offs (adjacent)
     size_t p_down = 0;             // pinned offsets: lens + offs, packed payload
     size_t p_payload = 0;
+    // device offsets the later phases use
+    size_t o_jobs = 0, o_pjobs = 0, o_queue = 0, o_cnt = 0, o_scan = 0, scan_bytes = 0;
+    uint64_t chunks = 0;           // work items of the first round
+    // serving rounds (kkw_serve_opts.round_steps > 0): walkers run at most
+    // round_steps steps per launch; the walk is relaunched on the suspended
+    // ones until none is left, then the rows are packed
+    bool rounds = false;
+    bool packing = false;          // the last round is done: pack and copy-back enqueued
+    int round = 0;
+    size_t o_state = 0, o_list[2] = {0, 0}, o_rcnt = 0;   // resume states, (job, row) lists, counters
+    size_t p_rcnt = 0;             // pinned slot (in the upload mirror) of the suspended count
 };
 
 struct ServeState {
@@ -605,12 +616,14 @@ struct ServeState {
 static size_t al256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 // Device bytes one request adds to a flight: rows, lengths, counts, offsets,
-// starts and scheme tables.
-static size_t request_dev_bytes(const PreparedJob &pj)
+// starts and scheme tables, and with rounds its resume states and its share
+// of the two suspended-walker lists.
+static size_t request_dev_bytes(const PreparedJob &pj, bool rounds)
 {
     const size_t n = pj.dj.walker_num;
     return al256(n * pj.stride * 4) + al256(n * 4) + al256(n * 8) * 2 + al256(pj.starts.size() * 4) +
-           al256(pj.scheme_len.size() * 4) + al256(pj.scheme_mask.size()) + 1024;
+           al256(pj.scheme_len.size() * 4) + al256(pj.scheme_mask.size()) + 1024 +
+           (rounds ? n * (sizeof(ResumeState) + 16) + 1024 : 0);
 }
 
 // Pinned bytes one request adds to a flight: its share of the upload mirror,
@@ -629,6 +642,9 @@ static void finish_request(ServeState *S, const std::shared_ptr<Request> &r, int
     r->t_done = now_ms();
     S->cv_done.notify_all();
 }
+
+static void enqueue_pack(ServeState *S, Flight &f);
+static void enqueue_round(ServeState *S, Flight &f, const DevGraph &g, uint64_t n_in);
 
 // Build and launch one flight.  Returns false (and leaves reqs pending) when
 // the device or pinned arena has no room for them yet; *pin_short tells which.  One stream-ordered
@@ -666,6 +682,9 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
     }
     const size_t o_queue = o;   // the walk's work counter, uploaded as zero
     o += 256;
+    f.rounds = S->opts.round_steps != 0;
+    f.o_rcnt = o;               // rounds: the two suspended-walker counters; pinned: the count copied back
+    o += 256;
     const size_t upload_bytes = o;
     f.o_lens = o;
     const size_t lens_bytes = ((N * 4 + 7) & ~(size_t)7);
@@ -684,6 +703,14 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
         o += al256((size_t)pj.dj.walker_num * pj.stride * 4);
         packed_max += (size_t)pj.dj.walker_num * pj.stride;
     }
+    if (f.rounds) {
+        f.o_state = o;
+        o += al256(N * sizeof(ResumeState));
+        f.o_list[0] = o;
+        o += al256(N * 8);
+        f.o_list[1] = o;
+        o += al256(N * 8);
+    }
     f.dev_bytes = o;
     // ---- pinned: [upload mirror][lens | offs][packed payload, at most rows x stride]
     f.p_down = upload_bytes;
@@ -697,7 +724,6 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
     }
     char *D = S->dev_base + f.dev_off;
     char *H = S->pin->base + f.pin_off;
-    char *HD = S->pin->dbase + f.pin_off;   // the same block as the kernels address it
     // ---- fill the upload mirror in pinned memory
     std::vector<DevJob> djs(nj);
     std::vector<PackJob> pjs(nj);
@@ -720,12 +746,22 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
         }
         d.chunk_begin = chunk_acc;
         chunk_acc += (d.walker_num + chunk - 1) / chunk;
+        d.resume = f.rounds ? (ResumeState *)(D + f.o_state) + f.row0[j] : nullptr;
         djs[j] = d;
         pjs[j] = PackJob{f.row0[j], d.walker_num, pj.stride, d.paths};
     }
     memcpy(H + o_jobs, djs.data(), sizeof(DevJob) * nj);
     memcpy(H + o_pjobs, pjs.data(), sizeof(PackJob) * nj);
     memset(H + o_queue, 0, sizeof(unsigned long long));
+    memset(H + f.o_rcnt, 0, 2 * sizeof(unsigned long long));
+    f.p_rcnt = f.o_rcnt;
+    f.o_jobs = o_jobs;
+    f.o_pjobs = o_pjobs;
+    f.o_queue = o_queue;
+    f.o_cnt = o_cnt;
+    f.o_scan = o_scan;
+    f.scan_bytes = scan_bytes;
+    f.chunks = chunk_acc;
     // ---- enqueue: upload, walk, pack into pinned memory, lengths + offsets
     DevGraph g;
     {
@@ -737,21 +773,93 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
             g.V != r->graph_V)
             throw Error(KKW_E_NOGRAPH, "the graph or the tables a query needs are gone");
     KKW_HIP(hipMemcpyAsync(D, H, upload_bytes, hipMemcpyHostToDevice, st));
+    KKW_HIP(hipEventCreateWithFlags(&f.ev, hipEventDisableTiming));
+    if (f.rounds && chunk_acc) {
+        enqueue_round(S, f, g, 0);
+        return true;
+    }
     if (chunk_acc) {
         uint64_t blocks_needed = (chunk_acc * 64 + 255) / 256;
         int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)eng->cu_count * 8, blocks_needed));
         // serving rows stay 16-byte groups (short PPR walks; full occupancy)
         launch_walk_persistent_mixed(g, (const DevJob *)(D + o_jobs), nj, chunk_acc, chunk,
                                      (unsigned long long *)(D + o_queue), false, nullptr, grid, 256, false, st);
-        launch_pack((const PackJob *)(D + o_pjobs), nj, (const uint32_t *)(D + f.o_lens), (uint64_t *)(D + o_cnt),
-                    (uint64_t *)(D + f.o_offs), N, (uint32_t *)(HD + f.p_payload), D + o_scan, scan_bytes, st);
+    }
+    enqueue_pack(S, f);
+    return true;
+}
+
+// After the walk (its last round): pack the rows straight into the flight's
+// pinned block, copy lengths and offsets beside them, record the event.
+static void enqueue_pack(ServeState *S, Flight &f)
+{
+    hipStream_t st = S->streams[f.sidx];
+    char *D = S->dev_base + f.dev_off;
+    char *H = S->pin->base + f.pin_off;
+    char *HD = S->pin->dbase + f.pin_off;   // the same pinned block as the kernels address it
+    const uint64_t N = f.walkers;
+    const int nj = (int)f.reqs.size();
+    if (f.chunks) {
+        launch_pack((const PackJob *)(D + f.o_pjobs), nj, (const uint32_t *)(D + f.o_lens), (uint64_t *)(D + f.o_cnt),
+                    (uint64_t *)(D + f.o_offs), N, (uint32_t *)(HD + f.p_payload), D + f.o_scan, f.scan_bytes, st);
     } else {
         launch_fill_u64((unsigned long long *)(D + f.o_offs), 0ull, 1, st);
     }
     KKW_HIP(hipGetLastError());
-    KKW_HIP(hipMemcpyAsync(H + f.p_down, D + f.o_lens, down_bytes, hipMemcpyDeviceToHost, st));
-    KKW_HIP(hipEventCreateWithFlags(&f.ev, hipEventDisableTiming));
+    const size_t lens_bytes = ((N * 4 + 7) & ~(size_t)7);
+    KKW_HIP(hipMemcpyAsync(H + f.p_down, D + f.o_lens, lens_bytes + (N + 1) * 8, hipMemcpyDeviceToHost, st));
     KKW_HIP(hipEventRecord(f.ev, st));
+    f.packing = true;
+}
+
+// One round of a flight walked in rounds: round r continues the walkers
+// listed by round r - 1 (all of them in round 0), lists the ones it suspends
+// and copies their count to the pinned slot; the event says when it is there.
+static void enqueue_round(ServeState *S, Flight &f, const DevGraph &g, uint64_t n_in)
+{
+    hipStream_t st = S->streams[f.sidx];
+    char *D = S->dev_base + f.dev_off;
+    char *H = S->pin->base + f.pin_off;
+    const uint32_t chunk = 64;
+    const int out = (f.round + 1) & 1;
+    unsigned long long *cnt = (unsigned long long *)(D + f.o_rcnt);
+    launch_fill_u64(cnt + out, 0ull, 1, st);
+    launch_fill_u64((unsigned long long *)(D + f.o_queue), 0ull, 1, st);
+    RoundArgs R;
+    R.round_steps = S->opts.round_steps;
+    R.in = f.round == 0 ? nullptr : (const uint32_t *)(D + f.o_list[f.round & 1]);
+    R.n_in = f.round == 0 ? 0 : n_in;
+    R.out = (uint32_t *)(D + f.o_list[out]);
+    R.n_out = cnt + out;
+    const uint64_t chunks = f.round == 0 ? f.chunks : (n_in + chunk - 1) / chunk;
+    const uint64_t blocks_needed = (chunks * 64 + 255) / 256;
+    const int grid =
+        (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)S->eng->cu_count * 8, blocks_needed));
+    launch_walk_rounds(g, (const DevJob *)(D + f.o_jobs), (int)f.reqs.size(), chunks, chunk,
+                       (unsigned long long *)(D + f.o_queue), R, grid, 256, st);
+    KKW_HIP(hipGetLastError());
+    KKW_HIP(hipMemcpyAsync(H + f.p_rcnt, cnt + out, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    KKW_HIP(hipEventRecord(f.ev, st));
+}
+
+// A flight walked in rounds whose round has completed: relaunch on the
+// suspended walkers, or pack when none is left.  Returns true when the flight
+// has more work on the device (not ready to retire).
+static bool advance_rounds(ServeState *S, Flight &f)
+{
+    if (!f.rounds || f.packing || !f.chunks) return false;
+    const uint64_t n = *(volatile const uint64_t *)(S->pin->base + f.pin_off + f.p_rcnt);
+    if (n == 0) {
+        enqueue_pack(S, f);
+        return true;
+    }
+    DevGraph g;
+    {
+        std::lock_guard<std::recursive_mutex> lk(S->eng->mu);
+        g = S->eng->gs.view();
+    }
+    f.round++;
+    enqueue_round(S, f, g, n);
     return true;
 }
 
@@ -869,6 +977,10 @@ static void serve_loop(ServeState *S)
             }
             progressed = true;
             try {
+                if (advance_rounds(S, f)) {   // another round, or the packing, is on its way
+                    ++it;
+                    continue;
+                }
                 f.t_phase1 = f.t_phase2 = now_ms();
                 flight_retire(S, f);
                 f.t_retired = now_ms();
@@ -883,6 +995,7 @@ static void serve_loop(ServeState *S)
                 }
             } catch (const std::exception &ex) {
                 const int code = dynamic_cast<const Error *>(&ex) ? static_cast<const Error &>(ex).code : KKW_E_NOMEM;
+                (void)hipStreamSynchronize(S->streams[f.sidx]);   // nothing of it may still run
                 for (auto &r : f.reqs)
                     if (r->status == 1) finish_request(S, r, code);
                 release_flight(S, f);
@@ -918,7 +1031,7 @@ static void serve_loop(ServeState *S)
                         S->cv_done.notify_all();
                         continue;
                     }
-                    size_t need = request_dev_bytes(r->pj) + 2 * pack_scan_bytes(r->pj.dj.walker_num + 1);
+                    size_t need = request_dev_bytes(r->pj, S->opts.round_steps != 0) + 2 * pack_scan_bytes(r->pj.dj.walker_num + 1);
                     if (need + 4096 > S->dev.cap || request_pin_bytes(r->pj) + 4096 > S->pin->a.cap) {
                         S->pending.pop_front();   // can never fit the arenas
                         r->status = KKW_E_NOMEM;

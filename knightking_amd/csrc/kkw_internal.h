@@ -79,6 +79,20 @@ void launch_walk_persistent_single(const DevGraph &g, const DevJob &J, uint64_t 
 void launch_walk_persistent_mixed(const DevGraph &g, const DevJob *jobs, int n_jobs, uint64_t n_chunks,
                                   uint32_t chunk, unsigned long long *queue, bool stats, DevStats *st,
                                   int grid, int block, bool lines, hipStream_t stream);
+// Serving rounds: every walker takes at most round_steps steps in one launch;
+// walkers still going are suspended (state saved, (job, row) appended to
+// `out`, counted in *n_out, which must be zero at launch).  The first round
+// (in == null) takes chunks of the jobs' walker ranges; later ones take
+// chunks of the n_in (job, row) pairs in `in`.
+struct RoundArgs {
+    uint32_t round_steps;
+    const uint32_t *in;
+    uint64_t n_in;
+    uint32_t *out;
+    unsigned long long *n_out;
+};
+void launch_walk_rounds(const DevGraph &g, const DevJob *jobs, int n_jobs, uint64_t n_chunks, uint32_t chunk,
+                        unsigned long long *queue, const RoundArgs &R, int grid, int block, hipStream_t stream);
 void launch_fill_u64(unsigned long long *p, unsigned long long v, int n, hipStream_t stream);
 void launch_stream_copy(const void *src, void *dst, uint64_t bytes, int cu_count, hipStream_t stream);
 void launch_random_gather(const void *buf, uint64_t bytes, uint32_t threads, uint32_t reads, uint64_t seed,

@@ -541,6 +541,126 @@ __global__ void __launch_bounds__(256) k_walk_persistent(DevGraph g, DevJob J0, 
 // Plain streaming copy (16 B per lane per iteration): the known-traffic kernel
 // that HBM counters and the achievable bandwidth are calibrated against.
 // four 16-byte loads in flight per lane, then their stores
+// ------------------------------------------------------ serving rounds ----
+// A walker back from a suspension: state from its resume slot, rows of the
+// current and previous vertices from row_ptr, the current 16-byte path group
+// from its row (stored whole at the suspension).
+__device__ __forceinline__ void resume_walker(const DevGraph &g, const DevJob &J, uint32_t job, uint32_t idx,
+                                              Walker &s)
+{
+    const ResumeState r = J.resume[idx];
+    s.job = job;
+    s.idx = idx;
+    s.wid = J.walker_begin + idx;
+    s.cur = r.cur;
+    s.prev = r.prev;
+    s.step = r.step;
+    s.trial = 0;
+    s.scheme = r.meta >> 8;
+    s.state = r.meta & 0xffu;
+    s.beg = g.row_ptr[s.cur];
+    s.end = g.row_ptr[s.cur + 1];
+    s.pbeg = s.step ? g.row_ptr[s.prev] : 0;
+    s.pend = s.step ? g.row_ptr[s.prev + 1] : 0;
+    s.cvm = J.kind == KIND_METAPATH ? g.vmask[s.cur] : 0u;
+    s.lines = false;
+    const uint4 q = *reinterpret_cast<const uint4 *>(J.paths + (uint64_t)idx * J.stride + (s.step & ~3u));
+    s.pg.q0 = q.x;
+    s.pg.q1 = q.y;
+    s.pg.q2 = q.z;
+    s.pg.q3 = q.w;
+}
+
+// The mixed-kind persistent loop of k_walk_persistent, bounded to
+// R.round_steps steps per walker: a walker that reaches the bound at a step
+// boundary stores its current path group, saves its state and is listed
+// (one atomic per wave) for the next round.  Draws are keyed by (walker,
+// step, trial) and trials restart at every step, so paths are the same with
+// or without rounds.
+__global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *jobs, int n_jobs, uint64_t n_chunks,
+                                                   uint32_t chunk, unsigned long long *queue, RoundArgs R)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    LaneStats ls = {0, 0, 0, 0, 0, 0, 0, 0};
+    Walker s;
+    bool active = false;
+    uint32_t round_start = 0;
+    uint32_t pool_job = 0;
+    uint64_t pool_next = 0, pool_end = 0;
+    bool drained = false;
+    for (;;) {
+        uint64_t need = __ballot(!active);
+        while (need && !drained) {
+            if (pool_next >= pool_end) {
+                unsigned long long c = 0;
+                if (lane == 0) c = atomicAdd(queue, 1ull);
+                c = __shfl(c, 0, 64);
+                if (c >= n_chunks) {
+                    drained = true;
+                    break;
+                }
+                if (R.in) {
+                    pool_next = c * chunk;
+                    const uint64_t e = pool_next + chunk;
+                    pool_end = e < R.n_in ? e : R.n_in;
+                } else {
+                    const uint32_t j = find_job(jobs, n_jobs, c);
+                    pool_job = j;
+                    pool_next = (c - jobs[j].chunk_begin) * chunk;
+                    const uint64_t e = pool_next + chunk;
+                    pool_end = e < jobs[j].walker_num ? e : jobs[j].walker_num;
+                }
+            }
+            const uint64_t avail = pool_end - pool_next;
+            const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
+            if (!active && rank < avail) {
+                if (R.in) {
+                    const uint64_t k = pool_next + rank;
+                    const uint32_t j = R.in[2 * k], idx = R.in[2 * k + 1];
+                    resume_walker(g, jobs[j], j, idx, s);
+                } else {
+                    init_walker<KIND_ANY>(g, jobs[pool_job], pool_job, (uint32_t)(pool_next + rank), s, ls, false);
+                }
+                round_start = s.step;
+                active = true;
+            }
+            const uint32_t want = (uint32_t)__popcll(need);
+            pool_next += want < avail ? want : avail;
+            need = __ballot(!active);
+        }
+        if (__ballot(active) == 0ull) break;
+        bool suspend = false;
+        if (active) {
+            const DevJob &J = jobs[s.job];
+            if (advance<KIND_ANY, false, false>(g, J, s, ls)) active = false;
+            else suspend = s.trial == 0 && s.step - round_start >= R.round_steps;
+        }
+        const uint64_t sm = __ballot(suspend);
+        if (sm) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)sm) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(R.n_out, (unsigned long long)__popcll(sm));
+            base = __shfl(base, (int)leader, 64);
+            if (suspend) {
+                const DevJob &J = jobs[s.job];
+                s.pg.store(J.paths + (uint64_t)s.idx * J.stride, s.step);
+                J.resume[s.idx] = ResumeState{s.cur, s.prev, s.step, (s.scheme << 8) | s.state};
+                const uint64_t pos = base + (uint64_t)__popcll(sm & lt_mask);
+                R.out[2 * pos] = s.job;
+                R.out[2 * pos + 1] = s.idx;
+                active = false;
+            }
+        }
+    }
+}
+
+void launch_walk_rounds(const DevGraph &g, const DevJob *jobs, int n_jobs, uint64_t n_chunks, uint32_t chunk,
+                        unsigned long long *queue, const RoundArgs &R, int grid, int block, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_walk_rounds, dim3(grid), dim3(block), 0, stream, g, jobs, n_jobs, n_chunks, chunk, queue, R);
+}
+
 __global__ void __launch_bounds__(256) k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
                                                      uint64_t n)
 {

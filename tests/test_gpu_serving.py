@@ -216,3 +216,48 @@ def test_graph_cannot_change_while_serving(served):
     eng.adj_index(False)   # allowed again
     eng.adj_index(True)
     assert eng.info() == (V, E)
+
+
+@pytest.mark.parametrize("round_steps", [1, 3, 16])
+def test_rounds_give_the_same_paths(served, round_steps):
+    """Flights walked in rounds of at most round_steps steps per walker
+    (walkers suspended at step boundaries and continued by the next launch):
+    every kind, every result equal to the oracle's walk of that query."""
+    from helpers import oracle_walk, packed_matches_oracle
+    from oracle import oracle as O
+    eng = served
+    V, _ = eng.info()
+    rp, col, w, et = eng.download(etype=True)
+    g = O.Graph(rp, col, w, et)
+    jobs = mixed_queries(V, 24, 5)
+    eng.serve_start(max_walkers_in_flight=20000, round_steps=round_steps)
+    tickets = [eng.submit(j) for j in jobs]
+    for i, t in enumerate(tickets):
+        got, _ = eng.wait(t, 60.0, packed=True)
+        op, ol, ot, _ = oracle_walk(g, jobs[i])
+        assert packed_matches_oracle(got, op, ol, ot), f"query {i} differs with rounds of {round_steps}"
+
+
+def test_rounds_let_a_short_query_pass_a_long_one(engine):
+    """A long node2vec query (2^20 walkers, L = 80, R-MAT scale 20) served in
+    rounds of 4 steps gives the GPU back between rounds: a short PPR query
+    submitted after it completes while the long one is still walking."""
+    print("[gpu] rounds: long node2vec query and a short PPR query on R-MAT-20", flush=True)
+    engine.graph_rmat(20, 16, seed=2, weighted=True)
+    engine.prepare()
+    import time
+    long_job = kk.node2vec_job(1 << 20, 80, 2.0, 0.5, True, seed=3)
+    short_job = kk.ppr_job(1000, 0.15, True, seed=4, start_vertices=np.array([1], np.uint32))
+    engine.serve_start(round_steps=4)
+    try:
+        t_long = engine.submit(long_job)
+        time.sleep(0.001)
+        t_short = engine.submit(short_job)
+        got, lat_short = engine.wait(t_short, 60.0, packed=True)
+        long_still_running = not engine.poll(t_long)
+        got_long, lat_long = engine.wait(t_long, 60.0, packed=True)
+        print(f"[gpu] short {lat_short:.3f} ms, long {lat_long:.3f} ms", flush=True)
+        assert got.lengths.shape == (1000,) and got_long.lengths.shape == (1 << 20,)
+        assert long_still_running, (lat_short, lat_long)
+    finally:
+        engine.serve_stop()
