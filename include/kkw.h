@@ -230,6 +230,27 @@ int kkw_paths_free(kkw_paths *paths);
 #define KKW_NOALLOC_SCRATCH_BYTES (4u << 20)
 int kkw_run_noalloc(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mode,
                     uint32_t *const *paths, const uint32_t *strides, uint32_t *const *lengths);
+/* Epochs streamed to a consumer on the GPU.  WalkConfig's output consumer
+ * receives one PathSet per epoch (include/walk.hpp:133-138, 826-842); here the
+ * job's walkers run in epochs of epoch_walkers (the last one shorter), each
+ * epoch's paths stay in HBM, and `consumer` is called on the calling thread
+ * as soon as an epoch has been walked, with the epoch's device paths
+ * (on_device = 1, valid during the call and for the work it enqueues) and a
+ * HIP stream (hipStream_t, passed as void *) on which to enqueue its GPU work:
+ * that stream is ordered after the epoch's walk, and the next epoch is
+ * already walking on the engine stream while the consumer's work runs.  An
+ * epoch's buffers are reused two epochs later, once the work the consumer
+ * enqueued on the stream has finished.  A nonzero return from the consumer
+ * stops the run (KKW_E_CANCELLED).  Blocks until the last consumer's work is
+ * done.  kkw_last_timing's kernel_ms is the sum of the epochs' walks. */
+typedef int (*kkw_epoch_consumer)(const kkw_paths *paths, void *stream, void *user);
+/* A host copy of device paths (an epoch handed to a consumer, a batch's
+ * KKW_OUT_DEVICE paths): row-major, the same stride; free with kkw_paths_free.
+ * Synchronous. */
+int kkw_paths_download(const kkw_paths *device_paths, kkw_paths *host_out);
+int kkw_run_epochs(kkw_engine *eng, const kkw_job *job, uint32_t epoch_walkers, int mode,
+                   kkw_epoch_consumer consumer, void *user);
+
 /* PathSet::dump text format (include/path.hpp:85-109): per walker, optional
  * "id len", then " v" per vertex, then newline; fopen mode "w" or "a". */
 int kkw_paths_dump(const kkw_paths *paths, const char *file, const char *mode,

@@ -23,6 +23,7 @@
 #include <string.h>
 #include <sys/mman.h>
 
+#include <exception>
 #include <functional>
 #include <memory>
 #include <mutex>
@@ -158,11 +159,14 @@ public:
     std::function<void(PathSet *)> output_consumer_func;
     bool device_consumer_flag;
     std::function<void(const kkw_paths &)> device_consumer_func;
+    bool device_stream_consumer_flag;
+    std::function<void(const kkw_paths &, void *)> device_stream_consumer_func;
     double rate;
 
     WalkConfig()
         : output_file_flag(false), print_with_head_info(false), output_consumer_flag(false),
-          output_consumer_func(nullptr), device_consumer_flag(false), device_consumer_func(nullptr), rate(1)
+          output_consumer_func(nullptr), device_consumer_flag(false), device_consumer_func(nullptr),
+          device_stream_consumer_flag(false), device_stream_consumer_func(nullptr), rate(1)
     {
     }
     void set_output_file(const char *prefix, bool with_head_info = false)
@@ -187,6 +191,17 @@ public:
         assert(f != nullptr);
         device_consumer_flag = true;
         device_consumer_func = f;
+    }
+    // No counterpart in the reference: each epoch's device paths with a HIP
+    // stream (hipStream_t as void *) ordered after the epoch's walk, for a
+    // consumer that enqueues GPU work there instead of finishing it in the
+    // call; the next epoch walks meanwhile (kkw_run_epochs).  The paths stay
+    // valid for the work enqueued on the stream.
+    void set_device_stream_consumer(std::function<void(const kkw_paths &, void *)> f)
+    {
+        assert(f != nullptr);
+        device_stream_consumer_flag = true;
+        device_stream_consumer_func = f;
     }
     void set_walk_rate(double r)
     {
@@ -543,6 +558,12 @@ public:
         walker_id_t per_iter = (walker_id_t)(walker_num * wc->rate);
         if (per_iter == 0) per_iter = 1;
         if (per_iter > walker_num) per_iter = walker_num;
+        if (wc->device_stream_consumer_flag) {
+            if (wc->device_consumer_flag)
+                throw std::invalid_argument("WalkConfig: set either a device consumer or a device stream consumer");
+            run_epochs_streamed(job, walker_num, per_iter, wc);
+            return;
+        }
         walker_id_t remained = walker_num;
         int iter = 0;
         while (remained != 0) {
@@ -651,6 +672,47 @@ public:
     }
 
 private:
+    // Epochs through kkw_run_epochs: the stream consumer gets each epoch in
+    // HBM while the next one walks; file and host consumers get host copies.
+    struct EpochCtx {
+        WalkConfig *wc;
+        int iter;
+        std::exception_ptr err;
+    };
+    static int epoch_trampoline(const kkw_paths *p, void *stream, void *user)
+    {
+        EpochCtx *c = static_cast<EpochCtx *>(user);
+        try {
+            c->wc->device_stream_consumer_func(*p, stream);
+            if (c->wc->output_file_flag || c->wc->output_consumer_flag) {
+                kkw_paths host{};
+                knightking_detail::check(kkw_paths_download(p, &host));
+                PathSet *ps = to_pathset(host);
+                kkw_paths_free(&host);
+                if (c->wc->output_file_flag) {
+                    std::string path = c->wc->output_path_prefix + ".0";
+                    ps->dump(path.c_str(), c->iter == 0 ? "w" : "a", c->wc->print_with_head_info);
+                }
+                if (c->wc->output_consumer_flag) c->wc->output_consumer_func(ps);
+                else delete ps;
+            }
+            c->iter++;
+            return 0;
+        } catch (...) {
+            c->err = std::current_exception();
+            return 1;
+        }
+    }
+    void run_epochs_streamed(kkw_job job, walker_id_t walker_num, walker_id_t per_iter, WalkConfig *wc)
+    {
+        job.walker_begin = 0;
+        job.walker_num = walker_num;
+        EpochCtx ctx{wc, 0, nullptr};
+        const int rc = kkw_run_epochs(eng_, &job, per_iter, KKW_MODE_PERSISTENT, epoch_trampoline, &ctx);
+        if (ctx.err) std::rethrow_exception(ctx.err);
+        knightking_detail::check(rc);
+    }
+
     void drop_mirror()
     {
         std::lock_guard<std::recursive_mutex> lk(mirror_mu_);

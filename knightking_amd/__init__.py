@@ -85,13 +85,16 @@ class _ServeOpts(C.Structure):
                 ("queue_capacity", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+# kkw_epoch_consumer: (paths, stream, user) -> 0 to go on
+EPOCH_CONSUMER = C.CFUNCTYPE(C.c_int, C.POINTER(_Paths), C.c_void_p, C.c_void_p)
+
 # Every symbol include/kkw.h declares (tests check the library exports them all).
 EXPORTED = [
     "kkw_last_error", "kkw_abi_version", "kkw_engine_create", "kkw_engine_destroy",
     "kkw_graph_from_edges", "kkw_graph_from_arrays", "kkw_graph_sort_adj", "kkw_graph_load_file", "kkw_graph_from_csr", "kkw_graph_rmat",
     "kkw_graph_set_edge_types", "kkw_graph_type_by_dst", "kkw_graph_info", "kkw_graph_download",
     "kkw_prepare", "kkw_graph_adj_index", "kkw_alias_download", "kkw_uniform_start", "kkw_run", "kkw_run_noalloc",
-    "kkw_paths_free",
+    "kkw_run_epochs", "kkw_paths_download", "kkw_paths_free",
     "kkw_paths_dump",
     "kkw_last_timing", "kkw_batch_create", "kkw_batch_set_shape", "kkw_batch_launch",
     "kkw_batch_sync", "kkw_batch_paths", "kkw_batch_destroy", "kkw_autotune", "kkw_diag_stream_copy",
@@ -165,6 +168,8 @@ def load_library(path: str = LIB_PATH):
         "kkw_run": (I32, [P, C.POINTER(_Job), I32, I32, I32, I32, C.POINTER(_Paths)]),
         "kkw_run_noalloc": (I32, [P, C.POINTER(_Job), I32, I32, P, P, P]),
         "kkw_uniform_start": (U32, [U64, U32, U32]),
+        "kkw_run_epochs": (I32, [P, C.POINTER(_Job), U32, I32, EPOCH_CONSUMER, P]),
+        "kkw_paths_download": (I32, [C.POINTER(_Paths), C.POINTER(_Paths)]),
         "kkw_paths_free": (I32, [C.POINTER(_Paths)]),
         "kkw_paths_dump": (I32, [C.POINTER(_Paths), C.c_char_p, C.c_char_p, I32]),
         "kkw_last_timing": (I32, [P, C.POINTER(_Timing)]),
@@ -646,6 +651,35 @@ class Engine:
         ss = (C.c_uint32 * n)(*strides)
         ll = (C.c_void_p * n)(*lengths)
         _check(_lib.kkw_run_noalloc(self.h, cj, n, mode, pp, ss, ll))
+
+    def run_epochs(self, job: Job, epoch_walkers: int, consumer, mode: int = MODE_PERSISTENT) -> None:
+        """kkw_run_epochs: walk `job` in epochs of epoch_walkers; consumer(paths,
+        stream) gets each epoch's device paths (a _Paths with on_device = 1)
+        and a HIP stream (int address) ordered after that epoch's walk, while
+        the next epoch walks.  A consumer that returns False or raises stops
+        the run (the exception is raised again here)."""
+        errors = []
+
+        def tramp(pp, stream, user):
+            try:
+                return 1 if consumer(pp.contents, stream) is False else 0
+            except BaseException as e:   # re-raised below, after the library has returned
+                errors.append(e)
+                return 1
+
+        fn = EPOCH_CONSUMER(tramp)
+        cj = job.to_c()
+        rc = _lib.kkw_run_epochs(self.h, C.byref(cj), epoch_walkers, mode, fn, None)
+        if errors:
+            raise errors[0]
+        _check(rc)
+
+    @staticmethod
+    def download_paths(device_paths: "_Paths") -> Paths:
+        """Host copy of device paths (kkw_paths_download)."""
+        out = _Paths()
+        _check(_lib.kkw_paths_download(C.byref(device_paths), C.byref(out)))
+        return _take_paths(out)
 
     def batch(self, jobs: Sequence[Job], mode: int = MODE_PERSISTENT, flags: int = 0) -> Batch:
         return Batch(self, jobs, mode, flags)

@@ -1732,6 +1732,154 @@ int kkw_run_noalloc(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mode, 
     });
 }
 
+int kkw_run_epochs(kkw_engine *eng, const kkw_job *job, uint32_t epoch_walkers, int mode,
+                   kkw_epoch_consumer consumer, void *user)
+{
+    return guarded([&]() -> int {
+        if (!eng || !job || !consumer) return fail(KKW_E_INVALID, "null argument");
+        if (epoch_walkers == 0) return fail(KKW_E_INVALID, "epoch_walkers must be positive");
+        if (mode != KKW_MODE_PLAIN && mode != KKW_MODE_PERSISTENT) return fail(KKW_E_INVALID, "unknown mode");
+        std::lock_guard<std::recursive_mutex> lk(eng->mu);
+        KKW_HIP(hipSetDevice(eng->device));
+        const uint32_t total = job->walker_num;
+        const uint32_t W = std::min(epoch_walkers, std::max(total, 1u));
+        kkw_job first = *job;
+        first.walker_num = std::min(W, total);
+        PreparedJob proto = prepare_job(eng, first, mode == KKW_MODE_PERSISTENT);   // tables built here
+        // two epochs' buffers (ping-pong), one work-queue word
+        DevBuf paths0((size_t)W * proto.stride * 4), paths1((size_t)W * proto.stride * 4);
+        DevBuf lens0((size_t)W * 4), lens1((size_t)W * 4), queue(256);
+        uint32_t *paths[2] = {(uint32_t *)paths0.p, (uint32_t *)paths1.p};
+        uint32_t *lens[2] = {(uint32_t *)lens0.p, (uint32_t *)lens1.p};
+        struct Res {
+            hipStream_t cs = nullptr;
+            hipEvent_t walked[2] = {nullptr, nullptr}, consumed[2] = {nullptr, nullptr}, k0[2] = {nullptr, nullptr};
+            ~Res()
+            {
+                if (cs) {
+                    (void)hipStreamSynchronize(cs);
+                    (void)hipStreamDestroy(cs);
+                }
+                for (int k = 0; k < 2; k++) {
+                    if (walked[k]) (void)hipEventDestroy(walked[k]);
+                    if (consumed[k]) (void)hipEventDestroy(consumed[k]);
+                    if (k0[k]) (void)hipEventDestroy(k0[k]);
+                }
+            }
+        } r;
+        KKW_HIP(hipStreamCreateWithFlags(&r.cs, hipStreamNonBlocking));
+        for (int k = 0; k < 2; k++) {
+            KKW_HIP(hipEventCreate(&r.walked[k]));
+            KKW_HIP(hipEventCreate(&r.k0[k]));
+            KKW_HIP(hipEventCreateWithFlags(&r.consumed[k], hipEventDisableTiming));
+        }
+        hipStream_t st = eng->stream;
+        const uint32_t n_epochs = total ? (total + W - 1) / W : 0;
+        std::vector<PreparedJob> ej(1, proto);
+        // upload the start list / schemes once; every epoch's DevJob points at them
+        std::vector<DevBuf> keep;
+        if (!proto.starts.empty()) {
+            keep.emplace_back(proto.starts.size() * 4);
+            KKW_HIP(hipMemcpyAsync(keep.back().p, proto.starts.data(), proto.starts.size() * 4,
+                                   hipMemcpyHostToDevice, st));
+            ej[0].dj.starts = (const uint32_t *)keep.back().p;
+        }
+        if (!proto.scheme_len.empty()) {
+            keep.emplace_back(proto.scheme_len.size() * 4);
+            KKW_HIP(hipMemcpyAsync(keep.back().p, proto.scheme_len.data(), proto.scheme_len.size() * 4,
+                                   hipMemcpyHostToDevice, st));
+            ej[0].dj.scheme_len = (const uint32_t *)keep.back().p;
+            keep.emplace_back(proto.scheme_mask.size());
+            KKW_HIP(hipMemcpyAsync(keep.back().p, proto.scheme_mask.data(), proto.scheme_mask.size(),
+                                   hipMemcpyHostToDevice, st));
+            ej[0].dj.scheme_mask = (const uint8_t *)keep.back().p;
+        }
+        auto enqueue_epoch = [&](uint32_t e) {
+            const int b = e & 1;
+            if (e >= 2) KKW_HIP(hipStreamWaitEvent(st, r.consumed[b], 0));   // buffer b is free again
+            DevJob &d = ej[0].dj;
+            d.walker_begin = job->walker_begin + e * W;
+            d.walker_num = std::min(W, total - e * W);
+            d.paths = paths[b];
+            d.lens = lens[b];
+            d.chunk_begin = 0;
+            WalkLaunch w;
+            w.mode = mode;
+            w.n_chunks = (d.walker_num + w.chunk - 1) / w.chunk;
+            w.d_queue = (unsigned long long *)queue.p;
+            KKW_HIP(hipEventRecord(r.k0[b], st));
+            enqueue_walk(eng, ej, w, st);
+            KKW_HIP(hipEventRecord(r.walked[b], st));
+        };
+        const double t_start = now_ms();
+        double kernel_ms = 0.0;
+        int rc = KKW_OK;
+        if (n_epochs) enqueue_epoch(0);
+        for (uint32_t e = 0; e < n_epochs; e++) {
+            const int b = e & 1;
+            const uint32_t begin = job->walker_begin + e * W, n = std::min(W, total - e * W);
+            if (e + 1 < n_epochs) enqueue_epoch(e + 1);   // walks while epoch e is consumed
+            KKW_HIP(hipEventSynchronize(r.walked[b]));
+            float ms = 0.0f;
+            KKW_HIP(hipEventElapsedTime(&ms, r.k0[b], r.walked[b]));
+            kernel_ms += ms;
+            KKW_HIP(hipStreamWaitEvent(r.cs, r.walked[b], 0));
+            kkw_paths p;
+            memset(&p, 0, sizeof(p));
+            p.walker_begin = begin;
+            p.walker_num = n;
+            p.stride = proto.stride;
+            p.on_device = 1;
+            p.vertices = paths[b];
+            p.lengths = lens[b];
+            const int c = consumer(&p, (void *)r.cs, user);
+            KKW_HIP(hipEventRecord(r.consumed[b], r.cs));
+            if (c != 0) {
+                rc = fail(KKW_E_CANCELLED, "the epoch consumer stopped the run");
+                break;
+            }
+        }
+        KKW_HIP(hipStreamSynchronize(st));
+        KKW_HIP(hipStreamSynchronize(r.cs));
+        eng->timing.kernel_ms = kernel_ms;
+        eng->timing.total_ms = now_ms() - t_start;
+        return rc;
+    });
+}
+
+int kkw_paths_download(const kkw_paths *src, kkw_paths *out)
+{
+    return guarded([&]() -> int {
+        if (!src || !out) return fail(KKW_E_INVALID, "null argument");
+        if (!src->on_device) return fail(KKW_E_INVALID, "paths are already on the host");
+        if (src->offsets) return fail(KKW_E_INVALID, "packed device paths are not supported");
+        const size_t n = src->walker_num, cells = n * src->stride;
+        memset(out, 0, sizeof(*out));
+        out->walker_begin = src->walker_begin;
+        out->walker_num = src->walker_num;
+        out->stride = src->stride;
+        out->vertices = (uint32_t *)malloc(cells ? cells * 4 : 4);
+        out->lengths = (uint32_t *)malloc(n ? n * 4 : 4);
+        if (!out->vertices || !out->lengths) {
+            free(out->vertices);
+            free(out->lengths);
+            memset(out, 0, sizeof(*out));
+            return fail(KKW_E_NOMEM, "host allocation for paths failed");
+        }
+        out->priv = (void *)1;
+        if (cells) KKW_HIP(hipMemcpy(out->vertices, src->vertices, cells * 4, hipMemcpyDeviceToHost));
+        if (n) KKW_HIP(hipMemcpy(out->lengths, src->lengths, n * 4, hipMemcpyDeviceToHost));
+        uint64_t steps = 0, trunc = 0;
+        for (size_t i = 0; i < n; i++) {
+            steps += (out->lengths[i] & 0x7fffffffu) - 1;
+            trunc += out->lengths[i] >> 31;
+        }
+        out->steps = steps;
+        out->truncated = trunc;
+        return KKW_OK;
+    });
+}
+
 int kkw_paths_free(kkw_paths *p)
 {
     if (!p) return KKW_OK;

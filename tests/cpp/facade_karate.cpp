@@ -191,6 +191,38 @@ int main(int argc, char **argv)
         for (vertex_id_t v = 0; v < 34; v++) distinct += std::count(want.begin(), want.end(), v) > 0;
         if (distinct < 10) return 17;   // drawn, not walker id modulo |V| alone
     }
+    {
+        // epochs streamed to a device consumer: each epoch's rows copied on the
+        // stream the consumer is handed, while the next epoch walks
+        WalkEngine<real_t, EmptyData> graph;
+        graph.load_graph(34, argv[1], false, GF_Edgelist);
+        WalkConfig wc;
+        wc.set_walk_rate(0.5);
+        wc.set_output_file((out + "_stream_dw").c_str(), true);
+        std::vector<std::vector<uint32_t> > rows_e, lens_e;
+        std::vector<uint32_t> strides;
+        wc.set_device_stream_consumer([&](const kkw_paths &p, void *stream) {
+            rows_e.emplace_back((size_t)p.walker_num * p.stride);
+            lens_e.emplace_back(p.walker_num);
+            strides.push_back(p.stride);
+            if (hipMemcpyAsync(rows_e.back().data(), p.vertices, rows_e.back().size() * 4, hipMemcpyDeviceToHost,
+                               (hipStream_t)stream) != hipSuccess ||
+                hipMemcpyAsync(lens_e.back().data(), p.lengths, lens_e.back().size() * 4, hipMemcpyDeviceToHost,
+                               (hipStream_t)stream) != hipSuccess)
+                throw std::runtime_error("copy failed");
+        });
+        deepwalk(&graph, 34, 12, &wc);   // seed 1, two epochs of 17
+        if (rows_e.size() != 2) return 18;
+        FILE *f = fopen((out + "_stream_dev.0").c_str(), "w");
+        if (!f) return 19;
+        for (size_t e = 0; e < rows_e.size(); e++)
+            for (size_t i = 0; i < lens_e[e].size(); i++) {
+                for (uint32_t k = 0; k < (lens_e[e][i] & 0x7fffffffu); k++)
+                    fprintf(f, k ? " %u" : "%u", rows_e[e][i * strides[e] + k]);
+                fprintf(f, "\n");
+            }
+        fclose(f);
+    }
     printf("facade ok\n");
     return 0;
 }

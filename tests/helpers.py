@@ -71,3 +71,33 @@ def job_desc(job):
 
 __all__ = ["oracle_walk", "rows_equal", "assert_rows_match_oracle", "packed_matches_oracle", "edge_keys",
            "edges_exist", "job_desc", "kk", "O"]
+
+
+class HipRuntime:
+    """The HIP runtime libkkw itself is bound to (found in /proc/self/maps),
+    through ctypes: device memory, pinned host memory and copies for tests
+    that must not bring a second runtime (torch's) into the process."""
+
+    def __init__(self):
+        import ctypes as C
+        with open("/proc/self/maps") as f:
+            paths = {line.split()[-1] for line in f if "libamdhip64.so" in line}
+        assert len(paths) == 1, f"expected the one HIP runtime libkkw uses, found {sorted(paths)}"
+        lib = C.CDLL(paths.pop())
+        P, Z, I = C.c_void_p, C.c_size_t, C.c_int
+        for name, args in (("hipMalloc", [C.POINTER(P), Z]), ("hipFree", [P]), ("hipHostMalloc", [C.POINTER(P), Z, C.c_uint]),
+                           ("hipHostFree", [P]), ("hipMemcpy", [P, P, Z, I]), ("hipMemcpyAsync", [P, P, Z, I, P]),
+                           ("hipStreamSynchronize", [P]), ("hipDeviceSynchronize", [])):
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = I
+        self.C = C
+        self.lib = lib
+
+    def host_alloc(self, nbytes):
+        p = self.C.c_void_p()
+        assert self.lib.hipHostMalloc(self.C.byref(p), max(nbytes, 16), 0) == 0
+        return p.value
+
+    def copy_async_d2h(self, dst, src, nbytes, stream):
+        assert self.lib.hipMemcpyAsync(dst, src, nbytes, 2, stream) == 0

@@ -86,6 +86,15 @@ def test_facade_matches_oracle(tmp_path, karate):
     assert np.array_equal(vm[:, 0], O.vmask_of(rp2, et))
     assert np.array_equal(vm[:, 1], np.diff(rp2.astype(np.int64)))
     assert np.array_equal(vm[:, 2], np.bincount(col2, minlength=34))
+    # epochs streamed to a device stream consumer (and to the output file)
+    paths, lens, _, _ = O.walk(g2, O.DEEPWALK, 34, 12, True, seed=1)
+    got = read_dump(prefix + "_stream_dw.0", True)
+    assert [x[0] for x in got] == list(range(34))
+    for i, (wid, n, verts) in enumerate(got):
+        assert n == lens[i] and verts == paths[i, :lens[i]].tolist(), f"streamed epochs, file, walker {i}"
+    got = read_dump(prefix + "_stream_dev.0", False)
+    for i, (_, n, verts) in enumerate(got):
+        assert verts == paths[i, :lens[i]].tolist(), f"streamed epochs, device copies, walker {i}"
     # lists sorted on the host through csr, then walked on the device
     rp3, col3, w3 = O.csr_from_edges(34, src, dst, w, False, True)
     paths, lens, _, _ = O.walk(O.Graph(rp3, col3, w3), O.DEEPWALK, 34, 12, True, seed=1)
