@@ -491,6 +491,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--calib", action="store_true",
                     help="profiling pass: run the calibration copy only twice")
+    ap.add_argument("--rebuilds", type=int, default=2,
+                    help="rebuild the alias tables this many times and time one persistent launch on each "
+                         "placement (the spread between table placements)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -562,11 +565,15 @@ def main():
 
     # ceiling for independent random 16-byte reads (one 128-byte line each),
     # against which the kernel's own rate of data-dependent random reads is set
-    gather_lps = None
+    gather_lps = gather_small = None
     if not args.calib:
         gather_lps = eng.random_gather_lines_per_s(GATHER_BYTES)
         log(f"random 16-byte reads over {GATHER_BYTES >> 30} GiB: {gather_lps / 1e9:.2f} G lines/s "
             f"({gather_lps * L2_LINE / 1e9:.0f} GB/s of 128-byte lines)")
+        # the same reads over 1 GiB, whose translations all hit the per-CU
+        # translation caches: what the rate is without translation misses
+        gather_small = eng.random_gather_lines_per_s(1 << 30)
+        log(f"random 16-byte reads over 1 GiB: {gather_small / 1e9:.2f} G lines/s")
     accesses = t["row_loads"] + t["trials"] + t["search_lines"]
     access_rate = accesses / kernel_s
 
@@ -589,6 +596,18 @@ def main():
         else:
             traffic_source = (f"none: {tf} was profiled on sources {prof.get('source_digest')}, "
                               f"this run's are {source_digest()}")
+
+    # the same launch on freshly rebuilt tables (new placements in HBM)
+    placements = []
+    for r in range(args.rebuilds):
+        pb.close()
+        eng.prepare()
+        pb = eng.batch([job], kk.MODE_PERSISTENT, kk.BATCH_GRAPH)
+        pb.set_shape(*shape)
+        _, kms_r = time_batch(pb, 1, 1)
+        placements.append(round(kms_r[0], 3))
+    if placements:
+        log(f"persistent launch on rebuilt tables: {placements} ms (first placement {kernel_s * 1e3:.3f} ms)")
 
     cpu = None
     parity = None
@@ -706,6 +725,8 @@ def main():
             "stream_copy_gbs": round(copy_gbs, 1),
             "random_accesses_per_s": round(access_rate, 1),
             "random_gather_lines_per_s": None if gather_lps is None else round(gather_lps, 1),
+            "random_gather_1GiB_lines_per_s": None if gather_small is None else round(gather_small, 1),
+            "placement_kernel_ms": [round(kernel_s * 1e3, 3)] + placements,
             "random_gather_gbs": None if gather_lps is None else round(gather_lps * L2_LINE / 1e9, 1),
             "frac_of_random_gather": None if gather_lps is None else round(access_rate / gather_lps, 4),
         },
