@@ -567,6 +567,11 @@ struct Request {
 
 // Layout of one flight inside the device arena (byte offsets from its base)
 // and inside the pinned staging arena.
+// Rounds of one flight queued on its stream at once: the next round is
+// already there when one ends (no host latency between rounds), and another
+// stream's work waits behind at most this many rounds of a long flight.
+constexpr uint32_t kRoundsAhead = 3;
+
 struct Flight {
     std::vector<std::shared_ptr<Request>> reqs;
     double t_admit = 0, t_enqueued = 0, t_phase1 = 0, t_phase2 = 0, t_retired = 0;   // flight trace
@@ -587,10 +592,16 @@ struct Flight {
     // round_steps steps per launch; the walk is relaunched on the suspended
     // ones until none is left, then the rows are packed
     bool rounds = false;
-    bool packing = false;          // the last round is done: pack and copy-back enqueued
-    int round = 0;
-    size_t o_state = 0, o_list[2] = {0, 0}, o_rcnt = 0;   // resume states, (job, row) lists, counters
-    size_t p_rcnt = 0;             // pinned slot (in the upload mirror) of the suspended count
+    bool packing = false;          // pack and copy-back enqueued: ev says when the flight is done
+    uint32_t round_steps = 0;      // steps per walker per launch (the option, or more for very long walks)
+    uint32_t n_rounds = 0;         // at most this many launches: ceil(longest path / round_steps)
+    uint32_t next_round = 0;       // rounds enqueued
+    uint32_t done_rounds = 0;      // rounds seen complete (their counts read)
+    hipEvent_t round_ev[kRoundsAhead] = {};   // round r records round_ev[r % kRoundsAhead]
+    DevGraph g{};                  // the graph view the flight was launched on (fixed while serving)
+    size_t o_state = 0, o_list[2] = {0, 0};   // resume states, (job, row) lists
+    size_t o_rq = 0, o_rcnt = 0;   // per-round work-queue words, per-round listed counts (zeroed by the upload)
+    size_t p_rcnt = 0;             // pinned: round r's listed count lands at p_rcnt + 8 (r + 1)
 };
 
 struct ServeState {
@@ -623,7 +634,7 @@ static size_t request_dev_bytes(const PreparedJob &pj, bool rounds)
     const size_t n = pj.dj.walker_num;
     return al256(n * pj.stride * 4) + al256(n * 4) + al256(n * 8) * 2 + al256(pj.starts.size() * 4) +
            al256(pj.scheme_len.size() * 4) + al256(pj.scheme_mask.size()) + 1024 +
-           (rounds ? n * (sizeof(ResumeState) + 16) + 1024 : 0);
+           (rounds ? n * (sizeof(ResumeState) + 16) + 1024 + 16 * 1024 + 512 : 0);
 }
 
 // Pinned bytes one request adds to a flight: its share of the upload mirror,
@@ -644,7 +655,7 @@ static void finish_request(ServeState *S, const std::shared_ptr<Request> &r, int
 }
 
 static void enqueue_pack(ServeState *S, Flight &f);
-static void enqueue_round(ServeState *S, Flight &f, const DevGraph &g, uint64_t n_in);
+static bool advance_rounds(ServeState *S, Flight &f);
 
 // Build and launch one flight.  Returns false (and leaves reqs pending) when
 // the device or pinned arena has no room for them yet; *pin_short tells which.  One stream-ordered
@@ -683,8 +694,20 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
     const size_t o_queue = o;   // the walk's work counter, uploaded as zero
     o += 256;
     f.rounds = S->opts.round_steps != 0;
-    f.o_rcnt = o;               // rounds: the two suspended-walker counters; pinned: the count copied back
-    o += 256;
+    if (f.rounds) {
+        // every walker ends within ceil(longest path / round_steps) rounds; a
+        // flight takes at most kMaxRounds launches (its rounds lengthen for
+        // longer walks)
+        constexpr uint64_t kMaxRounds = 1024;
+        uint64_t longest = 0;
+        for (auto &r : f.reqs) longest = std::max<uint64_t>(longest, r->pj.dj.walk_length);
+        f.round_steps = (uint32_t)std::max<uint64_t>(S->opts.round_steps, (longest + kMaxRounds - 1) / kMaxRounds);
+        f.n_rounds = (uint32_t)std::max<uint64_t>(1, (longest + f.round_steps - 1) / f.round_steps);
+        f.o_rq = o;             // uploaded as zero, like the counts after it
+        o += al256((size_t)f.n_rounds * 8);
+        f.o_rcnt = o;
+        o += al256(((size_t)f.n_rounds + 1) * 8);
+    }
     const size_t upload_bytes = o;
     f.o_lens = o;
     const size_t lens_bytes = ((N * 4 + 7) & ~(size_t)7);
@@ -753,8 +776,11 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
     memcpy(H + o_jobs, djs.data(), sizeof(DevJob) * nj);
     memcpy(H + o_pjobs, pjs.data(), sizeof(PackJob) * nj);
     memset(H + o_queue, 0, sizeof(unsigned long long));
-    memset(H + f.o_rcnt, 0, 2 * sizeof(unsigned long long));
-    f.p_rcnt = f.o_rcnt;
+    if (f.rounds) {
+        memset(H + f.o_rq, 0, (size_t)f.n_rounds * 8);
+        memset(H + f.o_rcnt, 0, ((size_t)f.n_rounds + 1) * 8);
+        f.p_rcnt = f.o_rcnt;    // a slot of the upload mirror, free once the upload has been read
+    }
     f.o_jobs = o_jobs;
     f.o_pjobs = o_pjobs;
     f.o_queue = o_queue;
@@ -775,7 +801,9 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
     KKW_HIP(hipMemcpyAsync(D, H, upload_bytes, hipMemcpyHostToDevice, st));
     KKW_HIP(hipEventCreateWithFlags(&f.ev, hipEventDisableTiming));
     if (f.rounds && chunk_acc) {
-        enqueue_round(S, f, g, 0);
+        for (auto &e : f.round_ev) KKW_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        f.g = g;
+        advance_rounds(S, f);   // the first rounds
         return true;
     }
     if (chunk_acc) {
@@ -812,55 +840,68 @@ static void enqueue_pack(ServeState *S, Flight &f)
     f.packing = true;
 }
 
-// One round of a flight walked in rounds: round r continues the walkers
-// listed by round r - 1 (all of them in round 0), lists the ones it suspends
-// and copies their count to the pinned slot; the event says when it is there.
-static void enqueue_round(ServeState *S, Flight &f, const DevGraph &g, uint64_t n_in)
+// Round r of a flight walked in rounds: continue the walkers round r - 1
+// listed (all of them in round 0), reading their count on the device, list
+// the ones it suspends, and copy that count to the round's pinned slot; the
+// round's event says when it is there.
+static void enqueue_round(ServeState *S, Flight &f, uint32_t r)
 {
     hipStream_t st = S->streams[f.sidx];
     char *D = S->dev_base + f.dev_off;
     char *H = S->pin->base + f.pin_off;
     const uint32_t chunk = 64;
-    const int out = (f.round + 1) & 1;
     unsigned long long *cnt = (unsigned long long *)(D + f.o_rcnt);
-    launch_fill_u64(cnt + out, 0ull, 1, st);
-    launch_fill_u64((unsigned long long *)(D + f.o_queue), 0ull, 1, st);
+    const uint64_t blocks_needed = (f.chunks * 64 + 255) / 256;   // round 0 has the most work
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)S->eng->cu_count * 8, blocks_needed));
     RoundArgs R;
-    R.round_steps = S->opts.round_steps;
-    R.in = f.round == 0 ? nullptr : (const uint32_t *)(D + f.o_list[f.round & 1]);
-    R.n_in = f.round == 0 ? 0 : n_in;
-    R.out = (uint32_t *)(D + f.o_list[out]);
-    R.n_out = cnt + out;
-    const uint64_t chunks = f.round == 0 ? f.chunks : (n_in + chunk - 1) / chunk;
-    const uint64_t blocks_needed = (chunks * 64 + 255) / 256;
-    const int grid =
-        (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)S->eng->cu_count * 8, blocks_needed));
-    launch_walk_rounds(g, (const DevJob *)(D + f.o_jobs), (int)f.reqs.size(), chunks, chunk,
-                       (unsigned long long *)(D + f.o_queue), R, grid, 256, st);
+    R.round_steps = f.round_steps;
+    R.in = r == 0 ? nullptr : (const uint32_t *)(D + f.o_list[r & 1]);
+    R.n_in = cnt + r;
+    R.out = (uint32_t *)(D + f.o_list[(r + 1) & 1]);
+    R.n_out = cnt + r + 1;
+    launch_walk_rounds(f.g, (const DevJob *)(D + f.o_jobs), (int)f.reqs.size(), f.chunks, chunk,
+                       (unsigned long long *)(D + f.o_rq) + r, R, grid, 256, st);
     KKW_HIP(hipGetLastError());
-    KKW_HIP(hipMemcpyAsync(H + f.p_rcnt, cnt + out, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    KKW_HIP(hipEventRecord(f.ev, st));
+    KKW_HIP(hipMemcpyAsync(H + f.p_rcnt + 8 * ((size_t)r + 1), cnt + r + 1, 8, hipMemcpyDeviceToHost, st));
+    KKW_HIP(hipEventRecord(f.round_ev[r % kRoundsAhead], st));
 }
 
-// A flight walked in rounds whose round has completed: relaunch on the
-// suspended walkers, or pack when none is left.  Returns true when the flight
-// has more work on the device (not ready to retire).
+// A flight walked in rounds: note the rounds that have completed, pack once
+// one leaves no walker (rounds still queued behind it find nothing to do),
+// and keep up to kRoundsAhead rounds queued on the flight's stream.  A walker
+// at its step cap is never suspended, so the n_rounds-th round ends every
+// walker; a count left after it is a device error.  Returns true when
+// anything moved.
 static bool advance_rounds(ServeState *S, Flight &f)
 {
-    if (!f.rounds || f.packing || !f.chunks) return false;
-    const uint64_t n = *(volatile const uint64_t *)(S->pin->base + f.pin_off + f.p_rcnt);
-    if (n == 0) {
-        enqueue_pack(S, f);
-        return true;
+    const char *H = S->pin->base + f.pin_off;
+    bool moved = false;
+    while (f.done_rounds < f.next_round &&
+           hipEventQuery(f.round_ev[f.done_rounds % kRoundsAhead]) == hipSuccess) {
+        const uint64_t left = *(volatile const uint64_t *)(H + f.p_rcnt + 8 * ((size_t)f.done_rounds + 1));
+        f.done_rounds++;
+        moved = true;
+        if (left == 0) {
+            enqueue_pack(S, f);
+            return true;
+        }
+        if (f.done_rounds == f.n_rounds) throw Error(KKW_E_DEVICE, "serving rounds left walkers unfinished");
     }
-    DevGraph g;
-    {
-        std::lock_guard<std::recursive_mutex> lk(S->eng->mu);
-        g = S->eng->gs.view();
+    while (f.next_round < f.n_rounds && f.next_round - f.done_rounds < kRoundsAhead) {
+        enqueue_round(S, f, f.next_round++);
+        moved = true;
     }
-    f.round++;
-    enqueue_round(S, f, g, n);
-    return true;
+    return moved;
+}
+
+static void destroy_events(Flight &f)
+{
+    if (f.ev) (void)hipEventDestroy(f.ev);
+    f.ev = nullptr;
+    for (auto &e : f.round_ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
 }
 
 // The flight has completed: hand each request its paths, in place in the
@@ -903,8 +944,8 @@ static void flight_retire(ServeState *S, Flight &f)
         finish_request(S, r, 0);
     }
     S->dev.release(f.dev_off, f.dev_bytes);
-    (void)hipEventDestroy(f.ev);
-    f.ev = nullptr;
+    f.dev_bytes = 0;
+    destroy_events(f);
 }
 
 // Copy finished results that still point into pinned memory to heap memory
@@ -953,10 +994,7 @@ static void release_flight(ServeState *S, Flight &f)
         S->pin->release(f.pin_off, f.pin_bytes);
         f.pin_bytes = 0;
     }
-    if (f.ev) {
-        (void)hipEventDestroy(f.ev);
-        f.ev = nullptr;
-    }
+    destroy_events(f);
 }
 
 static void serve_loop(ServeState *S)
@@ -968,19 +1006,23 @@ static void serve_loop(ServeState *S)
     uint64_t walkers_in_flight = 0;
     for (;;) {
         bool progressed = false;
-        // 1. advance flights whose copies have landed
+        // 1. advance flights: queue their next rounds, retire the ones whose
+        //    copies have landed
         for (auto it = flights.begin(); it != flights.end();) {
             Flight &f = *it;
-            if (hipEventQuery(f.ev) != hipSuccess) {
-                ++it;
-                continue;
-            }
-            progressed = true;
             try {
-                if (advance_rounds(S, f)) {   // another round, or the packing, is on its way
+                if (f.rounds && !f.packing) {   // rounds still to see end (or to enqueue)
+                    if (advance_rounds(S, f)) progressed = true;
+                    if (!f.packing) {
+                        ++it;
+                        continue;
+                    }
+                }
+                if (hipEventQuery(f.ev) != hipSuccess) {
                     ++it;
                     continue;
                 }
+                progressed = true;
                 f.t_phase1 = f.t_phase2 = now_ms();
                 flight_retire(S, f);
                 f.t_retired = now_ms();
@@ -994,6 +1036,7 @@ static void serve_loop(ServeState *S)
                     S->trace.push_back(line);
                 }
             } catch (const std::exception &ex) {
+                progressed = true;
                 const int code = dynamic_cast<const Error *>(&ex) ? static_cast<const Error &>(ex).code : KKW_E_NOMEM;
                 (void)hipStreamSynchronize(S->streams[f.sidx]);   // nothing of it may still run
                 for (auto &r : f.reqs)

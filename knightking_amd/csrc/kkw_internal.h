@@ -82,12 +82,14 @@ void launch_walk_persistent_mixed(const DevGraph &g, const DevJob *jobs, int n_j
 // Serving rounds: every walker takes at most round_steps steps in one launch;
 // walkers still going are suspended (state saved, (job, row) appended to
 // `out`, counted in *n_out, which must be zero at launch).  The first round
-// (in == null) takes chunks of the jobs' walker ranges; later ones take
-// chunks of the n_in (job, row) pairs in `in`.
+// (in == null) takes the n_chunks chunks of the jobs' walker ranges; a later
+// one takes chunks of the *n_in (job, row) pairs in `in` -- a count the
+// previous round left on the device, so rounds follow each other on the
+// stream without the host.
 struct RoundArgs {
     uint32_t round_steps;
     const uint32_t *in;
-    uint64_t n_in;
+    const unsigned long long *n_in;
     uint32_t *out;
     unsigned long long *n_out;
 };

@@ -589,6 +589,12 @@ __global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *j
     uint32_t pool_job = 0;
     uint64_t pool_next = 0, pool_end = 0;
     bool drained = false;
+    // a continuation round: the walkers the previous round listed.  The count
+    // was made by the previous launch's atomics: read it as an atomic (from
+    // L2), never as a plain load the compiler may turn into a scalar-cache read
+    const uint64_t n_in =
+        R.in ? __hip_atomic_load(R.n_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+    if (R.in) n_chunks = (n_in + chunk - 1) / chunk;
     for (;;) {
         uint64_t need = __ballot(!active);
         while (need && !drained) {
@@ -603,7 +609,7 @@ __global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *j
                 if (R.in) {
                     pool_next = c * chunk;
                     const uint64_t e = pool_next + chunk;
-                    pool_end = e < R.n_in ? e : R.n_in;
+                    pool_end = e < n_in ? e : n_in;
                 } else {
                     const uint32_t j = find_job(jobs, n_jobs, c);
                     pool_job = j;
@@ -633,8 +639,10 @@ __global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *j
         bool suspend = false;
         if (active) {
             const DevJob &J = jobs[s.job];
+            // a walker at its step cap ends on its next call: it is never
+            // suspended, so ceil(cap / round_steps) rounds end every walker
             if (advance<KIND_ANY, false, false>(g, J, s, ls)) active = false;
-            else suspend = s.trial == 0 && s.step - round_start >= R.round_steps;
+            else suspend = s.trial == 0 && s.step - round_start >= R.round_steps && s.step < J.walk_length;
         }
         const uint64_t sm = __ballot(suspend);
         if (sm) {
@@ -711,11 +719,28 @@ void launch_random_gather(const void *buf, uint64_t bytes, uint32_t threads, uin
 }
 
 // ----------------------------------------------------- packed output ----
-// lengths (without the cap flag) as 64-bit counts, for the offset scan
-__global__ void k_lens_to_counts(const uint32_t *__restrict__ lens, uint64_t n, uint64_t *__restrict__ cnt)
+// The job that owns flight row i: the last job whose row0 is <= i.
+__device__ __forceinline__ int pack_job_of(const PackJob *__restrict__ jobs, int n_jobs, uint64_t i)
+{
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].row0 <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// lengths (without the cap flag) as 64-bit counts, for the offset scan.  A
+// count never exceeds its row (the job's stride), so the packed output stays
+// within rows x stride entries whatever a length word holds.
+__global__ void k_lens_to_counts(const PackJob *__restrict__ jobs, int n_jobs, const uint32_t *__restrict__ lens,
+                                 uint64_t n, uint64_t *__restrict__ cnt)
 {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) cnt[i] = lens[i] & 0x7fffffffu;
+    if (i >= n) return;
+    const uint32_t len = lens[i] & 0x7fffffffu, cap = jobs[pack_job_of(jobs, n_jobs, i)].stride;
+    cnt[i] = len < cap ? len : cap;
 }
 
 // Copy every walker's row into the packed array at its offset.  Rows of a
@@ -743,13 +768,8 @@ __global__ void __launch_bounds__(256) k_pack_rows(const PackJob *__restrict__ j
     const uint32_t cnt = n - i0 < 64 ? (uint32_t)(n - i0) : 64u;
     if (lane < cnt) {
         const uint64_t i = i0 + lane;
-        int lo = 0, hi = n_jobs - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (jobs[mid].row0 <= i) lo = mid;
-            else hi = mid - 1;
-        }
-        s_src[wv][lane] = jobs[lo].paths + (i - jobs[lo].row0) * jobs[lo].stride;
+        const int j = pack_job_of(jobs, n_jobs, i);
+        s_src[wv][lane] = jobs[j].paths + (i - jobs[j].row0) * jobs[j].stride;
     }
     s_off[wv][lane] = offs[i0 + (lane < cnt ? lane : cnt)];
     if (lane == 0) s_off[wv][64] = offs[i0 + cnt];
@@ -772,7 +792,7 @@ void launch_pack(const PackJob *jobs, int n_jobs, const uint32_t *lens, uint64_t
 {
     if (n == 0) return;
     unsigned g = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_lens_to_counts, dim3(g), dim3(256), 0, s, lens, n, cnt);
+    hipLaunchKernelGGL(k_lens_to_counts, dim3(g), dim3(256), 0, s, jobs, n_jobs, lens, n, cnt);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long *)offs, 0ull, 1);
     size_t tb = scan_bytes;
     (void)hipcub::DeviceScan::InclusiveSum(scan_tmp, tb, cnt, offs + 1, n, s);
