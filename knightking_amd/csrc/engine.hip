@@ -599,7 +599,7 @@ struct Flight {
     uint32_t done_rounds = 0;      // rounds seen complete (their counts read)
     hipEvent_t round_ev[kRoundsAhead] = {};   // round r records round_ev[r % kRoundsAhead]
     DevGraph g{};                  // the graph view the flight was launched on (fixed while serving)
-    size_t o_state = 0, o_list[2] = {0, 0};   // resume states, (job, row) lists
+    size_t o_list[2] = {0, 0};     // suspended-walker records, one list per round parity
     size_t o_rq = 0, o_rcnt = 0;   // per-round work-queue words, per-round listed counts (zeroed by the upload)
     size_t p_rcnt = 0;             // pinned: round r's listed count lands at p_rcnt + 8 (r + 1)
 };
@@ -627,14 +627,14 @@ struct ServeState {
 static size_t al256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 // Device bytes one request adds to a flight: rows, lengths, counts, offsets,
-// starts and scheme tables, and with rounds its resume states and its share
-// of the two suspended-walker lists.
+// starts and scheme tables, and with rounds its share of the two
+// suspended-walker record lists and of the per-round counters.
 static size_t request_dev_bytes(const PreparedJob &pj, bool rounds)
 {
     const size_t n = pj.dj.walker_num;
     return al256(n * pj.stride * 4) + al256(n * 4) + al256(n * 8) * 2 + al256(pj.starts.size() * 4) +
            al256(pj.scheme_len.size() * 4) + al256(pj.scheme_mask.size()) + 1024 +
-           (rounds ? n * (sizeof(ResumeState) + 16) + 1024 + 16 * 1024 + 512 : 0);
+           (rounds ? 2 * n * ROUND_REC_PLANES * 16 + 512 + 16 * 1024 + 512 : 0);
 }
 
 // Pinned bytes one request adds to a flight: its share of the upload mirror,
@@ -727,12 +727,10 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
         packed_max += (size_t)pj.dj.walker_num * pj.stride;
     }
     if (f.rounds) {
-        f.o_state = o;
-        o += al256(N * sizeof(ResumeState));
         f.o_list[0] = o;
-        o += al256(N * 8);
+        o += al256(N * ROUND_REC_PLANES * 16);
         f.o_list[1] = o;
-        o += al256(N * 8);
+        o += al256(N * ROUND_REC_PLANES * 16);
     }
     f.dev_bytes = o;
     // ---- pinned: [upload mirror][lens | offs][packed payload, at most rows x stride]
@@ -769,7 +767,6 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
         }
         d.chunk_begin = chunk_acc;
         chunk_acc += (d.walker_num + chunk - 1) / chunk;
-        d.resume = f.rounds ? (ResumeState *)(D + f.o_state) + f.row0[j] : nullptr;
         djs[j] = d;
         pjs[j] = PackJob{f.row0[j], d.walker_num, pj.stride, d.paths};
     }
@@ -855,9 +852,10 @@ static void enqueue_round(ServeState *S, Flight &f, uint32_t r)
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)S->eng->cu_count * 8, blocks_needed));
     RoundArgs R;
     R.round_steps = f.round_steps;
-    R.in = r == 0 ? nullptr : (const uint32_t *)(D + f.o_list[r & 1]);
+    R.cap = f.walkers;
+    R.in = r == 0 ? nullptr : (const uint4 *)(D + f.o_list[r & 1]);
     R.n_in = cnt + r;
-    R.out = (uint32_t *)(D + f.o_list[(r + 1) & 1]);
+    R.out = (uint4 *)(D + f.o_list[(r + 1) & 1]);
     R.n_out = cnt + r + 1;
     launch_walk_rounds(f.g, (const DevJob *)(D + f.o_jobs), (int)f.reqs.size(), f.chunks, chunk,
                        (unsigned long long *)(D + f.o_rq) + r, R, grid, 256, st);

@@ -94,14 +94,6 @@ struct DevJob {
     uint32_t *paths;
     uint32_t *lens;
     uint64_t chunk_begin;      // first work-queue chunk of this job (persistent mode)
-    struct ResumeState *resume;   // serving rounds: [walker_num] states of suspended walkers, else null
-};
-
-// A walker suspended between two serving rounds, at a step boundary: where it
-// is, where it came from, its step and (metapath) scheme << 8 | state.  Its
-// path so far is in its row, the current 16-byte group included.
-struct ResumeState {
-    uint32_t cur, prev, step, meta;
 };
 
 struct DevStats {

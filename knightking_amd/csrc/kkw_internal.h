@@ -80,17 +80,20 @@ void launch_walk_persistent_mixed(const DevGraph &g, const DevJob *jobs, int n_j
                                   uint32_t chunk, unsigned long long *queue, bool stats, DevStats *st,
                                   int grid, int block, bool lines, hipStream_t stream);
 // Serving rounds: every walker takes at most round_steps steps in one launch;
-// walkers still going are suspended (state saved, (job, row) appended to
-// `out`, counted in *n_out, which must be zero at launch).  The first round
-// (in == null) takes the n_chunks chunks of the jobs' walker ranges; a later
-// one takes chunks of the *n_in (job, row) pairs in `in` -- a count the
-// previous round left on the device, so rounds follow each other on the
-// stream without the host.
+// walkers still going are suspended: appended to `out` as a record of
+// ROUND_REC_PLANES 16-byte words (plane p of record k at out[p * cap + k],
+// so consecutive lanes touch consecutive words) and counted in *n_out, which
+// must be zero at launch.  The first round (in == null) takes the n_chunks
+// chunks of the jobs' walker ranges; a later one takes chunks of the *n_in
+// records in `in` -- a count the previous round left on the device, so
+// rounds follow each other on the stream without the host.
+constexpr uint32_t ROUND_REC_PLANES = 5;
 struct RoundArgs {
     uint32_t round_steps;
-    const uint32_t *in;
+    uint64_t cap;                  // records per plane (the flight's walkers)
+    const uint4 *in;
     const unsigned long long *n_in;
-    uint32_t *out;
+    uint4 *out;
     unsigned long long *n_out;
 };
 void launch_walk_rounds(const DevGraph &g, const DevJob *jobs, int n_jobs, uint64_t n_chunks, uint32_t chunk,

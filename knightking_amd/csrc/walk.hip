@@ -542,39 +542,49 @@ __global__ void __launch_bounds__(256) k_walk_persistent(DevGraph g, DevJob J0, 
 // that HBM counters and the achievable bandwidth are calibrated against.
 // four 16-byte loads in flight per lane, then their stores
 // ------------------------------------------------------ serving rounds ----
-// A walker back from a suspension: state from its resume slot, rows of the
-// current and previous vertices from row_ptr, the current 16-byte path group
-// from its row (stored whole at the suspension).
-__device__ __forceinline__ void resume_walker(const DevGraph &g, const DevJob &J, uint32_t job, uint32_t idx,
+// A suspended walker's record: everything its next step needs -- job, row,
+// where it is and came from with both lists' bounds, step, scheme and state,
+// the type mask of its vertex, and its current 16-byte path group (not yet
+// in its row) -- so a resume reads nothing at random.  Plane p of record k is
+// rec[p * cap + k].
+__device__ __forceinline__ void save_walker(const Walker &s, uint4 *rec, uint64_t cap, uint64_t k)
+{
+    rec[k] = make_uint4(s.job, s.idx, s.cur, s.prev);
+    rec[cap + k] = make_uint4(s.step, (s.scheme << 8) | s.state, s.cvm, (uint32_t)(s.end - s.beg));
+    rec[2 * cap + k] = make_uint4((uint32_t)s.beg, (uint32_t)(s.beg >> 32), (uint32_t)s.pbeg, (uint32_t)(s.pbeg >> 32));
+    rec[3 * cap + k] = make_uint4((uint32_t)(s.pend - s.pbeg), 0u, 0u, 0u);
+    rec[4 * cap + k] = make_uint4(s.pg.q0, s.pg.q1, s.pg.q2, s.pg.q3);
+}
+
+__device__ __forceinline__ void resume_walker(const DevJob *jobs, const uint4 *rec, uint64_t cap, uint64_t k,
                                               Walker &s)
 {
-    const ResumeState r = J.resume[idx];
-    s.job = job;
-    s.idx = idx;
-    s.wid = J.walker_begin + idx;
-    s.cur = r.cur;
-    s.prev = r.prev;
-    s.step = r.step;
+    const uint4 a = rec[k], b = rec[cap + k], c = rec[2 * cap + k], d = rec[3 * cap + k], e = rec[4 * cap + k];
+    s.job = a.x;
+    s.idx = a.y;
+    s.cur = a.z;
+    s.prev = a.w;
+    s.wid = jobs[s.job].walker_begin + s.idx;
+    s.step = b.x;
     s.trial = 0;
-    s.scheme = r.meta >> 8;
-    s.state = r.meta & 0xffu;
-    s.beg = g.row_ptr[s.cur];
-    s.end = g.row_ptr[s.cur + 1];
-    s.pbeg = s.step ? g.row_ptr[s.prev] : 0;
-    s.pend = s.step ? g.row_ptr[s.prev + 1] : 0;
-    s.cvm = J.kind == KIND_METAPATH ? g.vmask[s.cur] : 0u;
+    s.scheme = b.y >> 8;
+    s.state = b.y & 0xffu;
+    s.cvm = b.z;
+    s.beg = (uint64_t)c.x | ((uint64_t)c.y << 32);
+    s.end = s.beg + b.w;
+    s.pbeg = (uint64_t)c.z | ((uint64_t)c.w << 32);
+    s.pend = s.pbeg + d.x;
     s.lines = false;
-    const uint4 q = *reinterpret_cast<const uint4 *>(J.paths + (uint64_t)idx * J.stride + (s.step & ~3u));
-    s.pg.q0 = q.x;
-    s.pg.q1 = q.y;
-    s.pg.q2 = q.z;
-    s.pg.q3 = q.w;
+    s.pg.q0 = e.x;
+    s.pg.q1 = e.y;
+    s.pg.q2 = e.z;
+    s.pg.q3 = e.w;
 }
 
 // The mixed-kind persistent loop of k_walk_persistent, bounded to
 // R.round_steps steps per walker: a walker that reaches the bound at a step
-// boundary stores its current path group, saves its state and is listed
-// (one atomic per wave) for the next round.  Draws are keyed by (walker,
+// boundary is saved as a record in the next round's list (one atomic per
+// wave; a wave's records are consecutive).  Draws are keyed by (walker,
 // step, trial) and trials restart at every step, so paths are the same with
 // or without rounds.
 __global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *jobs, int n_jobs, uint64_t n_chunks,
@@ -622,9 +632,7 @@ __global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *j
             const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
             if (!active && rank < avail) {
                 if (R.in) {
-                    const uint64_t k = pool_next + rank;
-                    const uint32_t j = R.in[2 * k], idx = R.in[2 * k + 1];
-                    resume_walker(g, jobs[j], j, idx, s);
+                    resume_walker(jobs, R.in, R.cap, pool_next + rank, s);
                 } else {
                     init_walker<KIND_ANY>(g, jobs[pool_job], pool_job, (uint32_t)(pool_next + rank), s, ls, false);
                 }
@@ -651,12 +659,7 @@ __global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *j
             if (lane == leader) base = atomicAdd(R.n_out, (unsigned long long)__popcll(sm));
             base = __shfl(base, (int)leader, 64);
             if (suspend) {
-                const DevJob &J = jobs[s.job];
-                s.pg.store(J.paths + (uint64_t)s.idx * J.stride, s.step);
-                J.resume[s.idx] = ResumeState{s.cur, s.prev, s.step, (s.scheme << 8) | s.state};
-                const uint64_t pos = base + (uint64_t)__popcll(sm & lt_mask);
-                R.out[2 * pos] = s.job;
-                R.out[2 * pos + 1] = s.idx;
+                save_walker(s, R.out, R.cap, base + (uint64_t)__popcll(sm & lt_mask));
                 active = false;
             }
         }
