@@ -664,7 +664,7 @@ def main():
         rp, _, _, _ = eng.download(weights=False)
         serving = serve_latency(eng, rp, [200, 500, 1000, 2000], args.serve_seconds)
         serving["mixed"] = [serve_mixed(eng, rp, r, args.serve_seconds, round_steps=k)
-                            for r, k in ((2000, 0), (8000, 0), (8000, 8))]
+                            for r, k in ((2000, 0), (2000, 8), (8000, 0), (8000, 8))]
 
     line = {
         "metric": "walker steps/s, node2vec RMAT-24 1 GPU (batched vs plain); p99 PPR query latency",

@@ -599,8 +599,11 @@ struct Flight {
     uint32_t done_rounds = 0;      // rounds seen complete (their counts read)
     hipEvent_t round_ev[kRoundsAhead] = {};   // round r records round_ev[r % kRoundsAhead]
     DevGraph g{};                  // the graph view the flight was launched on (fixed while serving)
+    int round_grid = 0;            // blocks of every round's launch
+    uint64_t rec_cap = 0;          // slots of a suspended-walker list: walkers + waves x ROUND_SLOTS
     size_t o_list[2] = {0, 0};     // suspended-walker records, one list per round parity
-    size_t o_rq = 0, o_rcnt = 0;   // per-round work-queue words, per-round listed counts (zeroed by the upload)
+    size_t o_rq = 0;               // per-round work-queue words (zeroed by the upload)
+    size_t o_rcnt = 0;             // per-round slot counts [n_rounds + 1], then listed-walker counts [n_rounds + 1]
     size_t p_rcnt = 0;             // pinned: round r's listed count lands at p_rcnt + 8 (r + 1)
 };
 
@@ -626,6 +629,21 @@ struct ServeState {
 
 static size_t al256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+// Blocks of a serving launch over `chunks` chunks of 64 walkers: one lane per
+// walker up to eight 256-thread blocks per CU.
+static int round_grid(const kkw_engine *eng, uint64_t chunks)
+{
+    const uint64_t blocks_needed = (chunks * 64 + 255) / 256;
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)eng->cu_count * 8, blocks_needed));
+}
+
+// Device bytes of a flight walked in rounds beyond its requests' shares: the
+// slots its launches' waves may leave unused in each of the two lists.
+static size_t rounds_flight_bytes(const kkw_engine *eng)
+{
+    return 2 * (size_t)eng->cu_count * 8 * 4 * ROUND_SLOTS * ROUND_REC_PLANES * 16 + 4096;
+}
+
 // Device bytes one request adds to a flight: rows, lengths, counts, offsets,
 // starts and scheme tables, and with rounds its share of the two
 // suspended-walker record lists and of the per-round counters.
@@ -634,7 +652,7 @@ static size_t request_dev_bytes(const PreparedJob &pj, bool rounds)
     const size_t n = pj.dj.walker_num;
     return al256(n * pj.stride * 4) + al256(n * 4) + al256(n * 8) * 2 + al256(pj.starts.size() * 4) +
            al256(pj.scheme_len.size() * 4) + al256(pj.scheme_mask.size()) + 1024 +
-           (rounds ? 2 * n * ROUND_REC_PLANES * 16 + 512 + 16 * 1024 + 512 : 0);
+           (rounds ? 2 * al256(n * ROUND_REC_PLANES * 16) + 2 * 1024 * 3 * 8 : 0);
 }
 
 // Pinned bytes one request adds to a flight: its share of the upload mirror,
@@ -703,10 +721,14 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
         for (auto &r : f.reqs) longest = std::max<uint64_t>(longest, r->pj.dj.walk_length);
         f.round_steps = (uint32_t)std::max<uint64_t>(S->opts.round_steps, (longest + kMaxRounds - 1) / kMaxRounds);
         f.n_rounds = (uint32_t)std::max<uint64_t>(1, (longest + f.round_steps - 1) / f.round_steps);
+        uint64_t chunks = 0;
+        for (auto &r : f.reqs) chunks += (r->pj.dj.walker_num + 63) / 64;
+        f.round_grid = round_grid(eng, chunks);
+        f.rec_cap = N + (uint64_t)f.round_grid * 4 * ROUND_SLOTS;
         f.o_rq = o;             // uploaded as zero, like the counts after it
         o += al256((size_t)f.n_rounds * 8);
         f.o_rcnt = o;
-        o += al256(((size_t)f.n_rounds + 1) * 8);
+        o += al256(2 * ((size_t)f.n_rounds + 1) * 8);
     }
     const size_t upload_bytes = o;
     f.o_lens = o;
@@ -728,9 +750,9 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
     }
     if (f.rounds) {
         f.o_list[0] = o;
-        o += al256(N * ROUND_REC_PLANES * 16);
+        o += al256(f.rec_cap * ROUND_REC_PLANES * 16);
         f.o_list[1] = o;
-        o += al256(N * ROUND_REC_PLANES * 16);
+        o += al256(f.rec_cap * ROUND_REC_PLANES * 16);
     }
     f.dev_bytes = o;
     // ---- pinned: [upload mirror][lens | offs][packed payload, at most rows x stride]
@@ -775,8 +797,8 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
     memset(H + o_queue, 0, sizeof(unsigned long long));
     if (f.rounds) {
         memset(H + f.o_rq, 0, (size_t)f.n_rounds * 8);
-        memset(H + f.o_rcnt, 0, ((size_t)f.n_rounds + 1) * 8);
-        f.p_rcnt = f.o_rcnt;    // a slot of the upload mirror, free once the upload has been read
+        memset(H + f.o_rcnt, 0, 2 * ((size_t)f.n_rounds + 1) * 8);
+        f.p_rcnt = f.o_rcnt;    // slots of the upload mirror, free once the upload has been read
     }
     f.o_jobs = o_jobs;
     f.o_pjobs = o_pjobs;
@@ -804,8 +826,7 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
         return true;
     }
     if (chunk_acc) {
-        uint64_t blocks_needed = (chunk_acc * 64 + 255) / 256;
-        int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)eng->cu_count * 8, blocks_needed));
+        const int grid = round_grid(eng, chunk_acc);
         // serving rows stay 16-byte groups (short PPR walks; full occupancy)
         launch_walk_persistent_mixed(g, (const DevJob *)(D + o_jobs), nj, chunk_acc, chunk,
                                      (unsigned long long *)(D + o_queue), false, nullptr, grid, 256, false, st);
@@ -848,19 +869,19 @@ static void enqueue_round(ServeState *S, Flight &f, uint32_t r)
     char *H = S->pin->base + f.pin_off;
     const uint32_t chunk = 64;
     unsigned long long *cnt = (unsigned long long *)(D + f.o_rcnt);
-    const uint64_t blocks_needed = (f.chunks * 64 + 255) / 256;   // round 0 has the most work
-    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)S->eng->cu_count * 8, blocks_needed));
+    unsigned long long *live = cnt + f.n_rounds + 1;
     RoundArgs R;
     R.round_steps = f.round_steps;
-    R.cap = f.walkers;
+    R.cap = f.rec_cap;
     R.in = r == 0 ? nullptr : (const uint4 *)(D + f.o_list[r & 1]);
     R.n_in = cnt + r;
     R.out = (uint4 *)(D + f.o_list[(r + 1) & 1]);
     R.n_out = cnt + r + 1;
+    R.n_live = live + r + 1;
     launch_walk_rounds(f.g, (const DevJob *)(D + f.o_jobs), (int)f.reqs.size(), f.chunks, chunk,
-                       (unsigned long long *)(D + f.o_rq) + r, R, grid, 256, st);
+                       (unsigned long long *)(D + f.o_rq) + r, R, f.round_grid, 256, st);
     KKW_HIP(hipGetLastError());
-    KKW_HIP(hipMemcpyAsync(H + f.p_rcnt + 8 * ((size_t)r + 1), cnt + r + 1, 8, hipMemcpyDeviceToHost, st));
+    KKW_HIP(hipMemcpyAsync(H + f.p_rcnt + 8 * ((size_t)r + 1), live + r + 1, 8, hipMemcpyDeviceToHost, st));
     KKW_HIP(hipEventRecord(f.round_ev[r % kRoundsAhead], st));
 }
 
@@ -1059,10 +1080,18 @@ static void serve_loop(ServeState *S)
             if (S->stop && S->pending.empty() && flights.empty()) break;
             if (sidx >= 0) {
                 size_t room = S->dev.largest();
-                size_t used = 4096;
+                const size_t flight_extra = S->opts.round_steps ? rounds_flight_bytes(eng) : 0;
+                size_t used = 4096 + flight_extra;
                 const size_t pin_room = S->pin->largest();
                 size_t pin_used = 4096;
                 uint64_t walkers = walkers_in_flight;
+                // one flight takes at most a quarter of the walker budget (a
+                // larger request goes alone): behind a backlog the four streams
+                // then hold four flights, and one's packing overlaps the others'
+                // walks, where one flight holding the whole budget would leave
+                // the device idle while it packs
+                const uint64_t flight_cap = std::max<uint64_t>(1, S->opts.max_walkers_in_flight / 4);
+                uint64_t flight_walkers = 0;
                 while (!S->pending.empty()) {
                     auto r = S->pending.front();
                     if (r->cancelled) {
@@ -1073,7 +1102,7 @@ static void serve_loop(ServeState *S)
                         continue;
                     }
                     size_t need = request_dev_bytes(r->pj, S->opts.round_steps != 0) + 2 * pack_scan_bytes(r->pj.dj.walker_num + 1);
-                    if (need + 4096 > S->dev.cap || request_pin_bytes(r->pj) + 4096 > S->pin->a.cap) {
+                    if (need + 4096 + flight_extra > S->dev.cap || request_pin_bytes(r->pj) + 4096 > S->pin->a.cap) {
                         S->pending.pop_front();   // can never fit the arenas
                         r->status = KKW_E_NOMEM;
                         r->t_done = now_ms();
@@ -1085,7 +1114,8 @@ static void serve_loop(ServeState *S)
                     bool fits_pin = pin_used + pneed <= pin_room;   // worst-case payloads of the whole flight
                     bool fits_slots = walkers + r->pj.dj.walker_num <= S->opts.max_walkers_in_flight ||
                                       (walkers == 0 && admit.empty());
-                    if (!fits_mem || !fits_pin || !fits_slots || admit.size() >= 1024) {
+                    const bool fits_flight = flight_walkers + r->pj.dj.walker_num <= flight_cap || admit.empty();
+                    if (!fits_mem || !fits_pin || !fits_slots || !fits_flight || admit.size() >= 1024) {
                         pin_blocked = admit.empty() && !fits_pin;
                         break;
                     }
@@ -1094,6 +1124,7 @@ static void serve_loop(ServeState *S)
                     used += need;
                     pin_used += pneed;
                     walkers += r->pj.dj.walker_num;
+                    flight_walkers += r->pj.dj.walker_num;
                 }
             }
             if (admit.empty() && flights.empty() && !pin_blocked) {

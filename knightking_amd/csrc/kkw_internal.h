@@ -80,21 +80,26 @@ void launch_walk_persistent_mixed(const DevGraph &g, const DevJob *jobs, int n_j
                                   uint32_t chunk, unsigned long long *queue, bool stats, DevStats *st,
                                   int grid, int block, bool lines, hipStream_t stream);
 // Serving rounds: every walker takes at most round_steps steps in one launch;
-// walkers still going are suspended: appended to `out` as a record of
+// walkers still going are suspended: written to `out` as a record of
 // ROUND_REC_PLANES 16-byte words (plane p of record k at out[p * cap + k],
-// so consecutive lanes touch consecutive words) and counted in *n_out, which
-// must be zero at launch.  The first round (in == null) takes the n_chunks
-// chunks of the jobs' walker ranges; a later one takes chunks of the *n_in
-// records in `in` -- a count the previous round left on the device, so
-// rounds follow each other on the stream without the host.
+// so consecutive lanes touch consecutive words).  Each wave reserves slots
+// ROUND_SLOTS at a time from *n_out and marks the ones it leaves unused as
+// holes; *n_live counts the walkers listed (both zero at launch).  The first
+// round (in == null) takes the n_chunks chunks of the jobs' walker ranges; a
+// later one takes chunks of the *n_in slots of `in` -- a count the previous
+// round left on the device, so rounds follow each other on the stream
+// without the host.  A list holds at most walkers + waves x ROUND_SLOTS slots.
 constexpr uint32_t ROUND_REC_PLANES = 5;
+constexpr uint32_t ROUND_SLOTS = 64;
+constexpr uint32_t ROUND_HOLE = 0xFFFFFFFFu;   // plane 0 .x (the job) of an unused slot
 struct RoundArgs {
     uint32_t round_steps;
-    uint64_t cap;                  // records per plane (the flight's walkers)
+    uint64_t cap;                  // slots per plane
     const uint4 *in;
     const unsigned long long *n_in;
     uint4 *out;
     unsigned long long *n_out;
+    unsigned long long *n_live;
 };
 void launch_walk_rounds(const DevGraph &g, const DevJob *jobs, int n_jobs, uint64_t n_chunks, uint32_t chunk,
                         unsigned long long *queue, const RoundArgs &R, int grid, int block, hipStream_t stream);

@@ -556,10 +556,10 @@ __device__ __forceinline__ void save_walker(const Walker &s, uint4 *rec, uint64_
     rec[4 * cap + k] = make_uint4(s.pg.q0, s.pg.q1, s.pg.q2, s.pg.q3);
 }
 
-__device__ __forceinline__ void resume_walker(const DevJob *jobs, const uint4 *rec, uint64_t cap, uint64_t k,
-                                              Walker &s)
+__device__ __forceinline__ void resume_walker(const DevJob *jobs, const uint4 &a, const uint4 *rec, uint64_t cap,
+                                              uint64_t k, Walker &s)
 {
-    const uint4 a = rec[k], b = rec[cap + k], c = rec[2 * cap + k], d = rec[3 * cap + k], e = rec[4 * cap + k];
+    const uint4 b = rec[cap + k], c = rec[2 * cap + k], d = rec[3 * cap + k], e = rec[4 * cap + k];
     s.job = a.x;
     s.idx = a.y;
     s.cur = a.z;
@@ -583,10 +583,14 @@ __device__ __forceinline__ void resume_walker(const DevJob *jobs, const uint4 *r
 
 // The mixed-kind persistent loop of k_walk_persistent, bounded to
 // R.round_steps steps per walker: a walker that reaches the bound at a step
-// boundary is saved as a record in the next round's list (one atomic per
-// wave; a wave's records are consecutive).  Draws are keyed by (walker,
-// step, trial) and trials restart at every step, so paths are the same with
-// or without rounds.
+// boundary is saved as a record in the next round's list.  A wave reserves
+// list slots ROUND_SLOTS at a time (one atomic on *n_out per reservation, not
+// one per suspension: every wave appending to one counter serialises the
+// round), and marks the slots it leaves unused as holes (plane 0 .x =
+// ROUND_HOLE) when it exits; the next round skips them.  *n_live counts the
+// walkers listed (one atomic per wave).  Draws are keyed by (walker, step,
+// trial) and trials restart at every step, so paths are the same with or
+// without rounds.
 __global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *jobs, int n_jobs, uint64_t n_chunks,
                                                    uint32_t chunk, unsigned long long *queue, RoundArgs R)
 {
@@ -599,6 +603,8 @@ __global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *j
     uint32_t pool_job = 0;
     uint64_t pool_next = 0, pool_end = 0;
     bool drained = false;
+    uint64_t res_base = 0;         // wave-uniform: next reserved slot, slots left, walkers listed
+    uint32_t res_left = 0, listed = 0;
     // a continuation round: the walkers the previous round listed.  The count
     // was made by the previous launch's atomics: read it as an atomic (from
     // L2), never as a plain load the compiler may turn into a scalar-cache read
@@ -632,12 +638,17 @@ __global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *j
             const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
             if (!active && rank < avail) {
                 if (R.in) {
-                    resume_walker(jobs, R.in, R.cap, pool_next + rank, s);
+                    const uint64_t k = pool_next + rank;
+                    const uint4 a = R.in[k];
+                    if (a.x != ROUND_HOLE) {
+                        resume_walker(jobs, a, R.in, R.cap, k, s);
+                        active = true;
+                    }
                 } else {
                     init_walker<KIND_ANY>(g, jobs[pool_job], pool_job, (uint32_t)(pool_next + rank), s, ls, false);
+                    active = true;
                 }
                 round_start = s.step;
-                active = true;
             }
             const uint32_t want = (uint32_t)__popcll(need);
             pool_next += want < avail ? want : avail;
@@ -654,16 +665,28 @@ __global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *j
         }
         const uint64_t sm = __ballot(suspend);
         if (sm) {
-            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)sm) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(R.n_out, (unsigned long long)__popcll(sm));
-            base = __shfl(base, (int)leader, 64);
+            const uint32_t m = (uint32_t)__popcll(sm), rank = (uint32_t)__popcll(sm & lt_mask);
+            unsigned long long fresh = 0;   // a new reservation when the current one is short
+            if (m > res_left) {
+                if (lane == 0) fresh = atomicAdd(R.n_out, (unsigned long long)ROUND_SLOTS);
+                fresh = __shfl(fresh, 0, 64);
+            }
             if (suspend) {
-                save_walker(s, R.out, R.cap, base + (uint64_t)__popcll(sm & lt_mask));
+                save_walker(s, R.out, R.cap, rank < res_left ? res_base + rank : fresh + (rank - res_left));
                 active = false;
             }
+            if (m > res_left) {
+                res_base = fresh + (m - res_left);
+                res_left = ROUND_SLOTS - (m - res_left);
+            } else {
+                res_base += m;
+                res_left -= m;
+            }
+            listed += m;
         }
     }
+    if (lane < res_left) R.out[res_base + lane] = make_uint4(ROUND_HOLE, 0u, 0u, 0u);
+    if (lane == 0 && listed) atomicAdd(R.n_live, (unsigned long long)listed);
 }
 
 void launch_walk_rounds(const DevGraph &g, const DevJob *jobs, int n_jobs, uint64_t n_chunks, uint32_t chunk,
