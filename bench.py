@@ -371,9 +371,10 @@ def serve_stream(eng, jobs, gaps):
     return np.array(lat), steps[0], time.perf_counter() - t_start, (failure[0] if failure else None)
 
 
-def serve_session(eng, jobs, gaps, round_steps=0):
+def serve_session(eng, jobs, gaps, round_steps=0, groups=None):
     """One serving session over a query stream, with the loop's flight trace:
-    returns the latency / throughput summary and the flights' occupancy."""
+    returns the latency / throughput summary and the flights' occupancy
+    (and p50/p99 of each named group of queries: name -> boolean mask)."""
     import tempfile
     fd, trace = tempfile.mkstemp(suffix=".csv")
     os.close(fd)
@@ -391,6 +392,10 @@ def serve_session(eng, jobs, gaps, round_steps=0):
             out.update({"p50_ms": round(float(np.percentile(lat, 50)), 3),
                         "p99_ms": round(float(np.percentile(lat, 99)), 3),
                         "steps_per_s": round(steps / span, 1), "occupancy": flight_occupancy(trace)})
+            if groups:
+                out["by_kind_ms"] = {k: {"p50": round(float(np.percentile(lat[m], 50)), 3),
+                                         "p99": round(float(np.percentile(lat[m], 99)), 3)}
+                                     for k, m in groups.items() if m.any()}
         return out
     finally:
         os.environ.pop("KKW_SERVE_TRACE", None)
@@ -469,7 +474,8 @@ def serve_mixed(eng, rp, rate, seconds, seed=8, round_steps=0):
             jobs.append(kk.node2vec_job(int(walkers[i]), 80, 2.0, 0.5, True, seed=3000 + i, walker_begin=begin))
     out = {"rate_qps": rate, "kinds": "PPR t=0.15 / DeepWalk L=80 / node2vec L=80, weighted",
            "walkers": "10^(3+2U)", "round_steps": round_steps}
-    out.update(serve_session(eng, jobs, gaps, round_steps))
+    groups = {"ppr": kinds == 0, "deepwalk": kinds == 1, "node2vec": kinds == 2}
+    out.update(serve_session(eng, jobs, gaps, round_steps, groups))
     log(f"mixed serving {rate} qps: {out}")
     return out
 

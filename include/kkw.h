@@ -339,7 +339,9 @@ int kkw_diag_random_gather(kkw_engine *eng, uint64_t bytes, int reads, int iters
  * still going are suspended at a step boundary and continued by the next
  * launch, so the persistent kernel of a long flight gives the GPU back between
  * rounds and flights admitted meanwhile run in between.  Paths are the same
- * either way. */
+ * either way.  Every query of a flight returns when its flight ends; a flight
+ * holds at most a quarter of max_walkers_in_flight (a larger query goes
+ * alone). */
 typedef struct kkw_serve_opts {
     uint32_t max_walkers_in_flight;  /* slot budget (walkers)                        */
     uint32_t round_steps;            /* walker steps per kernel round (0 = to end)    */
