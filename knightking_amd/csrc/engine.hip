@@ -561,6 +561,7 @@ struct Request {
     double t_submit = 0.0, t_done = 0.0;
     int status = 1;  // 1 pending, 0 done, <0 error
     bool cancelled = false;
+    bool taken = false;     // leaves the pending queue in this admission pass
     uint32_t graph_V = 0;   // vertex count of the graph the query was checked against
     kkw_paths result{};
 };
@@ -642,6 +643,16 @@ static int round_grid(const kkw_engine *eng, uint64_t chunks)
 static size_t rounds_flight_bytes(const kkw_engine *eng)
 {
     return 2 * (size_t)eng->cu_count * 8 * 4 * ROUND_SLOTS * ROUND_REC_PLANES * 16 + 4096;
+}
+
+// Length class of a query for flight grouping: 0 when a walker takes 16
+// steps or fewer on average (its cap, or a PPR walk's mean length 1 / t),
+// else 1.
+static int length_class(const PreparedJob &pj)
+{
+    double e = pj.dj.walk_length;
+    if (pj.dj.kind == KKW_PPR && pj.dj.ppr_ext < 1.0f) e = std::min(e, 1.0 / (1.0 - (double)pj.dj.ppr_ext));
+    return e <= 16.0 ? 0 : 1;
 }
 
 // Device bytes one request adds to a flight: rows, lengths, counts, offsets,
@@ -1092,10 +1103,20 @@ static void serve_loop(ServeState *S)
                 // the device idle while it packs
                 const uint64_t flight_cap = std::max<uint64_t>(1, S->opts.max_walkers_in_flight / 4);
                 uint64_t flight_walkers = 0;
-                while (!S->pending.empty()) {
-                    auto r = S->pending.front();
+                // A flight holds queries of one length class (the head's): short
+                // walks do not wait for long ones that happen to share their
+                // flight.  Queries of the other class keep their place in the
+                // queue for a flight of their own.  Taken queries are marked and
+                // removed in one pass; the scan looks at most kScan queries.
+                constexpr size_t kScan = 4096;
+                int cls = -1;
+                size_t looked = 0;
+                bool taken_any = false;
+                for (auto &r : S->pending) {
+                    if (++looked > kScan) break;
                     if (r->cancelled) {
-                        S->pending.pop_front();
+                        r->taken = true;
+                        taken_any = true;
                         r->status = KKW_E_CANCELLED;
                         r->t_done = now_ms();
                         S->cv_done.notify_all();
@@ -1103,12 +1124,15 @@ static void serve_loop(ServeState *S)
                     }
                     size_t need = request_dev_bytes(r->pj, S->opts.round_steps != 0) + 2 * pack_scan_bytes(r->pj.dj.walker_num + 1);
                     if (need + 4096 + flight_extra > S->dev.cap || request_pin_bytes(r->pj) + 4096 > S->pin->a.cap) {
-                        S->pending.pop_front();   // can never fit the arenas
+                        r->taken = true;   // can never fit the arenas
+                        taken_any = true;
                         r->status = KKW_E_NOMEM;
                         r->t_done = now_ms();
                         S->cv_done.notify_all();
                         continue;
                     }
+                    const int c = length_class(r->pj);
+                    if (cls >= 0 && c != cls) continue;
                     const size_t pneed = request_pin_bytes(r->pj);
                     bool fits_mem = used + need <= room;
                     bool fits_pin = pin_used + pneed <= pin_room;   // worst-case payloads of the whole flight
@@ -1119,13 +1143,19 @@ static void serve_loop(ServeState *S)
                         pin_blocked = admit.empty() && !fits_pin;
                         break;
                     }
-                    S->pending.pop_front();
+                    r->taken = true;
+                    taken_any = true;
+                    cls = c;
                     admit.push_back(r);
                     used += need;
                     pin_used += pneed;
                     walkers += r->pj.dj.walker_num;
                     flight_walkers += r->pj.dj.walker_num;
                 }
+                if (taken_any)
+                    S->pending.erase(std::remove_if(S->pending.begin(), S->pending.end(),
+                                                    [](const std::shared_ptr<Request> &q) { return q->taken; }),
+                                     S->pending.end());
             }
             if (admit.empty() && flights.empty() && !pin_blocked) {
                 if (S->stop && S->pending.empty()) break;
@@ -1174,7 +1204,10 @@ static void serve_loop(ServeState *S)
                 // no room after all: put them back at the head of the queue
                 {
                     std::lock_guard<std::mutex> lk(S->mu);
-                    for (auto it = admit.rbegin(); it != admit.rend(); ++it) S->pending.push_front(*it);
+                    for (auto it = admit.rbegin(); it != admit.rend(); ++it) {
+                        (*it)->taken = false;
+                        S->pending.push_front(*it);
+                    }
                 }
                 flights.pop_back();
                 if (!(pin_short && spill_uncollected(S) > 0)) progressed = false;   // wait for room
