@@ -341,7 +341,8 @@ int kkw_diag_random_gather(kkw_engine *eng, uint64_t bytes, int reads, int iters
  * rounds and flights admitted meanwhile run in between.  Paths are the same
  * either way.  Every query of a flight returns when its flight ends; a flight
  * holds at most a quarter of max_walkers_in_flight (a larger query goes
- * alone). */
+ * alone), and queries of one length class only (expected steps per walker at
+ * most 16, or more), so short queries do not wait for long ones. */
 typedef struct kkw_serve_opts {
     uint32_t max_walkers_in_flight;  /* slot budget (walkers)                        */
     uint32_t round_steps;            /* walker steps per kernel round (0 = to end)    */
