@@ -1674,7 +1674,8 @@ int kkw_graph_adj_index(kkw_engine *eng, int enable)
         eng->adj_index_auto = enable != 0;
         eng->graph_version++;
         if (!enable) {
-            free_adj_index(eng->gs);
+            if (eng->gs.adjset) (void)hipFree(eng->gs.adjset);
+            eng->gs.adjset = nullptr;
             return KKW_OK;
         }
         if (!eng->gs.row_ptr) return fail(KKW_E_NOGRAPH, "no graph loaded");

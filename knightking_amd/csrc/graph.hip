@@ -1024,8 +1024,6 @@ void GraphStore::release()
     dfree(lbucket);
     dfree(vsum);
     dfree(adjset);
-    dfree(efilter);
-    efilter_blocks = 0;
     V = 0;
     E = 0;
     sorted = false;
@@ -1049,8 +1047,6 @@ DevGraph GraphStore::view() const
     g.vsum = vsum;
     g.symmetric = symmetric ? 1u : 0u;
     g.adjset = adjset;
-    g.efilter = efilter;
-    g.efilter_blocks = efilter_blocks;
     return g;
 }
 
@@ -1316,52 +1312,6 @@ void sort_adjacency(GraphStore &gs, hipStream_t s)
     detect_symmetric(gs, s);
 }
 
-// Edge filter of every directed edge: one thread per edge finds its source
-// by binary search of row_ptr and sets the edge's bits.
-__global__ void k_efilter(const uint64_t *__restrict__ row_ptr, const uint32_t *__restrict__ col, uint32_t V,
-                          uint64_t E, uint32_t *__restrict__ f, uint64_t blocks)
-{
-    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= E) return;
-    uint32_t lo = 0, hi = V - 1;   // the last u with row_ptr[u] <= e owns edge e
-    while (lo < hi) {
-        const uint32_t mid = (uint32_t)(((uint64_t)lo + hi + 1) >> 1);
-        if (row_ptr[mid] <= e) lo = mid;
-        else hi = mid - 1;
-    }
-    const uint64_t h = efilter_hash(lo, col[e]);
-    uint32_t *blk = f + efilter_word(h, blocks);
-#pragma unroll
-    for (uint32_t k = 0; k < EFILTER_K; k++) {
-        const uint32_t b = (uint32_t)(h >> (7 * k)) & 127u;
-        atomicOr(blk + (b >> 5), 1u << (b & 31u));
-    }
-}
-
-// Bits per edge of the edge filter (KKW_EFILTER_BITS overrides; 0: no filter).
-static uint64_t efilter_bits_per_edge()
-{
-    if (const char *env = getenv("KKW_EFILTER_BITS")) return strtoull(env, nullptr, 10);
-    return 8;
-}
-
-static void build_edge_filter(GraphStore &gs, hipStream_t s)
-{
-    dfree(gs.efilter);
-    gs.efilter_blocks = 0;
-    const uint64_t bits = efilter_bits_per_edge();
-    if (!gs.E || !bits) return;
-    const uint64_t blocks = std::min<uint64_t>(std::max<uint64_t>(1, gs.E * bits / 128), 0xFFFFFFFFull);
-    DArr<uint32_t> f(blocks * 4);
-    KKW_HIP(hipMemsetAsync(f.p, 0, blocks * 16, s));
-    hipLaunchKernelGGL(k_efilter, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V, gs.E, f.p,
-                       blocks);
-    KKW_HIP(hipGetLastError());
-    KKW_HIP(hipStreamSynchronize(s));
-    gs.efilter = f.release();
-    gs.efilter_blocks = blocks;
-}
-
 void build_adj_index(GraphStore &gs, hipStream_t s)
 {
     if (!gs.row_ptr) throw Error(-2, "no graph loaded");
@@ -1433,14 +1383,6 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
     KKW_HIP(hipGetLastError());
     KKW_HIP(hipStreamSynchronize(s));
     gs.adjset = t.release();
-    build_edge_filter(gs, s);
-}
-
-void free_adj_index(GraphStore &gs)
-{
-    dfree(gs.adjset);
-    dfree(gs.efilter);
-    gs.efilter_blocks = 0;
 }
 
 void build_uniform_buckets(GraphStore &gs, hipStream_t s)

@@ -53,8 +53,6 @@ struct DevGraph {
     uint32_t symmetric;        // (u,v) is an edge iff (v,u) is (lists sorted)
     uint32_t packed_rows;      // buckets carry the rows of their candidates
     const uint32_t *adjset;    // null, or per-vertex neighbour hash sets (below)
-    const uint32_t *efilter;   // null, or the edge filter (below): efilter_blocks 16-byte blocks
-    uint64_t efilter_blocks;
 };
 
 // Neighbour hash sets: vertex v owns slots [4 (row_ptr[v] + v), + 4 deg(v) + 4)
@@ -72,45 +70,6 @@ __host__ __device__ __forceinline__ uint32_t adj_table_size(uint64_t deg) { retu
 __host__ __device__ __forceinline__ uint32_t adj_slot(uint32_t key, uint32_t size)
 {
     return (uint32_t)(((uint64_t)(key * 0x9E3779B1u) * size) >> 32);
-}
-
-// Edge filter: a blocked Bloom filter of the directed edges (u, v).  Edge
-// (u, v) sets EFILTER_K bits of one 16-byte block, all chosen by one 64-bit
-// hash of (u, v) (murmur3's fmix64 finaliser): the block from the high half,
-// bit k from bits 7k..7k+6.  A membership test reads that one block: a clear
-// bit proves (u, v) is no edge; all bits set sends the test on to the
-// neighbour hash set.  The filter is far smaller than the hash sets, so its
-// blocks stay within the reach of the translation caches.
-constexpr uint32_t EFILTER_K = 3;
-
-__host__ __device__ __forceinline__ uint64_t efilter_hash(uint32_t u, uint32_t v)
-{
-    uint64_t x = ((uint64_t)u << 32) | v;
-    x ^= x >> 33;
-    x *= 0xff51afd7ed558ccdull;
-    x ^= x >> 33;
-    x *= 0xc4ceb9fe1a85ec53ull;
-    x ^= x >> 33;
-    return x;
-}
-
-// first word of the block (blocks < 2^32)
-__host__ __device__ __forceinline__ uint64_t efilter_word(uint64_t h, uint64_t blocks)
-{
-    return 4 * (((h >> 32) * blocks) >> 32);
-}
-
-// true when every bit of h is set in the block (w0..w3)
-__host__ __device__ __forceinline__ bool efilter_maybe(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint64_t h)
-{
-    bool all = true;
-#pragma unroll
-    for (uint32_t k = 0; k < EFILTER_K; k++) {
-        const uint32_t b = (uint32_t)(h >> (7 * k)) & 127u, q = b >> 5;
-        const uint32_t w = q == 0 ? w0 : q == 1 ? w1 : q == 2 ? w2 : w3;
-        all = all && ((w >> (b & 31u)) & 1u);
-    }
-    return all;
 }
 
 // Device form of a job (host constants precomputed in float).

@@ -36,8 +36,6 @@ struct GraphStore {
     Bucket *lbucket = nullptr;   // alias tables holding local edge indices (walk programs)
     float *vsum = nullptr;
     uint32_t *adjset = nullptr;
-    uint32_t *efilter = nullptr;       // edge filter, built with the hash sets
-    uint64_t efilter_blocks = 0;
     bool sorted = false;
     bool symmetric = false;
     bool packable = false;   // every row fits the packed form
@@ -60,7 +58,6 @@ void type_by_dst(GraphStore &gs, int ntypes, uint64_t seed, hipStream_t s);
 // ids and rows; either way the per-vertex weight sums go to vsum
 void build_alias(GraphStore &gs, hipStream_t s, bool local = false);
 void build_adj_index(GraphStore &gs, hipStream_t s);
-void free_adj_index(GraphStore &gs);
 void build_uniform_buckets(GraphStore &gs, hipStream_t s);
 
 // one job's rows inside a serving flight (packed output)

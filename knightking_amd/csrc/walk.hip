@@ -148,13 +148,6 @@ __device__ __forceinline__ bool adj_member(const DevGraph &g, uint32_t prev, uin
                                            uint32_t x, uint32_t &probes, uint32_t &lines)
 {
     if (g.adjset) {
-        if (g.efilter) {   // a clear bit of the edge filter: no edge (prev, x)
-            const uint64_t h = efilter_hash(prev, x);
-            const uint4 q = *reinterpret_cast<const uint4 *>(g.efilter + efilter_word(h, g.efilter_blocks));
-            probes++;
-            lines++;
-            if (!efilter_maybe(q.x, q.y, q.z, q.w, h)) return false;
-        }
         const uint32_t *tab = g.adjset + adj_table_start(pbeg, prev);
         const uint32_t size = adj_table_size(pend - pbeg);
         uint32_t s = adj_slot(x, size);
@@ -848,7 +841,7 @@ size_t pack_scan_bytes(uint64_t n)
 // sorted list or hash set (SEARCH, HASH: node2vec adjacency test).  Lanes at different phases share each memory instruction,
 // so a wave is not held back by its longest search; the 4-entry probe also
 // ends a search up to two levels early.  Paths are identical to advance()'s.
-enum : uint32_t { PH_ROW = 0, PH_TRIAL = 1, PH_SEARCH = 2, PH_CROW = 3, PH_HASH = 4, PH_ALIAS = 5, PH_FILTER = 6 };
+enum : uint32_t { PH_ROW = 0, PH_TRIAL = 1, PH_SEARCH = 2, PH_CROW = 3, PH_HASH = 4, PH_ALIAS = 5 };
 
 
 // LINE_N: rows written in chunks of LINE_N entries through LineBufN (dynamic
@@ -882,7 +875,6 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
     uint64_t &toff = sbase;
     uint32_t &tsize = slen_all, &hslot = lo;
     uint32_t cand = 0, rz = 0, rw = 0;
-    uint64_t fh = 0;           // edge-filter hash of (prev, cand) while the filter block is read
     bool row_known = false;    // the candidate's row is already in cbeg/cend
     float dart = 0.0f;
     uintptr_t last_line = 0;   // instrumented builds: distinct 64-B lines per search
@@ -948,9 +940,6 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
         } else if (phase == PH_HASH) {
             blk = (toff + hslot) & ~(uint64_t)3;
             addr = g.adjset + blk;
-        } else if (phase == PH_FILTER) {
-            blk = efilter_word(fh, g.efilter_blocks);
-            addr = g.efilter + blk;
         } else {
             // aligned block of four entries holding the middle of the range
             blk = (sbase + lo + ((hi - lo) >> 1)) & ~(uint64_t)3;
@@ -1032,17 +1021,12 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
                 } else if (dart > J.acc_hi) {
                     rejected = true;
                 } else if (g.adjset) {
-                    ls.searches++;
-                    last_line = ~(uintptr_t)0;
-                    if (g.efilter) {   // the filter first: most candidates are not prev's neighbours
-                        fh = efilter_hash(prev, cand);
-                        phase = PH_FILTER;
-                        continue;
-                    }
                     toff = adj_table_start(pbeg, prev);
                     tsize = adj_table_size(pend - pbeg);
                     hslot = adj_slot(cand, tsize);
                     phase = PH_HASH;
+                    ls.searches++;
+                    last_line = ~(uintptr_t)0;
                     continue;
                 } else if (g.symmetric) {
                     if (!row_known) {
@@ -1078,20 +1062,6 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
             } else {
                 moved = true;   // DeepWalk: a static draw is accepted
             }
-        } else if (phase == PH_FILTER) {
-            if (STATS) {
-                ls.probes++;
-                ls.lines++;
-            }
-            if (efilter_maybe(v.x, v.y, v.z, v.w, fh)) {   // possibly an edge: ask the hash set
-                toff = adj_table_start(pbeg, prev);
-                tsize = adj_table_size(pend - pbeg);
-                hslot = adj_slot(cand, tsize);
-                phase = PH_HASH;
-                continue;
-            }
-            moved = dart <= J.inv_q;   // no edge (prev, cand)
-            rejected = !moved;
         } else if (phase == PH_HASH) {
             // slots from hslot to the end of this aligned block, in probe order
             if (STATS) {

@@ -597,36 +597,6 @@ def test_node2vec_with_and_without_hash_sets(engine, index):
     engine.adj_index(True)
 
 
-@pytest.mark.parametrize("bits", [0, 1, 8])
-def test_node2vec_edge_filter_sizes(engine, bits, monkeypatch):
-    """The edge filter in front of the hash sets never changes an answer: paths
-    equal the oracle's with no filter, with a 1-bit-per-edge filter (most
-    negatives pass it and go on to the hash set) and with the default 8 bits,
-    both launch forms.  The probe counts show the filter in use: with 1 bit per
-    edge a test reads about two blocks (filter, then hash set); without a
-    filter about one."""
-    monkeypatch.setenv("KKW_EFILTER_BITS", str(bits))
-    load_rmat(engine, 12)
-    engine.adj_index(False)
-    engine.adj_index(True)   # hash sets and filter rebuilt with this size
-    rp, col, w = O.rmat_graph(12, 16, seed=1, weighted=True)
-    g = O.Graph(rp, col, w)
-    op, ol, ot, st = O.walk(g, O.NODE2VEC, 4096, 40, True, p=2.0, q=0.5, seed=67)
-    for mode in (kk.MODE_PLAIN, kk.MODE_PERSISTENT):
-        (gpu,) = engine.run([kk.node2vec_job(4096, 40, 2.0, 0.5, True, seed=67)], mode=mode, stats=True)
-        assert_paths_equal(gpu, op, ol, ot, f"filter bits={bits} mode={mode}")
-        t = engine.timing()
-        assert t["searches"] > 0
-        per = t["search_probes"] / t["searches"]
-        if bits == 0:
-            assert per < 1.3, per
-        elif bits == 1:
-            assert per > 1.5, per
-    monkeypatch.delenv("KKW_EFILTER_BITS")
-    engine.adj_index(False)
-    engine.adj_index(True)
-
-
 ADJ_SEG = 16384   # slots of one LDS segment of a long list's hash set
 
 
