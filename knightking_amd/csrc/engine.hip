@@ -846,8 +846,8 @@ static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
     return true;
 }
 
-// After the walk (its last round): pack the rows straight into the flight's
-// pinned block, copy lengths and offsets beside them, record the event.
+// After the walk (its last round): pack the rows, their lengths and offsets
+// straight into the flight's pinned block, record the event.
 static void enqueue_pack(ServeState *S, Flight &f)
 {
     hipStream_t st = S->streams[f.sidx];
@@ -856,15 +856,18 @@ static void enqueue_pack(ServeState *S, Flight &f)
     char *HD = S->pin->dbase + f.pin_off;   // the same pinned block as the kernels address it
     const uint64_t N = f.walkers;
     const int nj = (int)f.reqs.size();
+    const size_t lens_bytes = ((N * 4 + 7) & ~(size_t)7);
     if (f.chunks) {
+        // the kernels write lengths and offsets straight into the pinned block
         launch_pack((const PackJob *)(D + f.o_pjobs), nj, (const uint32_t *)(D + f.o_lens), (uint64_t *)(D + f.o_cnt),
-                    (uint64_t *)(D + f.o_offs), N, (uint32_t *)(HD + f.p_payload), D + f.o_scan, f.scan_bytes, st);
+                    (uint64_t *)(D + f.o_offs), N, (uint32_t *)(HD + f.p_payload), (uint32_t *)(HD + f.p_down),
+                    (uint64_t *)(HD + f.p_down + lens_bytes), D + f.o_scan, f.scan_bytes, st);
+        KKW_HIP(hipGetLastError());
     } else {
         launch_fill_u64((unsigned long long *)(D + f.o_offs), 0ull, 1, st);
+        KKW_HIP(hipGetLastError());
+        KKW_HIP(hipMemcpyAsync(H + f.p_down, D + f.o_lens, lens_bytes + (N + 1) * 8, hipMemcpyDeviceToHost, st));
     }
-    KKW_HIP(hipGetLastError());
-    const size_t lens_bytes = ((N * 4 + 7) & ~(size_t)7);
-    KKW_HIP(hipMemcpyAsync(H + f.p_down, D + f.o_lens, lens_bytes + (N + 1) * 8, hipMemcpyDeviceToHost, st));
     KKW_HIP(hipEventRecord(f.ev, st));
     f.packing = true;
 }

@@ -107,8 +107,12 @@ void launch_fill_u64(unsigned long long *p, unsigned long long v, int n, hipStre
 void launch_stream_copy(const void *src, void *dst, uint64_t bytes, int cu_count, hipStream_t stream);
 void launch_random_gather(const void *buf, uint64_t bytes, uint32_t threads, uint32_t reads, uint64_t seed,
                           uint32_t *sink, hipStream_t stream);
+// Packed rows: counts (lengths clamped to their row), offsets (offs[0] = 0,
+// then the inclusive scan), rows copied to out at their offsets.  h_lens /
+// h_offs, when not null, receive the lengths and offsets too (pinned memory
+// the caller reads after the stream: no copy needed).
 void launch_pack(const PackJob *jobs, int n_jobs, const uint32_t *lens, uint64_t *cnt, uint64_t *offs, uint64_t n,
-                 uint32_t *out, void *scan_tmp, size_t scan_bytes, hipStream_t s);
+                 uint32_t *out, uint32_t *h_lens, uint64_t *h_offs, void *scan_tmp, size_t scan_bytes, hipStream_t s);
 size_t pack_scan_bytes(uint64_t n);
 
 }  // namespace kkw

@@ -757,16 +757,25 @@ __device__ __forceinline__ int pack_job_of(const PackJob *__restrict__ jobs, int
     return lo;
 }
 
-// lengths (without the cap flag) as 64-bit counts, for the offset scan.  A
-// count never exceeds its row (the job's stride), so the packed output stays
-// within rows x stride entries whatever a length word holds.
+// lengths (without the cap flag) as 64-bit counts, for the offset scan, and
+// offs[0] = 0.  A count never exceeds its row (the job's stride), so the
+// packed output stays within rows x stride entries whatever a length word
+// holds.  With h_lens / h_offs (the caller's pinned copies) the lengths and
+// offs[0] also go there.
 __global__ void k_lens_to_counts(const PackJob *__restrict__ jobs, int n_jobs, const uint32_t *__restrict__ lens,
-                                 uint64_t n, uint64_t *__restrict__ cnt)
+                                 uint64_t n, uint64_t *__restrict__ cnt, uint64_t *__restrict__ offs,
+                                 uint32_t *__restrict__ h_lens, uint64_t *__restrict__ h_offs)
 {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        offs[0] = 0;
+        if (h_offs) h_offs[0] = 0;
+    }
     if (i >= n) return;
-    const uint32_t len = lens[i] & 0x7fffffffu, cap = jobs[pack_job_of(jobs, n_jobs, i)].stride;
+    const uint32_t raw = lens[i];
+    const uint32_t len = raw & 0x7fffffffu, cap = jobs[pack_job_of(jobs, n_jobs, i)].stride;
     cnt[i] = len < cap ? len : cap;
+    if (h_lens) h_lens[i] = raw;
 }
 
 // Copy every walker's row into the packed array at its offset.  Rows of a
@@ -782,9 +791,11 @@ __device__ __forceinline__ void pack_wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// With h_offs, each wave also copies its rows' end offsets there (offs[0]
+// comes from k_lens_to_counts), so no copy of the offsets follows.
 __global__ void __launch_bounds__(256) k_pack_rows(const PackJob *__restrict__ jobs, int n_jobs,
                                                    const uint64_t *__restrict__ offs, uint64_t n,
-                                                   uint32_t *__restrict__ out)
+                                                   uint32_t *__restrict__ out, uint64_t *__restrict__ h_offs)
 {
     __shared__ uint64_t s_off[4][65];
     __shared__ const uint32_t *s_src[4][64];
@@ -800,6 +811,7 @@ __global__ void __launch_bounds__(256) k_pack_rows(const PackJob *__restrict__ j
     s_off[wv][lane] = offs[i0 + (lane < cnt ? lane : cnt)];
     if (lane == 0) s_off[wv][64] = offs[i0 + cnt];
     pack_wave_sync();
+    if (h_offs && lane < cnt) h_offs[i0 + lane + 1] = s_off[wv][lane + 1];
     const uint64_t base = s_off[wv][0];
     const uint64_t total = s_off[wv][cnt] - base;
     for (uint64_t o = lane; o < total; o += 64) {
@@ -814,15 +826,15 @@ __global__ void __launch_bounds__(256) k_pack_rows(const PackJob *__restrict__ j
 }
 
 void launch_pack(const PackJob *jobs, int n_jobs, const uint32_t *lens, uint64_t *cnt, uint64_t *offs, uint64_t n,
-                 uint32_t *out, void *scan_tmp, size_t scan_bytes, hipStream_t s)
+                 uint32_t *out, uint32_t *h_lens, uint64_t *h_offs, void *scan_tmp, size_t scan_bytes, hipStream_t s)
 {
     if (n == 0) return;
     unsigned g = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_lens_to_counts, dim3(g), dim3(256), 0, s, jobs, n_jobs, lens, n, cnt);
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long *)offs, 0ull, 1);
+    hipLaunchKernelGGL(k_lens_to_counts, dim3(g), dim3(256), 0, s, jobs, n_jobs, lens, n, cnt, offs, h_lens, h_offs);
     size_t tb = scan_bytes;
     (void)hipcub::DeviceScan::InclusiveSum(scan_tmp, tb, cnt, offs + 1, n, s);
-    hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, jobs, n_jobs, offs, n, out);
+    hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, jobs, n_jobs, offs, n, out,
+                       h_offs);
 }
 
 size_t pack_scan_bytes(uint64_t n)
