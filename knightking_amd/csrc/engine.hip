@@ -688,9 +688,10 @@ static bool advance_rounds(ServeState *S, Flight &f);
 
 // Build and launch one flight.  Returns false (and leaves reqs pending) when
 // the device or pinned arena has no room for them yet; *pin_short tells which.  One stream-ordered
-// sequence: upload the job tables, walk, pack the rows straight into the
-// flight's pinned block, copy lengths and offsets beside them, record the
-// event; the host touches the flight again only when it has completed.
+// sequence: upload the job tables, walk, pack the rows, their lengths and
+// offsets straight into the flight's pinned block, record the event; the host
+// touches the flight again only when it has completed (or, walked in rounds,
+// to queue its next rounds).
 static bool launch_flight(ServeState *S, Flight &f, bool *pin_short)
 {
     *pin_short = false;
