@@ -604,7 +604,7 @@ struct Flight {
     uint64_t rec_cap = 0;          // slots of a suspended-walker list: walkers + waves x ROUND_SLOTS
     size_t o_list[2] = {0, 0};     // suspended-walker records, one list per round parity
     size_t o_rq = 0;               // per-round work-queue words (zeroed by the upload)
-    size_t o_rcnt = 0;             // per-round slot counts [n_rounds + 1], then listed-walker counts [n_rounds + 1]
+    size_t o_rcnt = 0;             // per round [n_rounds + 1]: slot counts, then (ended waves, listed walkers) words
     size_t p_rcnt = 0;             // pinned: round r's listed count lands at p_rcnt + 8 (r + 1)
 };
 
@@ -875,13 +875,13 @@ static void enqueue_pack(ServeState *S, Flight &f)
 
 // Round r of a flight walked in rounds: continue the walkers round r - 1
 // listed (all of them in round 0), reading their count on the device, list
-// the ones it suspends, and copy that count to the round's pinned slot; the
-// round's event says when it is there.
+// the ones it suspends; the launch's last wave writes that count to the
+// round's pinned slot, and the round's event says when it is there.
 static void enqueue_round(ServeState *S, Flight &f, uint32_t r)
 {
     hipStream_t st = S->streams[f.sidx];
     char *D = S->dev_base + f.dev_off;
-    char *H = S->pin->base + f.pin_off;
+    char *HD = S->pin->dbase + f.pin_off;   // the pinned block as the kernels address it
     const uint32_t chunk = 64;
     unsigned long long *cnt = (unsigned long long *)(D + f.o_rcnt);
     unsigned long long *live = cnt + f.n_rounds + 1;
@@ -893,10 +893,10 @@ static void enqueue_round(ServeState *S, Flight &f, uint32_t r)
     R.out = (uint4 *)(D + f.o_list[(r + 1) & 1]);
     R.n_out = cnt + r + 1;
     R.n_live = live + r + 1;
+    R.h_live = (unsigned long long *)(HD + f.p_rcnt) + r + 1;
     launch_walk_rounds(f.g, (const DevJob *)(D + f.o_jobs), (int)f.reqs.size(), f.chunks, chunk,
                        (unsigned long long *)(D + f.o_rq) + r, R, f.round_grid, 256, st);
     KKW_HIP(hipGetLastError());
-    KKW_HIP(hipMemcpyAsync(H + f.p_rcnt + 8 * ((size_t)r + 1), live + r + 1, 8, hipMemcpyDeviceToHost, st));
     KKW_HIP(hipEventRecord(f.round_ev[r % kRoundsAhead], st));
 }
 

@@ -84,7 +84,8 @@ void launch_walk_persistent_mixed(const DevGraph &g, const DevJob *jobs, int n_j
 // ROUND_REC_PLANES 16-byte words (plane p of record k at out[p * cap + k],
 // so consecutive lanes touch consecutive words).  Each wave reserves slots
 // ROUND_SLOTS at a time from *n_out and marks the ones it leaves unused as
-// holes; *n_live counts the walkers listed (both zero at launch).  The first
+// holes; *n_live counts the walkers listed and the waves ended (both zero at
+// launch), and the launch's last wave copies the walkers listed to *h_live.  The first
 // round (in == null) takes the n_chunks chunks of the jobs' walker ranges; a
 // later one takes chunks of the *n_in slots of `in` -- a count the previous
 // round left on the device, so rounds follow each other on the stream
@@ -92,6 +93,7 @@ void launch_walk_persistent_mixed(const DevGraph &g, const DevJob *jobs, int n_j
 constexpr uint32_t ROUND_REC_PLANES = 5;
 constexpr uint32_t ROUND_SLOTS = 64;
 constexpr uint32_t ROUND_HOLE = 0xFFFFFFFFu;   // plane 0 .x (the job) of an unused slot
+constexpr uint32_t ROUND_WAVE_SHIFT = 44;      // *n_live: ended waves above, listed walkers below
 struct RoundArgs {
     uint32_t round_steps;
     uint64_t cap;                  // slots per plane
@@ -99,7 +101,8 @@ struct RoundArgs {
     const unsigned long long *n_in;
     uint4 *out;
     unsigned long long *n_out;
-    unsigned long long *n_live;
+    unsigned long long *n_live;    // waves ended << ROUND_WAVE_SHIFT | walkers listed
+    unsigned long long *h_live;    // host memory (device address): the walkers listed, once the launch ends
 };
 void launch_walk_rounds(const DevGraph &g, const DevJob *jobs, int n_jobs, uint64_t n_chunks, uint32_t chunk,
                         unsigned long long *queue, const RoundArgs &R, int grid, int block, hipStream_t stream);

@@ -588,7 +588,8 @@ __device__ __forceinline__ void resume_walker(const DevJob *jobs, const uint4 &a
 // one per suspension: every wave appending to one counter serialises the
 // round), and marks the slots it leaves unused as holes (plane 0 .x =
 // ROUND_HOLE) when it exits; the next round skips them.  *n_live counts the
-// walkers listed (one atomic per wave).  Draws are keyed by (walker, step,
+// walkers listed and the waves ended (one atomic per wave); the last wave to
+// end copies the count to *h_live (host memory).  Draws are keyed by (walker, step,
 // trial) and trials restart at every step, so paths are the same with or
 // without rounds.
 __global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *jobs, int n_jobs, uint64_t n_chunks,
@@ -686,7 +687,16 @@ __global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *j
         }
     }
     if (lane < res_left) R.out[res_base + lane] = make_uint4(ROUND_HOLE, 0u, 0u, 0u);
-    if (lane == 0 && listed) atomicAdd(R.n_live, (unsigned long long)listed);
+    if (lane == 0) {
+        // one atomic per wave adds the wave's end (high bits) and its listed
+        // walkers (low bits) to one word: the wave that sees every other end
+        // already counted holds the round's total and publishes it to the host
+        // (no fence needed: both counts live in the one word)
+        const unsigned long long waves = (unsigned long long)gridDim.x * (blockDim.x >> 6);
+        const unsigned long long old = atomicAdd(R.n_live, (1ull << ROUND_WAVE_SHIFT) + listed);
+        if ((old >> ROUND_WAVE_SHIFT) == waves - 1)
+            *R.h_live = (old + listed) & ((1ull << ROUND_WAVE_SHIFT) - 1);
+    }
 }
 
 void launch_walk_rounds(const DevGraph &g, const DevJob *jobs, int n_jobs, uint64_t n_chunks, uint32_t chunk,
