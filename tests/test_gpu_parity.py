@@ -409,6 +409,39 @@ def test_mixed_batch_equals_single_jobs(engine):
         assert_same_rows(alone, m)
 
 
+@pytest.mark.parametrize("n_jobs", [64, 80], ids=["table_in_lds", "table_in_hbm"])
+def test_mixed_batch_many_jobs(engine, n_jobs):
+    """Batches up to 64 jobs read their job table from a per-block LDS copy,
+    larger ones from device memory: both give every job the oracle's paths.
+    Metapath jobs carry different scheme sets, so each walker's scheme masks
+    (held in registers from its start) come from its own job."""
+    from helpers import assert_rows_match_oracle, oracle_walk
+    load_rmat(engine, 10)
+    engine.type_by_dst(5, 9)
+    rp, col, w, et = engine.download(etype=True)
+    g = O.Graph(rp, col, w, et)
+    rng = np.random.default_rng(n_jobs)
+    jobs = []
+    for i in range(n_jobs):
+        n = int(rng.integers(20, 300))
+        k = i % 4
+        if k == 0:
+            jobs.append(kk.deepwalk_job(n, int(rng.integers(5, 40)), bool(i & 4), seed=500 + i))
+        elif k == 1:
+            jobs.append(kk.ppr_job(n, 0.1, True, seed=500 + i))
+        elif k == 2:
+            schemes = [[int(rng.integers(1, 32)) for _ in range(int(rng.integers(1, 9)))]
+                       for _ in range(int(rng.integers(1, 4)))]
+            jobs.append(kk.metapath_job(n, int(rng.integers(5, 40)), schemes, True, seed=500 + i))
+        else:
+            jobs.append(kk.node2vec_job(n, int(rng.integers(5, 30)), 2.0, 0.5, True, seed=500 + i))
+    print(f"[gpu] {n_jobs} mixed jobs on R-MAT-10 in one persistent launch", flush=True)
+    mixed = engine.run(jobs, mode=kk.MODE_PERSISTENT)
+    for i, (j, m) in enumerate(zip(jobs, mixed)):
+        op, ol, ot, _ = oracle_walk(g, j)
+        assert_rows_match_oracle(m, op, ol, ot, what=f"job {i} of {n_jobs}")
+
+
 def test_graph_replay_matches(engine):
     """A batch captured in a hipGraph and replayed gives the same paths every time."""
     load_rmat(engine, 10)
