@@ -1376,6 +1376,7 @@ int kkw_engine_create(const kkw_engine_opts *opts, kkw_engine **out)
         if (eng->device < 0 || eng->device >= ndev) return fail(KKW_E_INVALID, "device ordinal out of range");
         KKW_HIP(hipSetDevice(eng->device));
         KKW_HIP(hipDeviceGetAttribute(&eng->cu_count, hipDeviceAttributeMultiprocessorCount, eng->device));
+
         KKW_HIP(hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking));
         KKW_HIP(hipStreamCreateWithFlags(&eng->aux_stream, hipStreamNonBlocking));
         // kkw_run_noalloc's scratch: this device's copy of the fixed array, and
@@ -1383,6 +1384,8 @@ int kkw_engine_create(const kkw_engine_opts *opts, kkw_engine **out)
         eng->noalloc_scratch = scratch_address(eng->device);
         KKW_HIP(hipEventCreate(&eng->noalloc_ev[0]));
         KKW_HIP(hipEventCreate(&eng->noalloc_ev[1]));
+        pool_attach(eng->device);
+        eng->pool_attached = true;
         *out = eng.release();
         return KKW_OK;
     });
@@ -1398,6 +1401,7 @@ int kkw_engine_destroy(kkw_engine *eng)
     if (eng->aux_stream) (void)hipStreamDestroy(eng->aux_stream);
     for (hipEvent_t e : eng->noalloc_ev)
         if (e) (void)hipEventDestroy(e);
+    if (eng->pool_attached) pool_detach(eng->device);
     delete eng;
     return KKW_OK;
 }
@@ -1678,7 +1682,7 @@ int kkw_graph_adj_index(kkw_engine *eng, int enable)
         eng->adj_index_auto = enable != 0;
         eng->graph_version++;
         if (!enable) {
-            if (eng->gs.adjset) (void)hipFree(eng->gs.adjset);
+            pool_free(eng->gs.adjset);
             eng->gs.adjset = nullptr;
             return KKW_OK;
         }
@@ -2189,11 +2193,18 @@ int kkw_serve_start(kkw_engine *eng, const kkw_serve_opts *opts)
         KKW_HIP(hipSetDevice(eng->device));
         for (auto &s : S->streams) KKW_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         // device arena: the admission budget (hbm_budget, else half the free HBM, at most 64 GiB)
+        // (the graph store's kept blocks count as free: they go back to the
+        // runtime when the arena needs the room)
         size_t free_b = 0, total_b = 0;
         KKW_HIP(hipMemGetInfo(&free_b, &total_b));
+        free_b += pool_kept_bytes(eng->device);
         size_t cap = eng->hbm_budget ? (size_t)eng->hbm_budget : std::min(free_b / 2, (size_t)64 << 30);
         if (cap > free_b) cap = free_b / 2;
-        KKW_HIP(hipMalloc((void **)&S->dev_base, cap ? cap : 256));
+        if (hipMalloc((void **)&S->dev_base, cap ? cap : 256) != hipSuccess) {
+            (void)hipGetLastError();
+            pool_trim(eng->device);
+            KKW_HIP(hipMalloc((void **)&S->dev_base, cap ? cap : 256));
+        }
         S->dev.init(cap);
         // pinned staging: flights pack their paths into it and results are
         // handed out in place (released by kkw_paths_free), so it is sized for

@@ -42,7 +42,9 @@ inline double now_ms()
         .count();
 }
 
-// Device allocation owned by a batch / request.
+// Device allocation owned by a batch / request (from the device pool, so a
+// batch's path buffers are kept for the next batch, and blocks the graph store
+// keeps go back to the runtime when a batch needs the room).
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -50,19 +52,12 @@ struct DevBuf {
     explicit DevBuf(size_t n)
     {
         bytes = n;
-        hipError_t e = hipMalloc(&p, n ? n : 1);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();   // a later launch check must not see this failure
-            throw Error(KKW_E_NOMEM, "hipMalloc of " + std::to_string(n) + " bytes: " + hipGetErrorString(e));
-        }
+        p = pool_alloc(n ? n : 1);   // Error(KKW_E_NOMEM) when out of memory
     }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; }
-    ~DevBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
+    ~DevBuf() { pool_free(p); }
 };
 
 }  // namespace kkw
@@ -88,6 +83,7 @@ struct kkw_engine {
     // (its device address, from hipGetSymbolAddress) and the two timing events
     void *noalloc_scratch = nullptr;
     hipEvent_t noalloc_ev[2] = {nullptr, nullptr};
+    bool pool_attached = false;   // registered with the device pool (kkw::pool_attach)
 };
 
 // Neighbour hash sets when the engine builds them on its own (first node2vec

@@ -12,30 +12,142 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 namespace kkw {
+
+// ------------------------------------------------------------ device pool ----
+namespace {
+
+struct Pool {
+    std::mutex mu;
+    std::map<int, std::multimap<size_t, void *>> kept;       // device -> (bytes -> block)
+    std::unordered_map<void *, std::pair<int, size_t>> owned;  // block -> (device, bytes), kept or in use
+    std::map<int, int> engines;                               // device -> registered engines
+};
+
+Pool &pool()
+{
+    static Pool *p = new Pool;   // never destroyed: engines may outlive static destruction order
+    return *p;
+}
+
+// kept blocks of `device` back to the runtime (pool lock held)
+void trim_locked(Pool &P, int device)
+{
+    auto it = P.kept.find(device);
+    if (it == P.kept.end()) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    if (cur != device) (void)hipSetDevice(device);
+    for (auto &kv : it->second) {
+        (void)hipFree(kv.second);
+        P.owned.erase(kv.second);
+    }
+    if (cur != device) (void)hipSetDevice(cur);
+    P.kept.erase(it);
+}
+
+}  // namespace
+
+void *pool_alloc(size_t bytes)
+{
+    if (bytes == 0) bytes = 1;
+    int dev = 0;
+    KKW_HIP(hipGetDevice(&dev));
+    Pool &P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto &kept = P.kept[dev];
+    auto it = kept.lower_bound(bytes);
+    if (it != kept.end() && (it->first - bytes <= bytes || it->first - bytes <= (1u << 20))) {
+        void *q = it->second;
+        kept.erase(it);
+        return q;
+    }
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess && !kept.empty()) {
+        (void)hipGetLastError();
+        trim_locked(P, dev);
+        e = hipMalloc(&q, bytes);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();   // a later launch check must not see this failure
+        throw Error(-3, "hipMalloc of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+    }
+    P.owned[q] = {dev, bytes};
+    return q;
+}
+
+void pool_free(void *p)
+{
+    if (!p) return;
+    Pool &P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto it = P.owned.find(p);
+    if (it == P.owned.end()) {   // not the pool's: hand it back directly
+        (void)hipFree(p);
+        return;
+    }
+    const int dev = it->second.first;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    if (cur != dev) (void)hipSetDevice(dev);
+    (void)hipDeviceSynchronize();   // no queued work may still use the block when it is handed out again
+    if (cur != dev) (void)hipSetDevice(cur);
+    P.kept[dev].emplace(it->second.second, p);
+}
+
+void pool_attach(int device)
+{
+    Pool &P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.engines[device]++;
+}
+
+void pool_detach(int device)
+{
+    Pool &P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    if (--P.engines[device] <= 0) {
+        P.engines.erase(device);
+        trim_locked(P, device);
+    }
+}
+
+size_t pool_kept_bytes(int device)
+{
+    Pool &P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    size_t n = 0;
+    auto it = P.kept.find(device);
+    if (it != P.kept.end())
+        for (auto &kv : it->second) n += kv.first;
+    return n;
+}
+
+void pool_trim(int device)
+{
+    Pool &P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    trim_locked(P, device);
+}
 
 namespace {
 
 template <typename T>
 T *dalloc(size_t n)
 {
-    T *p = nullptr;
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc((void **)&p, n * sizeof(T));
-    if (e != hipSuccess) {
-        (void)hipGetLastError();   // a later launch check must not see this failure
-        throw Error(-3, std::string("hipMalloc of ") + std::to_string(n * sizeof(T)) + " bytes: " +
-                            hipGetErrorString(e));
-    }
-    return p;
+    return static_cast<T *>(pool_alloc((n ? n : 1) * sizeof(T)));
 }
 
 template <typename T>
 void dfree(T *&p)
 {
-    if (p) (void)hipFree(p);
+    pool_free(p);
     p = nullptr;
 }
 
@@ -75,16 +187,13 @@ struct Temp {
     void ensure(size_t bytes)
     {
         if (bytes <= n) return;
-        if (p) (void)hipFree(p);
+        pool_free(p);
         p = nullptr;
         n = 0;
-        KKW_HIP(hipMalloc(&p, bytes ? bytes : 1));
+        p = pool_alloc(bytes ? bytes : 1);
         n = bytes;
     }
-    ~Temp()
-    {
-        if (p) (void)hipFree(p);
-    }
+    ~Temp() { pool_free(p); }
 };
 
 inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }

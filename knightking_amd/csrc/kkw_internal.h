@@ -21,6 +21,25 @@ struct Error : std::runtime_error {
             throw ::kkw::Error(-4, std::string(#call) + ": " + hipGetErrorString(e_));        \
     } while (0)
 
+// Device memory of the graph store and of the graph and table builders,
+// cached per device.  Freeing a large device buffer can hold the host for up
+// to a second of driver time (MI355X: 0.26-1.08 s to free ~36 GiB with the
+// device idle, while allocating is under a millisecond), and graph loads and
+// table builds free their temporaries, and rebuilt tables their
+// predecessors.  A freed block waits for the device to finish its work (as
+// hipFree does) and is kept; an allocation takes the smallest kept block of
+// its device that holds it without wasting more than half of it (or 1 MiB),
+// else asks the runtime, and when the runtime has no memory left every kept
+// block of the device goes back to it and the request is tried once more.
+// Engines register with their device; when the last one goes, so do the kept
+// blocks.  Thread-safe.
+void *pool_alloc(size_t bytes);   // on the current device; throws Error(-3) when out of memory
+void pool_free(void *p);          // null is ignored
+void pool_attach(int device);
+void pool_detach(int device);
+size_t pool_kept_bytes(int device);
+void pool_trim(int device);       // returns every kept block of the device to the runtime
+
 // Device-resident CSR and its derived arrays.  Every pointer is a device
 // allocation owned by the store (null when absent).
 struct GraphStore {
