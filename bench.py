@@ -603,7 +603,9 @@ def main():
             traffic_source = (f"none: {tf} was profiled on sources {prof.get('source_digest')}, "
                               f"this run's are {source_digest()}")
 
-    # the same launch on freshly rebuilt tables (new placements in HBM)
+    # the same launch on rebuilt tables: the library's device pool gives a
+    # rebuilt table the block its predecessor's predecessor held, so these
+    # launches see at most two table placements of this process
     placements = []
     for r in range(args.rebuilds):
         pb.close()
