@@ -440,7 +440,11 @@ __device__ __forceinline__ bool advance(const DevGraph &g, const DevJob &J, Walk
         return false;
     }
 
-    // First-order kinds: termination once per step (walk.hpp:956-973).
+    // First-order kinds: termination once per step (walk.hpp:956-973).  The
+    // trial's draw does not depend on the termination test's outcome: it is
+    // computed first, so that the two Philox blocks of a PPR step overlap
+    // (PPR-only batches 4 % faster; a walker that stops discards it).
+    const u32x4 r = draw(J.seed, s.wid, s.step, s.trial, TAG_TRIAL);
     if (s.trial == 0) {
         bool stop = deg == 0;
         if (!stop) {
@@ -472,7 +476,6 @@ __device__ __forceinline__ bool advance(const DevGraph &g, const DevJob &J, Walk
         return true;
     }
     ls.trials++;
-    u32x4 r = draw(J.seed, s.wid, s.step, s.trial, TAG_TRIAL);
     bool row_ok = false;
     uint64_t crow = 0;
     if (kind == KIND_METAPATH) {
