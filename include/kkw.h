@@ -135,7 +135,12 @@ const char *kkw_last_error(void);
 int kkw_abi_version(void);
 
 /* Engine lifetime.  Replaces WalkEngine() / GraphEngine() (include/walk.hpp:378-389,
- * include/graph.hpp:236-257) and set_concurrency (walk.hpp:391-396, a no-op here). */
+ * include/graph.hpp:236-257) and set_concurrency (walk.hpp:391-396, a no-op here).
+ * Device memory freed by graph loads, table builds and batches is kept by the
+ * library for reuse on that device (freeing large blocks to the runtime can
+ * stall the host for up to a second); the kept blocks go back to the runtime
+ * when an allocation would otherwise fail and when the last engine on the
+ * device is destroyed. */
 int kkw_engine_create(const kkw_engine_opts *opts, kkw_engine **out);
 int kkw_engine_destroy(kkw_engine *eng);
 
