@@ -466,6 +466,25 @@ def test_stats_variant_same_paths(engine):
     assert t["steps"] == a.steps and t["trials"] >= t["steps"]
 
 
+@pytest.mark.parametrize("mode", [kk.MODE_PLAIN, kk.MODE_PERSISTENT], ids=["plain", "persistent"])
+def test_node2vec_trial_cap(engine, mode):
+    """A return step accepted only when the dart is exactly 0 (p = 1e30 on a
+    single edge: 1/p = 1e-30, darts on the 2^-24 grid): most walkers spend the
+    per-step cap of 2^20 trials and end there, truncated, after the steps they
+    had.  GPU and oracle agree on every row, length and flag, in both launch
+    forms (the state-machine kernel writes such a walker's last line as it
+    writes the last line of any walk)."""
+    src, dst = np.array([0], np.uint32), np.array([1], np.uint32)
+    engine.graph_from_edges(2, src, dst, np.array([2.0], np.float32), make_undirected=True, sort_adj=True)
+    rp, col, w, _ = engine.download()
+    g = O.Graph(rp, col, w)
+    print(f"[gpu] node2vec trial cap, mode={mode}", flush=True)
+    (gpu,) = engine.run([kk.node2vec_job(16, 10, 1e30, 1.0, True, seed=5)], mode=mode)
+    op, ol, ot, _ = O.walk(g, O.NODE2VEC, 16, 10, True, p=1e30, q=1.0, seed=5)
+    assert ot.sum() >= 8, "the cap was expected to end most walks"
+    assert_paths_equal(gpu, op, ol, ot, "trial cap")
+
+
 def test_node2vec_needs_sorted_lists(engine, karate):
     src, dst, w = (a[::-1].copy() for a in karate)   # file order descending: lists unsorted
     engine.graph_from_edges(34, src, dst, w, sort_adj=False)
