@@ -20,6 +20,8 @@
 #include <thread>
 #include <vector>
 
+#include <sys/prctl.h>
+
 using namespace kkw;
 
 namespace {
@@ -1033,6 +1035,10 @@ static void release_flight(ServeState *S, Flight &f)
 
 static void serve_loop(ServeState *S)
 {
+    // The loop polls flight events with 10-microsecond sleeps; under Linux's
+    // default 50-microsecond timer slack each sleep lasted ~60 us.  With 1 us
+    // of slack: PPR query p50 0.22 -> 0.18 ms, p99 at 2000 qps 0.50 -> 0.44 ms.
+    (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0UL, 0UL, 0UL);
     kkw_engine *eng = S->eng;
     (void)hipSetDevice(eng->device);
     std::list<Flight> flights;
