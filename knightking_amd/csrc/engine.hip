@@ -1382,7 +1382,6 @@ int kkw_engine_create(const kkw_engine_opts *opts, kkw_engine **out)
         if (eng->device < 0 || eng->device >= ndev) return fail(KKW_E_INVALID, "device ordinal out of range");
         KKW_HIP(hipSetDevice(eng->device));
         KKW_HIP(hipDeviceGetAttribute(&eng->cu_count, hipDeviceAttributeMultiprocessorCount, eng->device));
-
         KKW_HIP(hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking));
         KKW_HIP(hipStreamCreateWithFlags(&eng->aux_stream, hipStreamNonBlocking));
         // kkw_run_noalloc's scratch: this device's copy of the fixed array, and
