@@ -303,7 +303,7 @@ static void enqueue_walk(const kkw_engine *eng, const std::vector<PreparedJob> &
         int grid = (int)std::max<uint64_t>(1, std::min(want, blocks_needed));
         if (jobs.size() == 1)
             launch_walk_persistent_single(g, jobs[0].dj, w.n_chunks, (uint32_t)w.chunk, w.d_queue, w.stats,
-                                          w.d_stats, grid, w.block, s);
+                                          w.d_stats, grid, w.block, eng->cu_count, s);
         else
             launch_walk_persistent_mixed(g, w.d_jobs, (int)jobs.size(), w.n_chunks, (uint32_t)w.chunk, w.d_queue,
                                          w.stats, w.d_stats, grid, w.block, w.line_rows, s);

@@ -90,9 +90,10 @@ struct PackJob {
 // walk.hip
 void launch_walk_plain(const DevGraph &g, const DevJob &J, bool stats, DevStats *st, int block,
                        hipStream_t stream);
+// cu_count: the device's CUs (the kernel form depends on walkers per CU)
 void launch_walk_persistent_single(const DevGraph &g, const DevJob &J, uint64_t n_chunks,
                                    uint32_t chunk, unsigned long long *queue, bool stats,
-                                   DevStats *st, int grid, int block, hipStream_t stream);
+                                   DevStats *st, int grid, int block, int cu_count, hipStream_t stream);
 // lines: some job's rows are a multiple of 32 entries (written as whole lines
 // through the LDS line buffer the launch then allocates)
 void launch_walk_persistent_mixed(const DevGraph &g, const DevJob *jobs, int n_jobs, uint64_t n_chunks,

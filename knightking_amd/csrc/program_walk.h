@@ -330,18 +330,22 @@ __device__ __forceinline__ void kkw_prog_flush(const uint32_t *buf, uint32_t *pa
 }
 
 // Persistent form: a grid sized to the device; each wave takes work items of
-// A.chunk consecutive walkers from A.queue, and a lane whose walker has ended
-// takes the next walker of the wave's item (ballot + prefix count), so lanes
-// freed by short walks (isolated starts, early termination) are refilled at
-// once.  Each loop iteration runs one trial per active lane, with the same
-// draws as the plain form; a wave leaves when the queue is drained and its
-// lanes are idle, so every wave reaches the exit.
+// A.chunk consecutive walkers, and a lane whose walker has ended takes the
+// next walker of the wave's item (ballot + prefix count), so lanes freed by
+// short walks (isolated starts, early termination) are refilled at once.  A
+// wave's first item is the one its own index names; later ones come from
+// A.queue, numbered past the grid's own items (with every item taken from
+// the queue, a wave whose first walkers ended at once could take a second
+// item before the last waves had a first, and walk it after its first
+// walkers while those waves left empty).  Each loop iteration runs one trial
+// per active lane, with the same draws as the plain form; a wave leaves when
+// the queue is drained and its lanes are idle, so every wave reaches the exit.
 extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk_persistent(kkw_prog_args A)
 {
     kkw_ctx c = A.c;   // per lane: the kept row changes as the walker moves
     const uint32_t lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
-    bool active = false, drained = false;
+    bool active = false, drained = false, first_item = true;
     uint32_t pool_next = 0, pool_end = 0;
     uint32_t i = 0, cur = 0, deg = 0, trial = 0, env = 0;
     uint64_t beg = 0;
@@ -359,9 +363,15 @@ extern "C" __global__ void __launch_bounds__(KKW_PROG_BLOCK) kkw_program_walk_pe
         unsigned long long need = __ballot(!active);
         while (need && !drained) {
             if (pool_next >= pool_end) {
+                const unsigned long long waves = (unsigned long long)gridDim.x * (blockDim.x >> 6);
                 unsigned long long q = 0;
-                if (lane == 0) q = atomicAdd(A.queue, 1ull);
-                q = __shfl(q, 0, 64);
+                if (first_item) {
+                    first_item = false;
+                    q = (unsigned long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+                } else {
+                    if (lane == 0) q = atomicAdd(A.queue, 1ull);
+                    q = __shfl(q, 0, 64) + waves;
+                }
                 if (q >= A.n_chunks) {
                     drained = true;
                     break;

@@ -549,6 +549,26 @@ __device__ __forceinline__ uint32_t find_job(const DevJob *jobs, int n_jobs, uin
     return (uint32_t)lo;
 }
 
+// The next work item (chunk of walkers) of this wave.  A wave's first item is
+// the one its own index names, taken without an atomic; later ones come from
+// the queue, numbered past the grid's own items, so every item is taken once.
+// When the first items too came from the queue, a wave whose first walkers
+// ended at once (isolated starts) took a second item before the last waves
+// had their first; with about one item per wave, the waves left without one
+// exited and that wave walked the rest of its second item only after its
+// first walkers, doubling the launch (R-MAT-14 DeepWalk, 256 items for 256
+// waves: 8 waves with two items, 131 us against 59 us for the plain form).
+__device__ __forceinline__ unsigned long long next_chunk(unsigned long long *queue, bool &first)
+{
+    if (first) {
+        first = false;
+        return (unsigned long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    }
+    unsigned long long c = 0;
+    if ((threadIdx.x & 63) == 0) c = atomicAdd(queue, 1ull);
+    return __shfl(c, 0, 64) + (unsigned long long)gridDim.x * (blockDim.x >> 6);
+}
+
 // Job tables of up to LDS_JOBS jobs are copied into LDS by every block of a
 // mixed-kind launch.  Every trial reads its walker's job fields (seed, kind,
 // length, path row); read from a table in device memory, each such read needs
@@ -594,14 +614,12 @@ __global__ void __launch_bounds__(256) k_walk_persistent(DevGraph g, DevJob J0, 
     bool active = false;
     // wave-uniform pool of walker rows [pool_next, pool_end) of job pool_job
     uint32_t pool_job = 0, pool_next = 0, pool_end = 0;
-    bool drained = false;
+    bool drained = false, first_chunk = true;
     for (;;) {
         uint64_t need = __ballot(!active);
         while (need && !drained) {
             if (pool_next >= pool_end) {
-                unsigned long long c = 0;
-                if (lane == 0) c = atomicAdd(queue, 1ull);
-                c = __shfl(c, 0, 64);
+                const unsigned long long c = next_chunk(queue, first_chunk);
                 if (c >= n_chunks) {
                     drained = true;
                     break;
@@ -703,7 +721,7 @@ __global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *j
     uint32_t round_start = 0;
     uint32_t pool_job = 0;
     uint64_t pool_next = 0, pool_end = 0;
-    bool drained = false;
+    bool drained = false, first_chunk = true;
     uint64_t res_base = 0;         // wave-uniform: next reserved slot, slots left, walkers listed
     uint32_t res_left = 0, listed = 0;
     // a continuation round: the walkers the previous round listed.  The count
@@ -716,9 +734,7 @@ __global__ void __launch_bounds__(256) k_walk_rounds(DevGraph g, const DevJob *j
         uint64_t need = __ballot(!active);
         while (need && !drained) {
             if (pool_next >= pool_end) {
-                unsigned long long c = 0;
-                if (lane == 0) c = atomicAdd(queue, 1ull);
-                c = __shfl(c, 0, 64);
+                const unsigned long long c = next_chunk(queue, first_chunk);
                 if (c >= n_chunks) {
                     drained = true;
                     break;
@@ -1006,7 +1022,7 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
     float dart = 0.0f;
     uintptr_t last_line = 0;   // instrumented builds: distinct 64-B lines per search
     uint32_t pool_next = 0, pool_end = 0;
-    bool drained = false;
+    bool drained = false, first_chunk = true;
     PathGroup pg = {0u, 0u, 0u, 0u};
     uint32_t *const paths = J.paths;
     const uint32_t stride = J.stride;
@@ -1019,9 +1035,7 @@ __global__ void __launch_bounds__(256) k_walk_sm(DevGraph g, DevJob J, uint64_t 
         uint64_t need = __ballot(!active);
         while (need && !drained) {
             if (pool_next >= pool_end) {
-                unsigned long long c = 0;
-                if (lane == 0) c = atomicAdd(queue, 1ull);
-                c = __shfl(c, 0, 64);
+                const unsigned long long c = next_chunk(queue, first_chunk);
                 if (c >= n_chunks) {
                     drained = true;
                     break;
@@ -1351,19 +1365,27 @@ void launch_fill_u64(unsigned long long *p, unsigned long long v, int n, hipStre
 template <int KIND, bool ALIAS>
 static void launch_single_t(const DevGraph &g, const DevJob &J, uint64_t n_chunks, uint32_t chunk,
                             unsigned long long *queue, bool stats, DevStats *st, int grid, int block,
-                            hipStream_t stream)
+                            int cu_count, hipStream_t stream)
 {
     // The state-machine kernel for node2vec (unless the walk needs the outlier
-    // appendix, handled by advance()) and DeepWalk; PPR and metapath run the
-    // trial-loop form, measured faster for them (R-MAT-22, 4 M walkers: PPR
-    // 6.1 vs 6.9 ms, metapath 6.9 vs 8.5 ms; DeepWalk 5.5 ms either way).
-    // KKW_WALK_KERNEL=trial selects the trial-loop form for every kind.
-    static const bool trial_form = [] {
-        const char *e = getenv("KKW_WALK_KERNEL");
-        return e && strcmp(e, "trial") == 0;
-    }();
+    // appendix, handled by advance()) and DeepWalk jobs of at least 16384
+    // walkers per CU; PPR, metapath and smaller jobs run the trial-loop form.
+    // Measured (one MI355X, 256 CUs): PPR 6.1 vs 6.9 ms and metapath 6.9 vs
+    // 8.5 ms at R-MAT-22 (4 M walkers); node2vec and DeepWalk equal at 4 M
+    // walkers, the state machine 1-3 % faster at R-MAT-24 (16.8 M) and the
+    // trial loop faster below: 0.38 vs 0.56 ms (node2vec) and 0.083 vs 0.185
+    // ms (DeepWalk) at R-MAT-14, 0.73 vs 0.86 ms (node2vec) at R-MAT-18, and
+    // 2.41 vs 2.43-2.64 ms at R-MAT-20.  With few waves per SIMD nothing hides
+    // the state machine's per-iteration work (3x the vector and 9x the scalar
+    // instructions of the plain kernel per wave at R-MAT-14).
+    // KKW_WALK_KERNEL=trial or =sm (read at every launch) fixes the form for
+    // every job size; =trial also for every kind.
+    const char *form = getenv("KKW_WALK_KERNEL");
+    const bool trial_form = form && strcmp(form, "trial") == 0;
+    const bool sm_form = form && strcmp(form, "sm") == 0;
+    const bool large = (uint64_t)J.walker_num >= (uint64_t)(cu_count > 0 ? cu_count : 1) * 16384u;
     constexpr bool SM_KIND = KIND == KIND_NODE2VEC || KIND == KIND_DEEPWALK;
-    const bool sm = SM_KIND && !trial_form && !(KIND == KIND_NODE2VEC && J.overflow > 0.0f);
+    const bool sm = SM_KIND && !trial_form && (large || sm_form) && !(KIND == KIND_NODE2VEC && J.overflow > 0.0f);
     if constexpr (SM_KIND) {
         if (sm) {
             // rows of whole lines: 128-byte aligned rows (the path buffer is 256-byte aligned)
@@ -1403,11 +1425,13 @@ static void launch_single_t(const DevGraph &g, const DevJob &J, uint64_t n_chunk
 // One job in a persistent launch (specialised on kind and alias use).
 void launch_walk_persistent_single(const DevGraph &g, const DevJob &J, uint64_t n_chunks,
                                    uint32_t chunk, unsigned long long *queue, bool stats,
-                                   DevStats *st, int grid, int block, hipStream_t stream)
+                                   DevStats *st, int grid, int block, int cu_count, hipStream_t stream)
 {
-#define KKW_SINGLE(KD)                                                                            \
-    if (J.use_alias) launch_single_t<KD, true>(g, J, n_chunks, chunk, queue, stats, st, grid, block, stream); \
-    else launch_single_t<KD, false>(g, J, n_chunks, chunk, queue, stats, st, grid, block, stream);
+#define KKW_SINGLE(KD)                                                                                       \
+    if (J.use_alias)                                                                                         \
+        launch_single_t<KD, true>(g, J, n_chunks, chunk, queue, stats, st, grid, block, cu_count, stream);   \
+    else                                                                                                     \
+        launch_single_t<KD, false>(g, J, n_chunks, chunk, queue, stats, st, grid, block, cu_count, stream);
     switch (J.kind) {
     case KIND_DEEPWALK: KKW_SINGLE(KIND_DEEPWALK) break;
     case KIND_PPR: KKW_SINGLE(KIND_PPR) break;

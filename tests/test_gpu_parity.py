@@ -14,6 +14,28 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
+# A persistent launch whose node2vec or DeepWalk job runs in the state-machine
+# kernel whatever its size (the library picks that kernel from 16384 walkers
+# per CU up, the trial-loop kernel below; other kinds run the trial loop).
+PERSISTENT_SM = "persistent_sm"
+MODES = [kk.MODE_PLAIN, kk.MODE_PERSISTENT, PERSISTENT_SM]
+MODE_IDS = ["plain", "persistent", "persistent_sm"]
+
+
+def run_jobs(engine, jobs, mode, **kw):
+    """engine.run in one of MODES."""
+    if mode != PERSISTENT_SM:
+        return engine.run(jobs, mode=mode, **kw)
+    old = os.environ.get("KKW_WALK_KERNEL")
+    os.environ["KKW_WALK_KERNEL"] = "sm"
+    try:
+        return engine.run(jobs, mode=kk.MODE_PERSISTENT, **kw)
+    finally:
+        if old is None:
+            del os.environ["KKW_WALK_KERNEL"]
+        else:
+            os.environ["KKW_WALK_KERNEL"] = old
+
 
 def assert_paths_equal(gpu, o_paths, o_lens, o_trunc, what=""):
     assert np.array_equal(gpu.lengths, o_lens), f"{what}: lengths differ"
@@ -229,7 +251,7 @@ def test_integer_weights_alias_and_walks(engine, seed):
     ave = np.repeat(vs_o / np.maximum(np.diff(rp.astype(np.int64)), 1), np.diff(rp.astype(np.int64)))
     assert np.any(wd == ave.astype(np.float32))
     g = O.Graph(rp, col, wd)
-    for mode in (kk.MODE_PLAIN, kk.MODE_PERSISTENT):
+    for mode in MODES:
         for job in (dict(kind=kk.DEEPWALK, walker_num=6000, walk_length=40, seed=60 + seed),
                     dict(kind=kk.NODE2VEC, walker_num=6000, walk_length=40, p=2.0, q=0.5, seed=70 + seed)):
             gpu, op, ol, ot = _walk_both(engine, g, mode, **dict(job))
@@ -242,7 +264,7 @@ def _walk_both(engine, g, mode, **job):
     L = job.pop("walk_length", 0)
     weighted = job.pop("use_weights", True)
     kj = kk.Job(kind, n, L, use_weights=weighted, **job)
-    (gpu,) = engine.run([kj], mode=mode)
+    (gpu,) = run_jobs(engine, [kj], mode)
     okw = dict(job)
     okw.pop("schemes", None)
     o_paths, o_lens, o_tr, st = O.walk(g, kind, n, L, weighted, schemes=job.get("schemes"), **okw)
@@ -272,7 +294,7 @@ WALK_CASES = {
 }
 
 
-@pytest.mark.parametrize("mode", [kk.MODE_PLAIN, kk.MODE_PERSISTENT], ids=["plain", "persistent"])
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
 @pytest.mark.parametrize("case", sorted(WALK_CASES))
 def test_walk_matches_oracle(engine, rmat10, case, mode):
     load_rmat(engine, 10)
@@ -302,9 +324,9 @@ def test_node2vec_rmat14_config(engine):
     load_rmat(engine, 14)
     rp, col, w = O.rmat_graph(14, 16, seed=1, weighted=True)
     g = O.Graph(rp, col, w)
-    for mode in (kk.MODE_PLAIN, kk.MODE_PERSISTENT):
-        (gpu,) = engine.run([kk.node2vec_job(16384, 80, 2.0, 0.5, True, seed=1)], mode=mode)
-        op, ol, ot, st = O.walk(g, O.NODE2VEC, 16384, 80, True, p=2.0, q=0.5, seed=1)
+    op, ol, ot, st = O.walk(g, O.NODE2VEC, 16384, 80, True, p=2.0, q=0.5, seed=1)
+    for mode in MODES:
+        (gpu,) = run_jobs(engine, [kk.node2vec_job(16384, 80, 2.0, 0.5, True, seed=1)], mode)
         assert_paths_equal(gpu, op, ol, ot, f"rmat14 mode {mode}")
         assert gpu.steps == st.steps
 
@@ -466,7 +488,7 @@ def test_stats_variant_same_paths(engine):
     assert t["steps"] == a.steps and t["trials"] >= t["steps"]
 
 
-@pytest.mark.parametrize("mode", [kk.MODE_PLAIN, kk.MODE_PERSISTENT], ids=["plain", "persistent"])
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
 def test_node2vec_trial_cap(engine, mode):
     """A return step accepted only when the dart is exactly 0 (p = 1e30 on a
     single edge: 1/p = 1e-30, darts on the 2^-24 grid): most walkers spend the
@@ -479,7 +501,7 @@ def test_node2vec_trial_cap(engine, mode):
     rp, col, w, _ = engine.download()
     g = O.Graph(rp, col, w)
     print(f"[gpu] node2vec trial cap, mode={mode}", flush=True)
-    (gpu,) = engine.run([kk.node2vec_job(16, 10, 1e30, 1.0, True, seed=5)], mode=mode)
+    (gpu,) = run_jobs(engine, [kk.node2vec_job(16, 10, 1e30, 1.0, True, seed=5)], mode)
     op, ol, ot, _ = O.walk(g, O.NODE2VEC, 16, 10, True, p=1e30, q=1.0, seed=5)
     assert ot.sum() >= 8, "the cap was expected to end most walks"
     assert_paths_equal(gpu, op, ol, ot, "trial cap")
@@ -516,6 +538,9 @@ def test_large_node2vec_properties(engine):
     (pl,) = engine.run([job], mode=kk.MODE_PLAIN)
     assert np.array_equal(pl.lengths, pp.lengths)
     assert np.array_equal(pl.vertices[sample, :81], pp.vertices[sample, :81])
+    # 2^20 walkers: the trial-loop kernel by default; the state machine too
+    (ps,) = run_jobs(engine, [job], PERSISTENT_SM)
+    assert_same_rows(ps, pp, "node2vec R-MAT-20, state machine")
 
 
 def _edge_keys(rp, col):
@@ -596,7 +621,7 @@ def test_mixed_jobs_rmat23_properties(engine):
 
 
 @pytest.mark.parametrize("p,q", [(2.0, 0.5), (0.5, 2.0), (1.0, 0.25)])
-@pytest.mark.parametrize("mode", [kk.MODE_PLAIN, kk.MODE_PERSISTENT], ids=["plain", "persistent"])
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
 def test_node2vec_directed_graph(engine, p, q, mode):
     """An asymmetric graph: the adjacency test must search the previous vertex's
     list (no shortcut through the candidate's list); where the return edge is
@@ -610,7 +635,7 @@ def test_node2vec_directed_graph(engine, p, q, mode):
     rp, col, wd, _ = engine.download()
     g = O.Graph(rp, col, wd)
     print(f"[gpu] directed node2vec p={p} q={q} mode={mode}", flush=True)
-    (gpu,) = engine.run([kk.node2vec_job(2000, 30, p, q, True, seed=9)], mode=mode)
+    (gpu,) = run_jobs(engine, [kk.node2vec_job(2000, 30, p, q, True, seed=9)], mode)
     op, ol, ot, st = O.walk(g, O.NODE2VEC, 2000, 30, True, p=p, q=q, seed=9)
     assert_paths_equal(gpu, op, ol, ot, "directed")
 
@@ -643,8 +668,8 @@ def test_node2vec_with_and_without_hash_sets(engine, index):
     g = O.Graph(rp, col, w)
     for weighted in (True, False):
         op, ol, ot, st = O.walk(g, O.NODE2VEC, 4096, 40, weighted, p=2.0, q=0.5, seed=61)
-        for mode in (kk.MODE_PLAIN, kk.MODE_PERSISTENT):
-            (gpu,) = engine.run([kk.node2vec_job(4096, 40, 2.0, 0.5, weighted, seed=61)], mode=mode)
+        for mode in MODES:
+            (gpu,) = run_jobs(engine, [kk.node2vec_job(4096, 40, 2.0, 0.5, weighted, seed=61)], mode)
             assert_paths_equal(gpu, op, ol, ot, f"index={index} weighted={weighted} mode={mode}")
     engine.adj_index(True)
 
@@ -718,9 +743,9 @@ def test_node2vec_hash_sets_of_hub_lists(engine, index, monkeypatch):
     n = 30000
     for weighted in (True, False):
         op, ol, ot, st = O.walk(g, O.NODE2VEC, n, 20, weighted, p=2.0, q=0.5, seed=71)
-        for mode in (kk.MODE_PLAIN, kk.MODE_PERSISTENT):
+        for mode in MODES:
             print(f"[gpu] hub lists index={index} weighted={weighted} mode={mode}", flush=True)
-            (gpu,) = engine.run([kk.node2vec_job(n, 20, 2.0, 0.5, weighted, seed=71)], mode=mode)
+            (gpu,) = run_jobs(engine, [kk.node2vec_job(n, 20, 2.0, 0.5, weighted, seed=71)], mode)
             assert_paths_equal(gpu, op, ol, ot, f"hubs index={index} weighted={weighted} mode={mode}")
     engine.adj_index(True)
 
