@@ -58,7 +58,7 @@ def write_program_sources() -> None:
 def build_lib(verbose: bool = False) -> str:
     write_program_sources()
     cmd = [hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-o", OUT + ".tmp", *[os.path.join(CSRC, s) for s in SOURCES],
-           "-lhiprtc"]
+           "-lhiprtc", "-lrocprofiler-sdk-roctx"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

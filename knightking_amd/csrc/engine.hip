@@ -329,6 +329,7 @@ static void batch_enqueue(kkw_batch *b)
 
 static void batch_launch(kkw_batch *b)
 {
+    TraceRange tr("walk_launch");
     kkw_engine *eng = b->eng;
     if (!eng->gs.row_ptr || eng->gs.V != b->graph_V || eng->gs.E != b->graph_E)
         throw Error(KKW_E_INVALID, "the graph changed since this batch was created; create a new batch");
@@ -370,6 +371,7 @@ static void batch_launch(kkw_batch *b)
 static double batch_sync(kkw_batch *b)
 {
     if (!b->launched) throw Error(KKW_E_INVALID, "batch was not launched");
+    TraceRange tr("walk_wait");
     KKW_HIP(hipEventSynchronize(b->e1));
     float ms = 0.0f;
     KKW_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
@@ -393,6 +395,7 @@ static double batch_sync(kkw_batch *b)
 
 static void fill_paths_host(const PreparedJob &pj, hipStream_t s, kkw_paths *out)
 {
+    TraceRange tr("paths_to_host");
     const DevJob &d = pj.dj;
     memset(out, 0, sizeof(*out));
     out->walker_begin = d.walker_begin;
@@ -602,6 +605,7 @@ struct Flight {
     uint32_t done_rounds = 0;      // rounds seen complete (their counts read)
     hipEvent_t round_ev[kRoundsAhead] = {};   // round r records round_ev[r % kRoundsAhead]
     DevGraph g{};                  // the graph view the flight was launched on (fixed while serving)
+    roctx_range_id_t trace_id = 0; // marker-trace range from launch to retirement (0: none)
     int round_grid = 0;            // blocks of every round's launch
     uint64_t rec_cap = 0;          // slots of a suspended-walker list: walkers + waves x ROUND_SLOTS
     size_t o_list[2] = {0, 0};     // suspended-walker records, one list per round parity
@@ -1039,6 +1043,7 @@ static void serve_loop(ServeState *S)
     // default 50-microsecond timer slack each sleep lasted ~60 us.  With 1 us
     // of slack: PPR query p50 0.22 -> 0.18 ms, p99 at 2000 qps 0.50 -> 0.44 ms.
     (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0UL, 0UL, 0UL);
+    roctxNameOsThread("kkw serving loop");
     kkw_engine *eng = S->eng;
     (void)hipSetDevice(eng->device);
     std::list<Flight> flights;
@@ -1083,6 +1088,7 @@ static void serve_loop(ServeState *S)
                     if (r->status == 1) finish_request(S, r, code);
                 release_flight(S, f);
             }
+            if (f.trace_id) roctxRangeStop(f.trace_id);
             stream_busy[f.sidx] = false;
             walkers_in_flight -= f.walkers;
             it = flights.erase(it);
@@ -1222,6 +1228,7 @@ static void serve_loop(ServeState *S)
                 flights.pop_back();
                 if (!(pin_short && spill_uncollected(S) > 0)) progressed = false;   // wait for room
             } else {
+                f.trace_id = roctxRangeStartA("serve_flight");
                 stream_busy[sidx] = true;
                 walkers_in_flight += f.walkers;
             }
@@ -1273,6 +1280,7 @@ static int run_noalloc(kkw_engine *eng, const kkw_job *jobs, int n_jobs, int mod
                        const uint32_t *strides, uint32_t *const *lengths)
 {
     if (!eng->noalloc_scratch) throw Error(KKW_E_DEVICE, "the engine has no scratch address");
+    TraceRange tr("run_noalloc");
     std::lock_guard<std::mutex> slk(scratch_mutex(eng->device));
     std::vector<PreparedJob> pjs;
     for (int i = 0; i < n_jobs; i++) pjs.push_back(prepare_job(eng, jobs[i], false, /*may_build=*/false));
@@ -1384,6 +1392,8 @@ int kkw_engine_create(const kkw_engine_opts *opts, kkw_engine **out)
         KKW_HIP(hipDeviceGetAttribute(&eng->cu_count, hipDeviceAttributeMultiprocessorCount, eng->device));
         KKW_HIP(hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking));
         KKW_HIP(hipStreamCreateWithFlags(&eng->aux_stream, hipStreamNonBlocking));
+        roctxNameHipStream("kkw engine", eng->stream);
+        roctxNameHipStream("kkw aux", eng->aux_stream);
         // kkw_run_noalloc's scratch: this device's copy of the fixed array, and
         // its events, set up here so that the call itself creates nothing
         eng->noalloc_scratch = scratch_address(eng->device);
@@ -1627,6 +1637,7 @@ int kkw_prepare(kkw_engine *eng)
         refuse_while_serving(eng);
         KKW_HIP(hipSetDevice(eng->device));
         if (!eng->gs.row_ptr) return fail(KKW_E_NOGRAPH, "no graph loaded");
+        TraceRange tr("prepare");
         eng->graph_version++;
         const double t0 = now_ms();
         if (eng->gs.w && eng->adj_index_auto && !eng->gs.adjset) {
@@ -1937,7 +1948,10 @@ int kkw_run_epochs(kkw_engine *eng, const kkw_job *job, uint32_t epoch_walkers, 
             const int b = e & 1;
             const uint32_t begin = job->walker_begin + e * W, n = std::min(W, total - e * W);
             if (e + 1 < n_epochs) enqueue_epoch(e + 1);   // walks while epoch e is consumed
-            KKW_HIP(hipEventSynchronize(r.walked[b]));
+            {
+                TraceRange tw("epoch_walk_wait");
+                KKW_HIP(hipEventSynchronize(r.walked[b]));
+            }
             float ms = 0.0f;
             KKW_HIP(hipEventElapsedTime(&ms, r.k0[b], r.walked[b]));
             kernel_ms += ms;
@@ -1950,7 +1964,11 @@ int kkw_run_epochs(kkw_engine *eng, const kkw_job *job, uint32_t epoch_walkers, 
             p.on_device = 1;
             p.vertices = paths[b];
             p.lengths = lens[b];
-            const int c = consumer(&p, (void *)r.cs, user);
+            int c;
+            {
+                TraceRange tc("epoch_consumer");
+                c = consumer(&p, (void *)r.cs, user);
+            }
             KKW_HIP(hipEventRecord(r.consumed[b], r.cs));
             if (c != 0) {
                 rc = fail(KKW_E_CANCELLED, "the epoch consumer stopped the run");
@@ -2023,6 +2041,7 @@ int kkw_paths_dump(const kkw_paths *p, const char *file, const char *mode, int w
     return guarded([&]() -> int {
         if (!p || !file || !mode) return fail(KKW_E_INVALID, "null argument");
         if (p->on_device) return fail(KKW_E_INVALID, "paths are on the device");
+        TraceRange tr("paths_dump");
         FILE *f = fopen(file, mode);
         if (!f) return fail(KKW_E_IO, std::string("cannot open ") + file);
         // PathSet::dump, include/path.hpp:85-109
@@ -2197,6 +2216,10 @@ int kkw_serve_start(kkw_engine *eng, const kkw_serve_opts *opts)
         if (S->opts.queue_capacity == 0) S->opts.queue_capacity = 1u << 16;
         KKW_HIP(hipSetDevice(eng->device));
         for (auto &s : S->streams) KKW_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        {
+            static const char *const names[4] = {"kkw serve 0", "kkw serve 1", "kkw serve 2", "kkw serve 3"};
+            for (int k = 0; k < 4; k++) roctxNameHipStream(names[k], S->streams[k]);
+        }
         // device arena: the admission budget (hbm_budget, else half the free HBM, at most 64 GiB)
         // (the graph store's kept blocks count as free: they go back to the
         // runtime when the arena needs the room)

@@ -1164,6 +1164,7 @@ void build_rmat(GraphStore &gs, int scale, int edge_factor, uint32_t thr_a, uint
 {
     if (scale < 1 || scale > 30 || edge_factor < 1)
         throw Error(-1, "rmat: scale must be in [1,30] and edge_factor >= 1");
+    TraceRange tr("graph_rmat");
     gs.release();
     building(gs, [&] {
         const uint32_t V = 1u << scale;
@@ -1226,6 +1227,7 @@ void build_from_edges(GraphStore &gs, uint32_t V, const uint32_t *h_src, const u
                       hipStream_t s)
 {
     if (V == 0) throw Error(-1, "v_num must be positive");
+    TraceRange tr("graph_from_edges");
     gs.release();
     building(gs, [&] {
         const uint64_t m = undirected ? 2 * n : n;
@@ -1289,6 +1291,7 @@ void build_from_csr(GraphStore &gs, uint32_t V, const uint64_t *h_row_ptr, const
 {
     if (V == 0) throw Error(-1, "v_num must be positive");
     if (h_row_ptr[0] != 0) throw Error(-1, "row_ptr[0] must be 0");
+    TraceRange tr("graph_from_csr");
     gs.release();
     building(gs, [&] {
         const uint64_t E = h_row_ptr[V];
@@ -1375,6 +1378,7 @@ void sort_adjacency(GraphStore &gs, hipStream_t s)
 {
     if (!gs.row_ptr) throw Error(-2, "no graph loaded");
     if (gs.sorted) return;
+    TraceRange tr("sort_adjacency");
     const uint64_t E = gs.E;
     if (E == 0) {
         gs.sorted = true;
@@ -1425,6 +1429,7 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
 {
     if (!gs.row_ptr) throw Error(-2, "no graph loaded");
     if (gs.adjset) return;
+    TraceRange tr("neighbour_sets");
     const uint64_t n = adj_table_start(gs.E, gs.V) + 4;   // every table ends by slot 4 (E + V)
     DArr<uint32_t> t(n);
     DArr<uint32_t> mid, big;    // freed after the stream is synchronised below
@@ -1498,6 +1503,7 @@ void build_uniform_buckets(GraphStore &gs, hipStream_t s)
 {
     if (!gs.row_ptr) throw Error(-2, "no graph loaded");
     if (gs.ubucket) return;
+    TraceRange tr("uniform_buckets");
     DArr<Bucket> t(gs.E);
     if (gs.E)
         hipLaunchKernelGGL(k_uniform_buckets, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.col, gs.etype, gs.vmask,
@@ -1511,6 +1517,7 @@ void build_alias(GraphStore &gs, hipStream_t s, bool local)
 {
     if (!gs.row_ptr) throw Error(-2, "no graph loaded");
     if (!gs.w) throw Error(-2, "graph has no weights: alias tables need a weighted graph");
+    TraceRange tr(local ? "alias_tables_local" : "alias_tables");
     Bucket *&target = local ? gs.lbucket : gs.bucket;
     dfree(target);
     DArr<Bucket> t(gs.E);

@@ -4,6 +4,7 @@
 #include "kkw_dev.h"
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <stdexcept>
 #include <string>
 
@@ -12,6 +13,17 @@ namespace kkw {
 struct Error : std::runtime_error {
     int code;
     Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+// A named span of host time on the calling thread (graph builds, table
+// builds, walk launches and waits, path output), recorded by `rocprofv3
+// --marker-trace` beside the kernels it encloses; without a profiler the two
+// calls return at once.
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
 };
 
 #define KKW_HIP(call)                                                                         \
