@@ -627,7 +627,16 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
 // entries go to the tails in pairing order (ballot again).  This is exactly the
 // sequential FIFO run k steps at a time.  Queue windows of AW_WIN entries live
 // in LDS; lists longer than that also keep their queues in global memory and
-// refill the windows from there.  The pairing works on local edge indices
+// refill the windows from there.  Lists of more than AW_POOL_MIN entries come
+// first in the list and take a whole block each: its one working wave has the
+// block's four windows as one of AW_POOL_WIN entries, so it refills a quarter
+// as often, and these lists start before the shorter ones.  The longest list
+// bounds the launch: on R-MAT-24 (largest list 406,605 entries) its wave
+// spends 3.1 ms on the sequential sum, 2.2 ms queueing and 4.7 ms pairing
+// (6,357 batches, 198 refills) of a 10.1-10.3 ms launch (10.8 ms with one
+// wave and 512-entry windows for every list); when the table builds run
+// together in kkw_prepare, the neighbour-set kernels fill the CUs this tail
+// leaves idle.  The pairing works on local edge indices
 // and reads global memory only to refill its windows (all loads before one
 // wait), so a batch costs its LDS round trip: each bucket is written with its
 // local indices and the list's first
@@ -636,6 +645,8 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
 // tables of walk programs keep the indices and add the ids).
 constexpr uint32_t AW_WIN = 512;
 constexpr int AW_WAVES = 4;
+constexpr uint32_t AW_POOL_WIN = AW_WIN * AW_WAVES;
+constexpr uint64_t AW_POOL_MIN = 4096;
 constexpr uint32_t AW_SUM_CHUNK = 256;
 constexpr uint32_t AW_Q_CHUNKS = 16;         // chunks of 64 weights per load group when queueing
 
@@ -672,26 +683,31 @@ __device__ __forceinline__ void finish_bucket(Bucket &bk, uint64_t b, const uint
     }
 }
 
+// list[0 .. n_pool): the lists of more than AW_POOL_MIN entries, one block each
 __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__restrict__ list, uint32_t n_list,
+                                                            uint32_t n_pool,
                                                             const uint64_t *__restrict__ row_ptr,
                                                             const float *__restrict__ w,
                                                             Bucket *__restrict__ out, float *__restrict__ vsum,
                                                             uint2 *__restrict__ qs, uint2 *__restrict__ ql)
 {
-    __shared__ uint2 win_s[AW_WAVES][AW_WIN], win_l[AW_WAVES][AW_WIN];
+    __shared__ uint2 win_s[AW_POOL_WIN], win_l[AW_POOL_WIN];
     __shared__ float stage[AW_WAVES][AW_SUM_CHUNK];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t idx = blockIdx.x * AW_WAVES + wave;
+    const bool pooled = blockIdx.x < n_pool;
+    if (pooled && wave != 0) return;   // the block's windows serve its first wave
+    const uint32_t idx = pooled ? blockIdx.x : n_pool + (blockIdx.x - n_pool) * AW_WAVES + wave;
     if (idx >= n_list) return;   // whole wave; nothing below synchronises across waves
+    const uint32_t WIN = pooled ? AW_POOL_WIN : AW_WIN;
     const uint32_t v = list[idx];
     const uint64_t b = row_ptr[v];
     const uint32_t d = (uint32_t)(row_ptr[v + 1] - b);
     const float *wv = w + b;
     uint2 *S = qs + b, *L = ql + b;
-    uint2 *sS = win_s[wave], *sL = win_l[wave];
+    uint2 *sS = win_s + (pooled ? 0u : wave * AW_WIN), *sL = win_l + (pooled ? 0u : wave * AW_WIN);
     float *sw = stage[wave];
     Bucket *o = out + b;
-    const bool spill = d > AW_WIN;   // queues also kept in global memory
+    const bool spill = d > WIN;   // queues also kept in global memory
     const uint64_t lt_mask = (1ull << lane) - 1ull;
 
     // 1. sequential float sum in edge order (walk.hpp:591-595)
@@ -738,12 +754,12 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
             if (small) {
                 const uint32_t pos = st + (uint32_t)__popcll(ms & lt_mask);
                 if (spill) S[pos] = ent;
-                if (pos < AW_WIN) sS[pos] = ent;
+                if (pos < WIN) sS[pos] = ent;
             }
             if (large) {
                 const uint32_t pos = lt + (uint32_t)__popcll(ml & lt_mask);
                 if (spill) L[pos] = ent;
-                if (pos < AW_WIN) sL[pos] = ent;
+                if (pos < WIN) sL[pos] = ent;
             }
             st += (uint32_t)__popcll(ms);
             lt += (uint32_t)__popcll(ml);
@@ -755,23 +771,26 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
     uint32_t sh = 0, lh = 0, ws = 0, wl = 0, ob = 0;
     while (sh != st) {
         const bool paired = lh != lt;
-        const uint32_t av_s = (st < ws + AW_WIN ? st : ws + AW_WIN) - sh;
-        const uint32_t av_l = paired ? (lt < wl + AW_WIN ? lt : wl + AW_WIN) - lh : 64u;
-        if ((av_s < 64 && st > ws + AW_WIN) || (av_l < 64 && paired && lt > wl + AW_WIN)) {
-            // refill both windows from the global queues at the current heads:
-            // all loads first, then the LDS writes (one wait per refill)
-            uint2 rs[AW_WIN / 64], rl[AW_WIN / 64];
+        const uint32_t av_s = (st < ws + WIN ? st : ws + WIN) - sh;
+        const uint32_t av_l = paired ? (lt < wl + WIN ? lt : wl + WIN) - lh : 64u;
+        if ((av_s < 64 && st > ws + WIN) || (av_l < 64 && paired && lt > wl + WIN)) {
+            // refill both windows from the global queues at the current heads
+            // (which hold every entry of a list that spills), AW_WIN entries
+            // at a time: all loads first, then the LDS writes (one wait each)
+            for (uint32_t c = 0; c < WIN; c += AW_WIN) {
+                uint2 rs[AW_WIN / 64], rl[AW_WIN / 64];
 #pragma unroll
-            for (uint32_t j = 0; j < AW_WIN / 64; j++) {
-                const uint32_t i = j * 64 + lane;
-                rs[j] = sh + i < st ? S[sh + i] : make_uint2(0, 0);
-                rl[j] = lh + i < lt ? L[lh + i] : make_uint2(0, 0);
-            }
+                for (uint32_t j = 0; j < AW_WIN / 64; j++) {
+                    const uint32_t i = c + j * 64 + lane;
+                    rs[j] = sh + i < st ? S[sh + i] : make_uint2(0, 0);
+                    rl[j] = lh + i < lt ? L[lh + i] : make_uint2(0, 0);
+                }
 #pragma unroll
-            for (uint32_t j = 0; j < AW_WIN / 64; j++) {
-                const uint32_t i = j * 64 + lane;
-                if (sh + i < st) sS[i] = rs[j];
-                if (lh + i < lt) sL[i] = rl[j];
+                for (uint32_t j = 0; j < AW_WIN / 64; j++) {
+                    const uint32_t i = c + j * 64 + lane;
+                    if (sh + i < st) sS[i] = rs[j];
+                    if (lh + i < lt) sL[i] = rl[j];
+                }
             }
             ws = sh;
             wl = lh;
@@ -811,12 +830,12 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
         if (to_s) {
             const uint32_t pos = st + (uint32_t)__popcll(ms & lt_mask);
             if (spill) S[pos] = ne;
-            if (pos < ws + AW_WIN) sS[pos - ws] = ne;
+            if (pos < ws + WIN) sS[pos - ws] = ne;
         }
         if (to_l) {
             const uint32_t pos = lt + (uint32_t)__popcll(ml & lt_mask);
             if (spill) L[pos] = ne;
-            if (pos < wl + AW_WIN) sL[pos - wl] = ne;
+            if (pos < wl + WIN) sL[pos - wl] = ne;
         }
         st += (uint32_t)__popcll(ms);
         lt += (uint32_t)__popcll(ml);
@@ -828,7 +847,7 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
     // 4. leftover large entries (walk.hpp:635-647), in queue order
     for (uint32_t i = lane; lh + i < lt; i += 64) {
         const uint32_t pos = lh + i;
-        const uint32_t e = pos < wl + AW_WIN ? sL[pos - wl].x : L[pos].x;
+        const uint32_t e = pos < wl + WIN ? sL[pos - wl].x : L[pos].x;
         Bucket bk;
         bk.p = 1.0f;
         bk.primary = e;
@@ -1534,18 +1553,24 @@ void build_alias(GraphStore &gs, hipStream_t s, bool local)
             hipLaunchKernelGGL(k_neighbour_rows, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.col, gs.E, gs.row_ptr,
                                nrow.p);
         }
-        // lists of AW_MIN or more entries: one wave each; the rest: one lane each
+        // lists of more than AW_POOL_MIN entries: one block each, listed
+        // first; the rest of AW_MIN or more: one wave each; shorter lists:
+        // one lane each
         DArr<uint32_t> big(gs.V);
-        DArr<unsigned int> n_big_d(1);
+        DArr<unsigned int> n_big_d(2);
         KKW_HIP(hipMemsetAsync(n_big_d.p, 0, sizeof(unsigned int), s));
         hipLaunchKernelGGL(k_list_range, dim3(grid_for(gs.V, LIST_BLOCK)), dim3(LIST_BLOCK), 0, s, gs.row_ptr, gs.V,
-                           (uint64_t)AW_MIN, ~0ull, big.p, n_big_d.p);
-        unsigned int n_big = 0;
-        KKW_HIP(hipMemcpyAsync(&n_big, n_big_d.p, sizeof(n_big), hipMemcpyDeviceToHost, s));
+                           AW_POOL_MIN + 1, ~0ull, big.p, n_big_d.p);
+        KKW_HIP(hipMemcpyAsync(n_big_d.p + 1, n_big_d.p, sizeof(unsigned int), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_list_range, dim3(grid_for(gs.V, LIST_BLOCK)), dim3(LIST_BLOCK), 0, s, gs.row_ptr, gs.V,
+                           (uint64_t)AW_MIN, AW_POOL_MIN, big.p, n_big_d.p);
+        unsigned int counts[2] = {0, 0};   // all listed, pooled (listed first)
+        KKW_HIP(hipMemcpyAsync(counts, n_big_d.p, sizeof(counts), hipMemcpyDeviceToHost, s));
         KKW_HIP(hipStreamSynchronize(s));
+        const unsigned int n_big = counts[0], n_pool = counts[1];
         if (n_big)
-            hipLaunchKernelGGL(k_alias_wave, dim3(grid_for(n_big, AW_WAVES)), dim3(64 * AW_WAVES), 0, s, big.p,
-                               n_big, gs.row_ptr, gs.w, t.p, vsum, qs.p, ql.p);
+            hipLaunchKernelGGL(k_alias_wave, dim3(n_pool + grid_for(n_big - n_pool, AW_WAVES)), dim3(64 * AW_WAVES), 0,
+                               s, big.p, n_big, n_pool, gs.row_ptr, gs.w, t.p, vsum, qs.p, ql.p);
         hipLaunchKernelGGL(k_alias, dim3(grid_for(gs.V, 64)), dim3(64), 0, s, gs.row_ptr, gs.w, gs.V, t.p, vsum);
         if (gs.E)
             hipLaunchKernelGGL(k_finish_buckets, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, t.p, gs.E, gs.col,

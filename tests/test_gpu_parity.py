@@ -173,12 +173,13 @@ def test_alias_matches_oracle(engine, scale):
 
 
 def _hub_graph(seed):
-    """CSR whose long lists cover the workgroup-per-vertex alias build: lists
-    just below, at and above 1024 entries and up to 300k, with weight shapes
-    that drive the small/large queues differently (uniform, heavy-tailed, all
-    equal, one dominant weight, one tiny weight)."""
+    """CSR whose long lists cover the wave-per-list alias build and its
+    block-per-list form for lists of more than 4096 entries: lists just below,
+    at and above 1024 and 4096 entries and up to 300k, with weight shapes that
+    drive the small/large queues differently (uniform, heavy-tailed, all equal,
+    one dominant weight, one tiny weight)."""
     rng = np.random.default_rng(seed)
-    degs = [3, 1023, 1024, 1025, 4096, 70000, 300000, 5000, 5000, 5000, 2048, 17]
+    degs = [3, 1023, 1024, 1025, 4096, 70000, 300000, 5000, 5000, 5000, 2048, 17, 4097, 2049]
     V = len(degs) + 64
     degs = degs + [0] * (V - len(degs))
     rp = np.zeros(V + 1, np.uint64)
