@@ -531,14 +531,25 @@ __global__ void __launch_bounds__(256) k_vmask(const uint64_t *row_ptr, const ui
 // Such a list never has more than AS_RING - 1 live queue entries, so each
 // lane keeps its two queues as rings in LDS (mass and local index apart,
 // rows padded against bank conflicts) and the pairing makes no global memory
-// round trip.  Buckets leave with local indices and the list's first edge in
-// prow, for k_finish_buckets.
+// round trip.  Buckets leave as AliasRec records (local indices), for
+// k_finish_buckets.
 constexpr uint32_t AS_RING = 32;
+
+// A bucket as the alias kernels leave it: the coin's threshold p, the local
+// indices of its primary and alias edges, and the vertex whose list it
+// belongs to.  Records are dense (16 bytes, where the finished bucket takes
+// 32): the alias kernels write and k_finish_buckets reads half the bytes that
+// whole buckets cost.
+struct __align__(16) AliasRec {
+    float p;
+    uint32_t primary, alias, v;
+};
+static_assert(sizeof(AliasRec) == 16, "alias record is 16 bytes");
 constexpr uint32_t AW_MIN = 32;   // lists of AW_MIN or more entries: k_alias_wave
 static_assert(AW_MIN <= AS_RING, "a short list's live queue entries must fit its ring");
 
 __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_ptr, const float *__restrict__ w,
-                                              uint32_t V, Bucket *__restrict__ out, float *__restrict__ vsum)
+                                              uint32_t V, AliasRec *__restrict__ out, float *__restrict__ vsum)
 {
     __shared__ float sp[64][AS_RING + 1], lp[64][AS_RING + 1];   // masses of the small / large rings
     __shared__ uint8_t si[64][AS_RING + 4], li[64][AS_RING + 4];  // their local edge indices
@@ -571,11 +582,9 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
             lt++;
         }
     }
-    Bucket *o = out + b;
-    Bucket bk;
-    bk.meta = 0;
-    bk.prow = b;
-    bk.arow = 0;
+    AliasRec *o = out + b;
+    AliasRec bk;
+    bk.v = (uint32_t)v;
     uint32_t sh = 0, lh = 0;
     while (sh != st) {
         const float ps = sp[lane][sh & M];
@@ -638,11 +647,11 @@ __global__ void __launch_bounds__(64) k_alias(const uint64_t *__restrict__ row_p
 // together in kkw_prepare, the neighbour-set kernels fill the CUs this tail
 // leaves idle.  The pairing works on local edge indices
 // and reads global memory only to refill its windows (all loads before one
-// wait), so a batch costs its LDS round trip: each bucket is written with its
-// local indices and the list's first
-// edge in prow, and k_finish_buckets then gives every bucket its neighbour
-// ids, rows and edge types in one edge-parallel pass (finish_bucket; the
-// tables of walk programs keep the indices and add the ids).
+// wait), so a batch costs its LDS round trip: each bucket leaves as an
+// AliasRec (local indices and the list's vertex), and k_finish_buckets then
+// gives every bucket its neighbour ids, rows and edge types in one
+// edge-parallel pass (finish_bucket; the tables of walk programs keep the
+// indices and add the ids).
 constexpr uint32_t AW_WIN = 512;
 constexpr int AW_WAVES = 4;
 constexpr uint32_t AW_POOL_WIN = AW_WIN * AW_WAVES;
@@ -688,7 +697,7 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
                                                             uint32_t n_pool,
                                                             const uint64_t *__restrict__ row_ptr,
                                                             const float *__restrict__ w,
-                                                            Bucket *__restrict__ out, float *__restrict__ vsum,
+                                                            AliasRec *__restrict__ out, float *__restrict__ vsum,
                                                             uint2 *__restrict__ qs, uint2 *__restrict__ ql)
 {
     __shared__ uint2 win_s[AW_POOL_WIN], win_l[AW_POOL_WIN];
@@ -706,7 +715,7 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
     uint2 *S = qs + b, *L = ql + b;
     uint2 *sS = win_s + (pooled ? 0u : wave * AW_WIN), *sL = win_l + (pooled ? 0u : wave * AW_WIN);
     float *sw = stage[wave];
-    Bucket *o = out + b;
+    AliasRec *o = out + b;
     const bool spill = d > WIN;   // queues also kept in global memory
     const uint64_t lt_mask = (1ull << lane) - 1ull;
 
@@ -800,10 +809,8 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
         uint32_t k = av_s < av_l ? av_s : av_l;
         k = k < 64 ? k : 64;
         const bool act = lane < k;
-        Bucket bk;
-        bk.meta = 0;
-        bk.prow = b;
-        bk.arow = 0;
+        AliasRec bk;
+        bk.v = v;
         bool to_s = false, to_l = false;
         uint2 ne = make_uint2(0, 0);
         if (act) {
@@ -848,38 +855,35 @@ __global__ void __launch_bounds__(64 * AW_WAVES) k_alias_wave(const uint32_t *__
     for (uint32_t i = lane; lh + i < lt; i += 64) {
         const uint32_t pos = lh + i;
         const uint32_t e = pos < wl + WIN ? sL[pos - wl].x : L[pos].x;
-        Bucket bk;
+        AliasRec bk;
         bk.p = 1.0f;
         bk.primary = e;
         bk.alias = e;
-        bk.meta = 0;
-        bk.prow = b;
-        bk.arow = 0;
+        bk.v = v;
         o[ob + i] = bk;
     }
 }
 
 typedef uint32_t rows_u32x4 __attribute__((ext_vector_type(4), aligned(4)));
 
-// Every bucket as the alias kernels left it (local indices, prow = the list's
-// first edge) gets its neighbour ids, rows and edge types (finish_bucket), one
-// lane per bucket.
-__global__ void k_finish_buckets(Bucket *__restrict__ out, uint64_t E, const uint32_t *__restrict__ col,
+// Every bucket as the alias kernels left it (an AliasRec: local indices and
+// the list's vertex) gets its neighbour ids, rows and edge types
+// (finish_bucket), one lane per bucket.
+__global__ void k_finish_buckets(const AliasRec *__restrict__ rec, Bucket *__restrict__ out, uint64_t E,
+                                 const uint64_t *__restrict__ row_ptr, const uint32_t *__restrict__ col,
                                  const uint8_t *__restrict__ etype, bool to_ids, const uint64_t *__restrict__ nrow,
                                  const uint8_t *__restrict__ vmask)
 {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= E) return;
-    uint32_t *bw = reinterpret_cast<uint32_t *>(out + k);
-    const rows_u32x4 h0 = *reinterpret_cast<const rows_u32x4 *>(bw);       // {p, primary, prow = list start}
-    const rows_u32x4 h1 = *reinterpret_cast<const rows_u32x4 *>(bw + 4);   // {alias, -, -}
+    const AliasRec r = rec[k];
     Bucket bk;
-    bk.p = __uint_as_float(h0.x);
-    bk.primary = h0.y;
-    bk.alias = h1.x;
+    bk.p = r.p;
+    bk.primary = r.primary;
+    bk.alias = r.alias;
     bk.prow = 0;
     bk.arow = 0;
-    finish_bucket(bk, (uint64_t)h0.z | ((uint64_t)h0.w << 32), col, etype, to_ids, nrow, vmask);
+    finish_bucket(bk, row_ptr[r.v], col, etype, to_ids, nrow, vmask);
     out[k] = bk;
 }
 
@@ -1540,6 +1544,7 @@ void build_alias(GraphStore &gs, hipStream_t s, bool local)
     Bucket *&target = local ? gs.lbucket : gs.bucket;
     dfree(target);
     DArr<Bucket> t(gs.E);
+    DArr<AliasRec> rec(gs.E);   // freed (kept by the device pool) when the build ends
     // both tables give the same sums; keep the array (and its address) if present
     DArr<float> vs;
     float *vsum = gs.vsum;
@@ -1570,11 +1575,11 @@ void build_alias(GraphStore &gs, hipStream_t s, bool local)
         const unsigned int n_big = counts[0], n_pool = counts[1];
         if (n_big)
             hipLaunchKernelGGL(k_alias_wave, dim3(n_pool + grid_for(n_big - n_pool, AW_WAVES)), dim3(64 * AW_WAVES), 0,
-                               s, big.p, n_big, n_pool, gs.row_ptr, gs.w, t.p, vsum, qs.p, ql.p);
-        hipLaunchKernelGGL(k_alias, dim3(grid_for(gs.V, 64)), dim3(64), 0, s, gs.row_ptr, gs.w, gs.V, t.p, vsum);
+                               s, big.p, n_big, n_pool, gs.row_ptr, gs.w, rec.p, vsum, qs.p, ql.p);
+        hipLaunchKernelGGL(k_alias, dim3(grid_for(gs.V, 64)), dim3(64), 0, s, gs.row_ptr, gs.w, gs.V, rec.p, vsum);
         if (gs.E)
-            hipLaunchKernelGGL(k_finish_buckets, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, t.p, gs.E, gs.col,
-                               gs.etype, !local, nrow.p, gs.vmask);
+            hipLaunchKernelGGL(k_finish_buckets, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, rec.p, t.p, gs.E,
+                               gs.row_ptr, gs.col, gs.etype, !local, nrow.p, gs.vmask);
         KKW_HIP(hipGetLastError());
         KKW_HIP(hipStreamSynchronize(s));
     }
