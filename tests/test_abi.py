@@ -28,6 +28,19 @@ def test_library_exports_every_declared_symbol():
     assert lib.kkw_abi_version() == 3
 
 
+def test_library_marks_its_ranges_for_the_profiler():
+    """The marker-trace ranges are compiled in: libkkw imports the roctx range
+    calls and carries the names of its ranges."""
+    import subprocess
+    so = os.path.join(ROOT, "knightking_amd", "libkkw.so")
+    und = subprocess.run(["nm", "-D", "--undefined-only", so], capture_output=True, text=True, check=True).stdout
+    for sym in ("roctxRangePushA", "roctxRangePop", "roctxRangeStartA", "roctxRangeStop"):
+        assert re.search(r"\b" + sym + r"\b", und), sym
+    blob = open(so, "rb").read()
+    for name in (b"alias_tables", b"neighbour_sets", b"walk_launch", b"walk_wait", b"serve_flight"):
+        assert name + b"\0" in blob, name
+
+
 def test_no_gpu_means_no_engine():
     """Without a GPU the engine cannot be created (no CPU fallback).  torch is
     kept out of this process: its bundled HIP runtime beside libkkw's is what
