@@ -86,19 +86,30 @@ void pool_free(void *p)
 {
     if (!p) return;
     Pool &P = pool();
-    std::lock_guard<std::mutex> lk(P.mu);
-    auto it = P.owned.find(p);
-    if (it == P.owned.end()) {   // not the pool's: hand it back directly
-        (void)hipFree(p);
-        return;
+    int dev = 0;
+    size_t bytes = 0;
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        auto it = P.owned.find(p);
+        if (it == P.owned.end()) {   // not the pool's: hand it back directly
+            (void)hipFree(p);
+            return;
+        }
+        dev = it->second.first;
+        bytes = it->second.second;
     }
-    const int dev = it->second.first;
+    // No queued work may still use the block when it is handed out again.  The
+    // wait happens outside the lock: a thread waiting here for the device (the
+    // two table builds of kkw_prepare run on two threads) must not hold up the
+    // other thread's allocations; the block is not in the kept list until the
+    // wait is over, so nobody can take it early.
     int cur = 0;
     (void)hipGetDevice(&cur);
     if (cur != dev) (void)hipSetDevice(dev);
-    (void)hipDeviceSynchronize();   // no queued work may still use the block when it is handed out again
+    (void)hipDeviceSynchronize();
     if (cur != dev) (void)hipSetDevice(cur);
-    P.kept[dev].emplace(it->second.second, p);
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.kept[dev].emplace(bytes, p);
 }
 
 void pool_attach(int device)
@@ -419,8 +430,9 @@ __global__ void k_check_symmetric(const uint64_t *row_ptr, const uint32_t *col, 
 // random line of HBM -- and longer lists in segments of ADJ_SEG slots, one
 // workgroup per segment in LDS (k_adjset_seg), the few keys whose probe runs
 // past a segment's end inserted afterwards with global atomics
-// (k_adjset_overflow).  Empty lists get four empty slots from
-// k_adjset_small.
+// (k_adjset_overflow).  Empty lists and lists of at most ADJ_TINY_MAX_DEG
+// entries take one lane each (k_adjset_tiny).
+constexpr uint32_t ADJ_TINY_MAX_DEG = 7;     // table of at most 4 * 7 + 4 = 32 slots
 constexpr uint32_t ADJ_LDS_MAX_DEG = 255;    // table of at most 4 * 255 + 4 = 1024 slots
 constexpr uint32_t ADJ_MID_MAX_DEG = 4095;   // table of at most 16384 slots (64 KiB of LDS)
 constexpr uint32_t ADJ_SEG = 16384;          // slots of one LDS segment of a longer list's table
@@ -887,33 +899,71 @@ __global__ void k_finish_buckets(const AliasRec *__restrict__ rec, Bucket *__res
     out[k] = bk;
 }
 
-// Neighbour hash sets of lists with 1..ADJ_LDS_MAX_DEG entries: one wave per
-// vertex (vertices strided over the grid's waves) builds the table in LDS with
-// LDS atomics and writes it out whole.
-__global__ void __launch_bounds__(256) k_adjset_small(const uint64_t *row_ptr, const uint32_t *col, uint32_t V,
-                                                      uint32_t *t)
+// Neighbour hash sets of empty lists and lists of at most ADJ_TINY_MAX_DEG
+// entries: one lane per vertex.  The lane inserts its keys in list order into
+// its own slice of LDS (linear probing from each key's home slot, as the
+// probes expect; no other lane writes there) and writes the table out in
+// 16-byte groups.  Consecutive vertices' tables are consecutive in memory, so
+// a wave's row reads and table writes are nearly contiguous.  (With one wave
+// per vertex, each such list was a chain of dependent row, key and LDS round
+// trips: R-MAT-24's 15 M lists under 32 entries, 6.8 M of them empty, held
+// that kernel at 3.9 ms.)
+__global__ void __launch_bounds__(256) k_adjset_tiny(const uint64_t *__restrict__ row_ptr,
+                                                     const uint32_t *__restrict__ col, uint32_t V,
+                                                     uint32_t *__restrict__ t)
+{
+    constexpr uint32_t SLICE = 4 * ADJ_TINY_MAX_DEG + 4 + 1;   // odd stride: lanes hit different banks
+    __shared__ uint32_t tabs[256 * SLICE];
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const uint64_t b = row_ptr[v], e = row_ptr[v + 1];
+    uint4 *gt = reinterpret_cast<uint4 *>(t + adj_table_start(b, v));
+    if (e == b) {   // an empty list's table: four empty slots
+        gt[0] = make_uint4(ADJ_EMPTY, ADJ_EMPTY, ADJ_EMPTY, ADJ_EMPTY);
+        return;
+    }
+    if (e - b > ADJ_TINY_MAX_DEG) return;
+    const uint32_t d = (uint32_t)(e - b), size = adj_table_size(d);
+    uint32_t keys[ADJ_TINY_MAX_DEG];
+#pragma unroll
+    for (uint32_t k = 0; k < ADJ_TINY_MAX_DEG; k++) keys[k] = k < d ? col[b + k] : 0u;
+    uint32_t *lt = tabs + threadIdx.x * SLICE;
+    for (uint32_t i = 0; i < size; i++) lt[i] = ADJ_EMPTY;
+#pragma unroll
+    for (uint32_t k = 0; k < ADJ_TINY_MAX_DEG; k++) {
+        if (k < d) {   // d < size: the probe finds a free slot
+            uint32_t slot = adj_slot(keys[k], size);
+            while (lt[slot] != ADJ_EMPTY && lt[slot] != keys[k]) slot = slot + 1 == size ? 0 : slot + 1;
+            lt[slot] = keys[k];
+        }
+    }
+    for (uint32_t g = 0; g < size / 4; g++) gt[g] = make_uint4(lt[4 * g], lt[4 * g + 1], lt[4 * g + 2], lt[4 * g + 3]);
+}
+
+// Neighbour hash sets of lists with ADJ_TINY_MAX_DEG + 1 .. ADJ_LDS_MAX_DEG
+// entries (list ids from k_list_range, *n of them): one wave per list, lists
+// strided over the grid's waves, builds the table in LDS with LDS atomics and
+// writes it out whole.
+__global__ void __launch_bounds__(256) k_adjset_small(const uint32_t *__restrict__ list, const unsigned int *n,
+                                                      const uint64_t *__restrict__ row_ptr,
+                                                      const uint32_t *__restrict__ col, uint32_t *__restrict__ t)
 {
     __shared__ uint32_t tabs[4][4 * ADJ_LDS_MAX_DEG + 4];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t *lt = tabs[wave];
-    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
-    for (uint64_t v = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave; v < V; v += waves) {   // wave-uniform
+    const uint32_t count = *n;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + wave; i < count; i += waves) {   // wave-uniform
+        const uint32_t v = list[i];
         const uint64_t b = row_ptr[v], e = row_ptr[v + 1];
-        if (e == b) {   // an empty list's table: four empty slots
-            if (lane == 0)
-                *reinterpret_cast<uint4 *>(t + adj_table_start(b, v)) = make_uint4(ADJ_EMPTY, ADJ_EMPTY, ADJ_EMPTY,
-                                                                                   ADJ_EMPTY);
-            continue;
-        }
-        if (e - b > ADJ_LDS_MAX_DEG) continue;
         const uint32_t size = adj_table_size(e - b);
-        for (uint32_t i = lane; i < size; i += 64) lt[i] = ADJ_EMPTY;
+        for (uint32_t j = lane; j < size; j += 64) lt[j] = ADJ_EMPTY;
         wave_sync();
         for (uint64_t k = b + lane; k < e; k += 64) adj_insert(lt, size, col[k]);
         wave_sync();
         uint32_t *gt = t + adj_table_start(b, v);
-        for (uint32_t i = lane; i < size; i += 64) gt[i] = lt[i];
-        wave_sync();   // the next vertex reuses the LDS table
+        for (uint32_t j = lane; j < size; j += 64) gt[j] = lt[j];
+        wave_sync();   // the next list reuses the LDS table
     }
 }
 
@@ -1455,14 +1505,20 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
     TraceRange tr("neighbour_sets");
     const uint64_t n = adj_table_start(gs.E, gs.V) + 4;   // every table ends by slot 4 (E + V)
     DArr<uint32_t> t(n);
-    DArr<uint32_t> mid, big;    // freed after the stream is synchronised below
-    DArr<unsigned int> n_mid, n_big, n_ovf;
+    DArr<uint32_t> small, mid, big;    // freed after the stream is synchronised below
+    DArr<unsigned int> n_small, n_mid, n_big, n_ovf;
     DArr<uint2> items_d, ovf;
-    if (gs.V) {   // lists up to ADJ_LDS_MAX_DEG entries, and the empty lists' tables
-        const uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)gs.V + 3) / 4, 1u << 16);   // 4 waves each
-        hipLaunchKernelGGL(k_adjset_small, dim3(blocks), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V, t.p);
-    }
+    if (gs.V)   // empty lists and lists up to ADJ_TINY_MAX_DEG entries, one lane each
+        hipLaunchKernelGGL(k_adjset_tiny, dim3(grid_for(gs.V, 256)), dim3(256), 0, s, gs.row_ptr, gs.col, gs.V, t.p);
     if (gs.E) {
+        // lists up to ADJ_LDS_MAX_DEG entries: one wave each, the grid striding over them
+        small.p = dalloc<uint32_t>(gs.V);
+        n_small.p = dalloc<unsigned int>(1);
+        KKW_HIP(hipMemsetAsync(n_small.p, 0, sizeof(unsigned int), s));
+        hipLaunchKernelGGL(k_list_range, dim3(grid_for(gs.V, LIST_BLOCK)), dim3(LIST_BLOCK), 0, s, gs.row_ptr, gs.V,
+                           (uint64_t)ADJ_TINY_MAX_DEG + 1, (uint64_t)ADJ_LDS_MAX_DEG, small.p, n_small.p);
+        hipLaunchKernelGGL(k_adjset_small, dim3((uint32_t)std::min<uint64_t>(((uint64_t)gs.V + 3) / 4, 1u << 16)),
+                           dim3(256), 0, s, small.p, n_small.p, gs.row_ptr, gs.col, t.p);
         mid.p = dalloc<uint32_t>(gs.V);
         n_mid.p = dalloc<unsigned int>(1);
         KKW_HIP(hipMemsetAsync(n_mid.p, 0, sizeof(unsigned int), s));
