@@ -1002,7 +1002,8 @@ __global__ void __launch_bounds__(BLOCK) k_adjset_mid(const uint32_t *__restrict
 // at multiples of ADJ_SEG slots of 16-byte aligned tables, and table sizes
 // are multiples of four).  A key whose probe runs past the segment's end goes
 // to ovf (cap entries) for k_adjset_overflow.
-__global__ void __launch_bounds__(256) k_adjset_seg(const uint2 *__restrict__ items, uint32_t n_items,
+__global__ void __launch_bounds__(256) k_adjset_seg(const uint2 *__restrict__ items,
+                                                    const uint32_t *__restrict__ n_items_d,
                                                     const uint32_t *__restrict__ list,
                                                     const uint64_t *__restrict__ row_ptr,
                                                     const uint32_t *__restrict__ col, uint32_t *__restrict__ t,
@@ -1010,6 +1011,7 @@ __global__ void __launch_bounds__(256) k_adjset_seg(const uint2 *__restrict__ it
 {
     __shared__ uint4 tab4[ADJ_SEG / 4];
     uint32_t *tab = reinterpret_cast<uint32_t *>(tab4);
+    const uint32_t n_items = *n_items_d;
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {   // workgroup-uniform
         const uint2 item = items[it];   // {index into list, segment}
         const uint32_t v = list[item.x];
@@ -1040,12 +1042,29 @@ __global__ void __launch_bounds__(256) k_adjset_seg(const uint2 *__restrict__ it
     }
 }
 
-// deg[i] = degree of vertex list[i]
-__global__ void k_list_degrees(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ row_ptr,
-                               uint32_t *__restrict__ deg)
+// Work items of k_adjset_seg, made on the device: segs[i] = the number of
+// ADJ_SEG-slot segments of list[i]'s table (segs[n] = 0, for the scan), then,
+// with seg_off its exclusive prefix sum, item (i, g) for every segment g.  (A
+// host loop over the degrees had the first build of a process wait 5.4 ms for
+// its first pageable device-to-host copy.)
+__global__ void k_list_segments(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ row_ptr,
+                                uint32_t *__restrict__ segs)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) deg[i] = (uint32_t)(row_ptr[list[i] + 1] - row_ptr[list[i]]);
+    if (i < n) {
+        const uint64_t d = row_ptr[list[i] + 1] - row_ptr[list[i]];
+        segs[i] = (uint32_t)(((uint64_t)adj_table_size(d) + ADJ_SEG - 1) / ADJ_SEG);
+    } else if (i == n) {
+        segs[n] = 0;
+    }
+}
+
+__global__ void k_segment_items(const uint32_t *__restrict__ segs, const uint32_t *__restrict__ seg_off, uint32_t n,
+                                uint2 *__restrict__ items)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        for (uint32_t g = 0; g < segs[i]; g++) items[seg_off[i] + g] = make_uint2(i, g);
 }
 
 // ids of the vertices whose lists hold lo .. hi entries, in no particular order.
@@ -1537,37 +1556,35 @@ void build_adj_index(GraphStore &gs, hipStream_t s)
         KKW_HIP(hipMemcpyAsync(&nb, n_big.p, sizeof(nb), hipMemcpyDeviceToHost, s));
         KKW_HIP(hipStreamSynchronize(s));
         if (nb) {
-            DArr<uint32_t> deg_d(nb);
-            hipLaunchKernelGGL(k_list_degrees, dim3(grid_for(nb, 256)), dim3(256), 0, s, big.p, nb, gs.row_ptr,
-                               deg_d.p);
-            std::vector<uint32_t> deg(nb);
-            KKW_HIP(hipMemcpyAsync(deg.data(), deg_d.p, (size_t)nb * 4, hipMemcpyDeviceToHost, s));
-            KKW_HIP(hipStreamSynchronize(s));
-            std::vector<uint2> items;
-            uint64_t keys = 0;
-            for (uint32_t i = 0; i < nb; i++) {
-                const uint32_t segs = (uint32_t)((adj_table_size(deg[i]) + ADJ_SEG - 1) / ADJ_SEG);
-                for (uint32_t g = 0; g < segs; g++) items.push_back(make_uint2(i, g));
-                keys += deg[i];
-            }
+            // the segments of every long list as work items, counted and listed on the device
+            DArr<uint32_t> segs(nb + 1), seg_off(nb + 1);
+            hipLaunchKernelGGL(k_list_segments, dim3(grid_for(nb + 1, 256)), dim3(256), 0, s, big.p, nb, gs.row_ptr,
+                               segs.p);
+            size_t scan_bytes = 0;
+            KKW_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, segs.p, seg_off.p, nb + 1, s));
+            DArr<unsigned char> scan_tmp(scan_bytes ? scan_bytes : 1);
+            KKW_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, scan_bytes, segs.p, seg_off.p, nb + 1, s));
+            // at most (4 E + 4 nb) / ADJ_SEG segments plus one partial segment per list
+            const uint64_t max_items = (4 * gs.E + 4 * (uint64_t)nb + ADJ_SEG - 1) / ADJ_SEG + nb;
+            items_d.p = dalloc<uint2>(max_items);
+            hipLaunchKernelGGL(k_segment_items, dim3(grid_for(nb, 256)), dim3(256), 0, s, segs.p, seg_off.p, nb,
+                               items_d.p);
             // a few keys per table run past a segment; KKW_ADJ_OVERFLOW_CAP (tests) shrinks the buffer
-            uint64_t cap = std::min<uint64_t>(keys, ADJ_OVERFLOW_CAP);
+            uint64_t cap = std::min<uint64_t>(gs.E, ADJ_OVERFLOW_CAP);
             if (const char *env = getenv("KKW_ADJ_OVERFLOW_CAP"))
                 cap = std::min<uint64_t>(cap, strtoull(env, nullptr, 10));
-            items_d.p = dalloc<uint2>(items.size());
-            ovf.p = dalloc<uint2>(cap);
+            ovf.p = dalloc<uint2>(cap ? cap : 1);
             n_ovf.p = dalloc<unsigned int>(1);
-            KKW_HIP(hipMemcpyAsync(items_d.p, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice, s));
             KKW_HIP(hipMemsetAsync(n_ovf.p, 0, sizeof(unsigned int), s));
-            // two 64 KiB workgroups fit a CU's LDS; the grid strides over the items
-            hipLaunchKernelGGL(k_adjset_seg, dim3((unsigned)std::min<size_t>(items.size(), 8192)), dim3(256), 0, s,
-                               items_d.p, (uint32_t)items.size(), big.p, gs.row_ptr, gs.col, t.p, ovf.p, (uint32_t)cap,
+            // two 64 KiB workgroups fit a CU's LDS; the grid strides over the items (their count is seg_off[nb])
+            hipLaunchKernelGGL(k_adjset_seg, dim3((unsigned)std::min<uint64_t>(max_items, 8192)), dim3(256), 0, s,
+                               items_d.p, seg_off.p + nb, big.p, gs.row_ptr, gs.col, t.p, ovf.p, (uint32_t)cap,
                                n_ovf.p);
             hipLaunchKernelGGL(k_adjset_overflow, dim3(1024), dim3(256), 0, s, ovf.p, n_ovf.p, (uint32_t)cap,
                                gs.row_ptr, t.p);
             unsigned int no = 0;
             KKW_HIP(hipMemcpyAsync(&no, n_ovf.p, sizeof(no), hipMemcpyDeviceToHost, s));
-            KKW_HIP(hipStreamSynchronize(s));   // also before the host vectors and deg_d go
+            KKW_HIP(hipStreamSynchronize(s));   // also before the segment and scan buffers go
             if (no > cap)   // the buffer could not hold them all: insert every key of the long lists
                 hipLaunchKernelGGL(k_adjset_insert_long, dim3(grid_for(gs.E, 256)), dim3(256), 0, s, gs.row_ptr,
                                    gs.col, gs.V, gs.E, t.p);
