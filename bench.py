@@ -371,7 +371,7 @@ def serve_stream(eng, jobs, gaps):
     return np.array(lat), steps[0], time.perf_counter() - t_start, (failure[0] if failure else None)
 
 
-def serve_session(eng, jobs, gaps, round_steps=0, groups=None):
+def serve_session(eng, jobs, gaps, round_steps=0, groups=None, idle_poll_us=0):
     """One serving session over a query stream, with the loop's flight trace:
     returns the latency / throughput summary and the flights' occupancy
     (and p50/p99 of each named group of queries: name -> boolean mask)."""
@@ -380,7 +380,7 @@ def serve_session(eng, jobs, gaps, round_steps=0, groups=None):
     os.close(fd)
     os.environ["KKW_SERVE_TRACE"] = trace
     try:
-        eng.serve_start(max_walkers_in_flight=1 << 22, round_steps=round_steps)
+        eng.serve_start(max_walkers_in_flight=1 << 22, round_steps=round_steps, idle_poll_us=idle_poll_us)
         try:
             lat, steps, span, err = serve_stream(eng, jobs, gaps)
         finally:
@@ -402,23 +402,24 @@ def serve_session(eng, jobs, gaps, round_steps=0, groups=None):
         os.unlink(trace)
 
 
-def serve_latency(eng, rp, rates, seconds, seed=7):
+def serve_latency(eng, rp, rates, seconds, seed=7, idle_poll_us=0):
     """Poisson PPR queries (t=0.15, 1e3..1e5 walkers log-uniform, source =
     source of a uniformly drawn directed edge) at each rate; latency from
-    submit to the query's last path on the host."""
+    submit to the query's last path on the host.  idle_poll_us: the serving
+    loop's idle polling window (0: it sleeps until a query arrives)."""
     import knightking_amd as kk
-    rng = np.random.default_rng(seed)
     E = int(rp[-1])
     out = {}
     for rate in rates:
+        rng = np.random.default_rng((seed, rate))   # one stream per rate, the same with and without polling
         n_q = int(rate * seconds)
         gaps = rng.exponential(1.0 / rate, n_q)
         srcs = np.searchsorted(rp, rng.integers(0, E, n_q), side="right") - 1
         walkers = np.round(10 ** (3 + 2 * rng.random(n_q))).astype(np.int64)
         jobs = [kk.ppr_job(int(walkers[i]), 0.15, True, seed=1000 + i,
                            start_vertices=np.array([srcs[i]], np.uint32)) for i in range(n_q)]
-        out[str(rate)] = serve_session(eng, jobs, gaps)
-        log(f"serving {rate} qps: {out[str(rate)]}")
+        out[str(rate)] = serve_session(eng, jobs, gaps, idle_poll_us=idle_poll_us)
+        log(f"serving {rate} qps{' (idle polling %d us)' % idle_poll_us if idle_poll_us else ''}: {out[str(rate)]}")
         if "error" in out[str(rate)]:
             break
     return out
@@ -671,6 +672,9 @@ def main():
     if not args.no_serve:
         rp, _, _, _ = eng.download(weights=False)
         serving = serve_latency(eng, rp, [200, 500, 1000, 2000], args.serve_seconds)
+        # the same streams with the idle loop polling for 10 ms after its last
+        # flight (it then admits an arrival without a wake-up)
+        serving["idle_poll_10ms"] = serve_latency(eng, rp, [200, 2000], args.serve_seconds, idle_poll_us=10000)
         serving["mixed"] = [serve_mixed(eng, rp, r, args.serve_seconds, round_steps=k)
                             for r, k in ((2000, 0), (2000, 8), (8000, 0), (8000, 8))]
 

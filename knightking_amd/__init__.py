@@ -82,7 +82,7 @@ class _Timing(C.Structure):
 
 class _ServeOpts(C.Structure):
     _fields_ = [("max_walkers_in_flight", C.c_uint32), ("round_steps", C.c_uint32),
-                ("queue_capacity", C.c_uint32), ("reserved", C.c_uint32)]
+                ("queue_capacity", C.c_uint32), ("idle_poll_us", C.c_uint32)]
 
 
 # kkw_epoch_consumer: (paths, stream, user) -> 0 to go on
@@ -706,10 +706,12 @@ class Engine:
         return r.value
 
     # ---- serving ---------------------------------------------------------
-    def serve_start(self, max_walkers_in_flight=1 << 22, queue_capacity=1 << 16, round_steps=0):
+    def serve_start(self, max_walkers_in_flight=1 << 22, queue_capacity=1 << 16, round_steps=0, idle_poll_us=0):
         """Start the serving loop; round_steps > 0 walks each flight in kernel
-        launches of at most that many steps per walker (same paths)."""
-        o = _ServeOpts(max_walkers_in_flight, round_steps, queue_capacity, 0)
+        launches of at most that many steps per walker (same paths);
+        idle_poll_us > 0 keeps the idle loop polling for new queries that long
+        before it sleeps."""
+        o = _ServeOpts(max_walkers_in_flight, round_steps, queue_capacity, idle_poll_us)
         _check(_lib.kkw_serve_start(self.h, C.byref(o)))
 
     def submit(self, job: Job) -> int:

@@ -1049,7 +1049,9 @@ static void serve_loop(ServeState *S)
     std::list<Flight> flights;
     bool stream_busy[4] = {false, false, false, false};
     uint64_t walkers_in_flight = 0;
+    double last_active = now_ms();   // last time a flight ran or a query was admitted
     for (;;) {
+        if (!flights.empty()) last_active = now_ms();
         bool progressed = false;
         // 1. advance flights: queue their next rounds, retire the ones whose
         //    copies have landed
@@ -1175,6 +1177,15 @@ static void serve_loop(ServeState *S)
             }
             if (admit.empty() && flights.empty() && !pin_blocked) {
                 if (S->stop && S->pending.empty()) break;
+                // idle: with idle_poll_us, keep looking for new queries that
+                // long after the last activity (an arrival is then admitted
+                // microseconds later instead of after a condition-variable
+                // wake-up), then sleep until kkw_submit signals
+                if (S->opts.idle_poll_us && now_ms() - last_active < 1e-3 * S->opts.idle_poll_us) {
+                    lk.unlock();
+                    std::this_thread::sleep_for(std::chrono::microseconds(2));
+                    continue;
+                }
                 S->cv_work.wait_for(lk, std::chrono::milliseconds(20));
                 continue;
             }

@@ -91,6 +91,30 @@ def test_results_same_alone_and_in_the_mix(served):
         assert same(got, alone[i]), f"query {i} differs in the mix"
 
 
+def test_idle_polling_serves_the_same_paths(served):
+    """With idle_poll_us the idle loop keeps looking for queries instead of
+    sleeping: queries spaced so that the loop goes idle before each (some
+    inside its polling window, one after it) come back equal to the oracle's
+    walks, and the loop stops while it polls."""
+    import time
+    from helpers import assert_rows_match_oracle, oracle_walk
+    from oracle import oracle as O
+    eng = served
+    V, _ = eng.info()
+    rp, col, w, et = eng.download(etype=True)
+    g = O.Graph(rp, col, w, et)
+    jobs = mixed_queries(V, 8, 5)
+    eng.serve_start(max_walkers_in_flight=20000, idle_poll_us=3000)
+    for i, job in enumerate(jobs):
+        t = eng.submit(job)
+        got, lat = eng.wait(t, 60.0)
+        assert lat >= 0
+        op, ol, ot, _ = oracle_walk(g, job)
+        assert_rows_match_oracle(got, op, ol, ot, what=f"query {i} served with idle polling")
+        time.sleep(0.005 if i == 3 else 0.001)   # once past the 3 ms window, else inside it
+    eng.serve_stop()
+
+
 def test_budget_smaller_than_request_fails_it(engine):
     eng = kk.Engine(hbm_budget=1 << 20)
     try:
